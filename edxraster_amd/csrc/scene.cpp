@@ -1,0 +1,254 @@
+// scene.cpp -- seeded synthetic meshes and camera matrices (host code).
+//
+// Stand-ins for Mesh::LoadSphere / Mesh::LoadPlane (EDXRaster/Utils/Mesh.cpp:36-70),
+// which delegate to EDXUtil's ObjMesh (not available).  Geometry is computed in
+// double and rounded once to fp32; random numbers come from SplitMix64 so every
+// consumer (HIP pipeline, CPU oracle, tests, benchmark) sees identical bytes.
+// Winding: front faces are clockwise on screen (det > 0 in
+// RasterTriangle::Setup, EDXRaster/Core/RasterTriangle.h:49-51).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/edxraster.h"
+
+namespace {
+
+inline uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double u01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+inline float u01f(uint64_t& s) { return (float)((splitmix64(s) >> 40) * (1.0 / 16777216.0)); }
+
+bool alloc_mesh(edxr_host_mesh* m, uint64_t V, uint64_t T) {
+    memset(m, 0, sizeof(*m));
+    if (V > 0xFFFFFFFFull || T > 0x55555555ull) return false;
+    float** planes[9] = {&m->px, &m->py, &m->pz, &m->nx, &m->ny, &m->nz, &m->cr, &m->cg, &m->cb};
+    for (auto p : planes) {
+        *p = (float*)malloc(sizeof(float) * (V ? V : 1));
+        if (!*p) return false;
+    }
+    m->indices = (uint32_t*)malloc(sizeof(uint32_t) * 3 * (T ? T : 1));
+    if (!m->indices) return false;
+    m->num_vertices = (uint32_t)V;
+    m->num_triangles = (uint32_t)T;
+    return true;
+}
+
+void fill_colors(edxr_host_mesh* m, uint64_t seed) {
+    uint64_t s = seed;
+    for (uint32_t i = 0; i < m->num_vertices; i++) {
+        m->cr[i] = u01f(s);
+        m->cg[i] = u01f(s);
+        m->cb[i] = u01f(s);
+    }
+}
+
+// Lattice value in [-1, 1] for octave o at integer point (ix, iz).
+inline double lattice(uint64_t seed, int o, int64_t ix, int64_t iz) {
+    uint64_t s = seed ^ (0x632BE59BD9B4E019ull * (uint64_t)(o + 1)) ^ (0x8CB92BA72F3D8DD7ull * (uint64_t)ix) ^
+                 (0xD6E8FEB86659FD93ull * (uint64_t)iz);
+    splitmix64(s);
+    return u01(s) * 2.0 - 1.0;
+}
+inline double smooth(double t) { return t * t * (3.0 - 2.0 * t); }
+
+double value_noise(uint64_t seed, int o, double x, double z) {
+    double fx = std::floor(x), fz = std::floor(z);
+    int64_t ix = (int64_t)fx, iz = (int64_t)fz;
+    double tx = smooth(x - fx), tz = smooth(z - fz);
+    double a = lattice(seed, o, ix, iz), b = lattice(seed, o, ix + 1, iz);
+    double c = lattice(seed, o, ix, iz + 1), d = lattice(seed, o, ix + 1, iz + 1);
+    double ab = a + (b - a) * tx, cd = c + (d - c) * tx;
+    return ab + (cd - ab) * tz;
+}
+
+void build_sphere(edxr_host_mesh* m, uint32_t voff, uint32_t toff, uint32_t slices, uint32_t stacks, double radius,
+                  const double c[3]) {
+    const uint32_t row = slices + 1;
+    for (uint32_t i = 0; i <= stacks; i++) {
+        const double theta = M_PI * (double)i / (double)stacks;
+        for (uint32_t j = 0; j <= slices; j++) {
+            const double phi = 2.0 * M_PI * (double)j / (double)slices;
+            const double n[3] = {std::sin(theta) * std::cos(phi), std::cos(theta), std::sin(theta) * std::sin(phi)};
+            const uint32_t v = voff + i * row + j;
+            m->px[v] = (float)(c[0] + radius * n[0]);
+            m->py[v] = (float)(c[1] + radius * n[1]);
+            m->pz[v] = (float)(c[2] + radius * n[2]);
+            m->nx[v] = (float)n[0];
+            m->ny[v] = (float)n[1];
+            m->nz[v] = (float)n[2];
+        }
+    }
+    uint32_t t = toff;
+    for (uint32_t i = 0; i < stacks; i++)
+        for (uint32_t j = 0; j < slices; j++) {
+            const uint32_t a = voff + i * row + j, b = a + 1, cc = a + row, d = cc + 1;
+            uint32_t* I = m->indices + 3ull * t;
+            I[0] = a; I[1] = b; I[2] = cc;
+            I[3] = b; I[4] = d; I[5] = cc;
+            t += 2;
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+int edxr_scene_sphere(edxr_host_mesh* out, uint32_t slices, uint32_t stacks, float radius, const float center[3],
+                      uint64_t color_seed) {
+    if (!out || slices < 3 || stacks < 2) return EDXR_E_INVALID;
+    const uint64_t V = (uint64_t)(stacks + 1) * (slices + 1), T = 2ull * slices * stacks;
+    if (!alloc_mesh(out, V, T)) return EDXR_E_NOMEM;
+    const double c[3] = {center ? center[0] : 0.0, center ? center[1] : 0.0, center ? center[2] : 0.0};
+    build_sphere(out, 0, 0, slices, stacks, radius, c);
+    fill_colors(out, color_seed);
+    return EDXR_OK;
+}
+
+int edxr_scene_heightfield(edxr_host_mesh* out, uint32_t n, float amplitude, uint64_t height_seed,
+                           uint64_t color_seed) {
+    if (!out || n < 2) return EDXR_E_INVALID;
+    const uint64_t V = (uint64_t)n * n, T = 2ull * (n - 1) * (n - 1);
+    if (!alloc_mesh(out, V, T)) return EDXR_E_NOMEM;
+    std::vector<double> h(V);
+    double wsum = 0.0;
+    for (int o = 0; o < 8; o++) wsum += std::ldexp(1.0, -o);
+    const double step = 2.0 / (double)(n - 1);
+    auto rows = [&](uint32_t r0, uint32_t r1) {
+        for (uint32_t i = r0; i < r1; i++)
+            for (uint32_t j = 0; j < n; j++) {
+                const double x = -1.0 + step * (double)j, z = -1.0 + step * (double)i;
+                double s = 0.0;
+                for (int o = 0; o < 8; o++) {
+                    const double f = 4.0 * std::ldexp(1.0, o);
+                    s += std::ldexp(1.0, -o) * value_noise(height_seed, o, f * (x + 1.0), f * (z + 1.0));
+                }
+                h[(size_t)i * n + j] = (double)amplitude * s / wsum;
+            }
+    };
+    {
+        const uint32_t nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (uint32_t k = 0; k < nt; k++) pool.emplace_back(rows, (uint32_t)((uint64_t)n * k / nt), (uint32_t)((uint64_t)n * (k + 1) / nt));
+        for (auto& th : pool) th.join();
+    }
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < n; j++) {
+            const size_t v = (size_t)i * n + j;
+            out->px[v] = (float)(-1.0 + step * (double)j);
+            out->pz[v] = (float)(-1.0 + step * (double)i);
+            out->py[v] = (float)h[v];
+            const uint32_t j0 = j ? j - 1 : j, j1 = j + 1 < n ? j + 1 : j;
+            const uint32_t i0 = i ? i - 1 : i, i1 = i + 1 < n ? i + 1 : i;
+            const double dhdx = (h[(size_t)i * n + j1] - h[(size_t)i * n + j0]) / (step * (double)(j1 - j0));
+            const double dhdz = (h[(size_t)i1 * n + j] - h[(size_t)i0 * n + j]) / (step * (double)(i1 - i0));
+            const double len = std::sqrt(dhdx * dhdx + 1.0 + dhdz * dhdz);
+            out->nx[v] = (float)(-dhdx / len);
+            out->ny[v] = (float)(1.0 / len);
+            out->nz[v] = (float)(-dhdz / len);
+        }
+    uint64_t t = 0;
+    for (uint32_t i = 0; i + 1 < n; i++)
+        for (uint32_t j = 0; j + 1 < n; j++) {
+            const uint32_t a = i * n + j, b = a + 1, c = a + n, d = c + 1;
+            uint32_t* I = out->indices + 3 * t;
+            I[0] = a; I[1] = c; I[2] = b;
+            I[3] = b; I[4] = c; I[5] = d;
+            t += 2;
+        }
+    fill_colors(out, color_seed);
+    return EDXR_OK;
+}
+
+int edxr_scene_sphere_grid(edxr_host_mesh* out, uint32_t grid, uint32_t segments, uint64_t seed) {
+    if (!out || grid == 0 || segments < 3) return EDXR_E_INVALID;
+    const uint64_t vs = (uint64_t)(segments + 1) * (segments + 1), ts = 2ull * segments * segments;
+    const uint64_t count = (uint64_t)grid * grid;
+    if (!alloc_mesh(out, vs * count, ts * count)) return EDXR_E_NOMEM;
+    uint64_t s = seed;
+    for (uint32_t i = 0; i < grid; i++)
+        for (uint32_t j = 0; j < grid; j++) {
+            const uint64_t k = (uint64_t)i * grid + j;
+            const double radius = 0.5 + 1.5 * u01(s);
+            const double c[3] = {5.0 * i - 2.5 * (grid - 1), 0.0, 5.0 * j - 2.5 * (grid - 1)};
+            build_sphere(out, (uint32_t)(vs * k), (uint32_t)(ts * k), segments, segments, radius, c);
+            for (uint64_t v = vs * k; v < vs * (k + 1); v++) {
+                out->cr[v] = u01f(s);
+                out->cg[v] = u01f(s);
+                out->cb[v] = u01f(s);
+            }
+        }
+    return EDXR_OK;
+}
+
+int edxr_scene_merge(edxr_host_mesh* a, const edxr_host_mesh* b) {
+    if (!a || !b) return EDXR_E_INVALID;
+    edxr_host_mesh m;
+    const uint64_t V = (uint64_t)a->num_vertices + b->num_vertices;
+    const uint64_t T = (uint64_t)a->num_triangles + b->num_triangles;
+    if (!alloc_mesh(&m, V, T)) return EDXR_E_NOMEM;
+    float* const* pa[9] = {&a->px, &a->py, &a->pz, &a->nx, &a->ny, &a->nz, &a->cr, &a->cg, &a->cb};
+    float* const* pb[9] = {&b->px, &b->py, &b->pz, &b->nx, &b->ny, &b->nz, &b->cr, &b->cg, &b->cb};
+    float** pm[9] = {&m.px, &m.py, &m.pz, &m.nx, &m.ny, &m.nz, &m.cr, &m.cg, &m.cb};
+    for (int k = 0; k < 9; k++) {
+        memcpy(*pm[k], *pa[k], sizeof(float) * a->num_vertices);
+        memcpy(*pm[k] + a->num_vertices, *pb[k], sizeof(float) * b->num_vertices);
+    }
+    memcpy(m.indices, a->indices, 12ull * a->num_triangles);
+    for (uint64_t i = 0; i < 3ull * b->num_triangles; i++)
+        m.indices[3ull * a->num_triangles + i] = b->indices[i] + a->num_vertices;
+    edxr_scene_free(a);
+    *a = m;
+    return EDXR_OK;
+}
+
+void edxr_scene_free(edxr_host_mesh* m) {
+    if (!m) return;
+    float** planes[9] = {&m->px, &m->py, &m->pz, &m->nx, &m->ny, &m->nz, &m->cr, &m->cg, &m->cb};
+    for (auto p : planes) {
+        free(*p);
+        *p = nullptr;
+    }
+    free(m->indices);
+    m->indices = nullptr;
+    m->num_vertices = m->num_triangles = 0;
+}
+
+void edxr_look_at(const float eye[3], const float at[3], const float up[3], float out[16]) {
+    double z[3] = {(double)at[0] - eye[0], (double)at[1] - eye[1], (double)at[2] - eye[2]};
+    double l = std::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+    for (double& v : z) v /= l;
+    // x = up x z (left-handed basis: x right, y up, z forward)
+    double x[3] = {up[1] * z[2] - up[2] * z[1], up[2] * z[0] - up[0] * z[2], up[0] * z[1] - up[1] * z[0]};
+    l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (double& v : x) v /= l;
+    const double y[3] = {z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2], z[0] * x[1] - z[1] * x[0]};
+    const double e[3] = {eye[0], eye[1], eye[2]};
+    const double m[16] = {x[0], x[1], x[2], -(x[0] * e[0] + x[1] * e[1] + x[2] * e[2]),
+                          y[0], y[1], y[2], -(y[0] * e[0] + y[1] * e[1] + y[2] * e[2]),
+                          z[0], z[1], z[2], -(z[0] * e[0] + z[1] * e[1] + z[2] * e[2]),
+                          0.0,  0.0,  0.0,  1.0};
+    for (int i = 0; i < 16; i++) out[i] = (float)m[i];
+}
+
+void edxr_perspective(float fov_y_deg, float aspect, float znear, float zfar, float out[16]) {
+    const double ys = 1.0 / std::tan((double)fov_y_deg * M_PI / 360.0), xs = ys / (double)aspect;
+    const double q = (double)zfar / ((double)zfar - (double)znear);
+    const double m[16] = {xs, 0, 0, 0, 0, ys, 0, 0, 0, 0, q, -q * (double)znear, 0, 0, 1, 0};
+    for (int i = 0; i < 16; i++) out[i] = (float)m[i];
+}
+
+void edxr_raster_matrix(uint32_t width, uint32_t height, float out[16]) {
+    const double hw = 0.5 * width, hh = 0.5 * height;
+    const double m[16] = {hw, 0, 0, hw, 0, -hh, 0, hh, 0, 0, 1, 0, 0, 0, 0, 1};
+    for (int i = 0; i < 16; i++) out[i] = (float)m[i];
+}
+
+}  // extern "C"

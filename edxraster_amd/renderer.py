@@ -1,0 +1,253 @@
+"""Python mirror of EDXRaster's host API over the C ABI.
+
+``Renderer`` follows ``EDX::RasterRenderer::Renderer`` (EDXRaster/Core/Renderer.h:37-50):
+``initialize``, ``resize``, ``set_transform``, ``render_mesh``, ``get_back_buffer``,
+``set_hierarchical_rasterize``; the pixel-shader object (Core/Shader.h:173-282) is
+selected with ``set_pixel_shader`` and the light with ``set_light``.  ``Mesh``
+follows ``EDX::RasterRenderer::Mesh`` (EDXRaster/Utils/Mesh.h:17-69) with SoA
+fp32 arrays.  Every render goes through libedxraster.so (HIP, gfx950).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _check(ctx, rc):
+    if rc != L.EDXR_OK:
+        msg = L.lib().edxr_last_error(ctx).decode() if ctx else ""
+        raise L.EdxrError(rc, msg)
+
+
+def _mat(m):
+    a = np.ascontiguousarray(np.asarray(m, dtype=np.float32).reshape(16))
+    return a
+
+
+class Mesh:
+    """SoA mesh: positions, normals, RGB colours (float32, V each) and uint32 indices (3T)."""
+
+    def __init__(self, px, py, pz, nx, ny, nz, cr, cg, cb, indices):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        self.px, self.py, self.pz = f(px), f(py), f(pz)
+        self.nx, self.ny, self.nz = f(nx), f(ny), f(nz)
+        self.cr, self.cg, self.cb = f(cr), f(cg), f(cb)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+
+    @property
+    def num_vertices(self):
+        return int(self.px.shape[0])
+
+    @property
+    def num_triangles(self):
+        return int(self.indices.shape[0] // 3)
+
+    def planes(self):
+        return [self.px, self.py, self.pz, self.nx, self.ny, self.nz, self.cr, self.cg, self.cb]
+
+    def positions(self):
+        return np.stack([self.px, self.py, self.pz], axis=1)
+
+    def c_struct(self):
+        """edxr_mesh over this mesh's host arrays (keep the Mesh alive meanwhile)."""
+        return L.Mesh(*[L.fptr(a) for a in self.planes()], L.uptr(self.indices),
+                      self.num_vertices, self.num_triangles, L.MEM_HOST)
+
+    @staticmethod
+    def _from_host(hm):
+        V, T = hm.num_vertices, hm.num_triangles
+        planes = []
+        for name in ("px", "py", "pz", "nx", "ny", "nz", "cr", "cg", "cb"):
+            p = getattr(hm, name)
+            planes.append(np.ctypeslib.as_array(p, shape=(V,)).copy() if V else np.zeros(0, np.float32))
+        idx = np.ctypeslib.as_array(hm.indices, shape=(3 * T,)).copy() if T else np.zeros(0, np.uint32)
+        L.lib().edxr_scene_free(C.byref(hm))
+        return Mesh(*planes, idx)
+
+    # Mesh::LoadSphere (EDXRaster/Utils/Mesh.cpp:53-70)
+    @staticmethod
+    def load_sphere(slices=64, stacks=64, radius=1.0, center=(0.0, 0.0, 0.0), color_seed=1):
+        hm = L.HostMesh()
+        c = np.asarray(center, dtype=np.float32)
+        rc = L.lib().edxr_scene_sphere(C.byref(hm), slices, stacks, radius, L.fptr(c), color_seed)
+        _check(None, rc)
+        return Mesh._from_host(hm)
+
+    # Mesh::LoadPlane stand-in: a noise heightfield (EDXRaster/Utils/Mesh.cpp:36-51)
+    @staticmethod
+    def load_heightfield(n, amplitude=0.2, height_seed=3, color_seed=33):
+        hm = L.HostMesh()
+        _check(None, L.lib().edxr_scene_heightfield(C.byref(hm), n, amplitude, height_seed, color_seed))
+        return Mesh._from_host(hm)
+
+    @staticmethod
+    def sphere_grid(grid=16, segments=64, seed=2):
+        hm = L.HostMesh()
+        _check(None, L.lib().edxr_scene_sphere_grid(C.byref(hm), grid, segments, seed))
+        return Mesh._from_host(hm)
+
+    @staticmethod
+    def merge(a, b):
+        planes = [np.concatenate([x, y]) for x, y in zip(a.planes(), b.planes())]
+        idx = np.concatenate([a.indices, b.indices + np.uint32(a.num_vertices)])
+        return Mesh(*planes, idx)
+
+
+def look_at(eye, at, up=(0.0, 1.0, 0.0)):
+    out = np.zeros(16, np.float32)
+    e, a, u = (np.asarray(v, np.float32) for v in (eye, at, up))
+    L.lib().edxr_look_at(L.fptr(e), L.fptr(a), L.fptr(u), L.fptr(out))
+    return out.reshape(4, 4)
+
+
+def perspective(fov_y_deg, aspect, znear, zfar):
+    out = np.zeros(16, np.float32)
+    L.lib().edxr_perspective(fov_y_deg, aspect, znear, zfar, L.fptr(out))
+    return out.reshape(4, 4)
+
+
+def raster_matrix(width, height):
+    out = np.zeros(16, np.float32)
+    L.lib().edxr_raster_matrix(width, height, L.fptr(out))
+    return out.reshape(4, 4)
+
+
+def default_material(kind):
+    m = L.Material()
+    L.lib().edxr_default_material(kind, C.byref(m))
+    return m
+
+
+class Renderer:
+    """EDX::RasterRenderer::Renderer on one MI355X (one context, one HIP stream)."""
+
+    def __init__(self, width=None, height=None):
+        self._ctx = None
+        if width is not None:
+            self.initialize(width, height)
+
+    # Renderer::Initialize (Core/Renderer.cpp:22-62)
+    def initialize(self, width, height):
+        self.close()
+        ctx = C.c_void_p()
+        rc = L.lib().edxr_create(C.byref(ctx), width, height)
+        if rc != L.EDXR_OK:
+            raise L.EdxrError(rc, "edxr_create failed (is a gfx950 GPU visible?)")
+        self._ctx = ctx
+        self.width, self.height = int(width), int(height)
+
+    def close(self):
+        if self._ctx:
+            L.lib().edxr_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    # Renderer::Resize (Core/Renderer.cpp:64-83)
+    def resize(self, width, height):
+        _check(self._ctx, L.lib().edxr_resize(self._ctx, width, height))
+        self.width, self.height = int(width), int(height)
+
+    # Renderer::SetTransform (Core/Renderer.cpp:85-92)
+    def set_transform(self, model_view, proj, to_raster):
+        mv, p, r = _mat(model_view), _mat(proj), _mat(to_raster)
+        _check(self._ctx, L.lib().edxr_set_transform(self._ctx, L.fptr(mv), L.fptr(p), L.fptr(r)))
+
+    def set_light(self, direction):
+        d = np.ascontiguousarray(direction, dtype=np.float32)
+        _check(self._ctx, L.lib().edxr_set_light(self._ctx, L.fptr(d)))
+
+    def set_pixel_shader(self, kind, material=None):
+        _check(self._ctx, L.lib().edxr_set_pixel_shader(self._ctx, kind, C.byref(material) if material else None))
+
+    # Renderer::SetHierarchicalRasterize (Core/Renderer.h:49)
+    def set_hierarchical_rasterize(self, on):
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_HIERARCHICAL, 1 if on else 0))
+
+    def set_bin_chunks(self, n):
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_BIN_CHUNKS, int(n)))
+
+    # Renderer::RenderMesh (Core/Renderer.cpp:100-118)
+    def render_mesh(self, mesh):
+        if isinstance(mesh, Mesh):
+            cm = mesh.c_struct()
+        else:
+            cm = mesh  # an edxr_mesh structure (e.g. device pointers)
+        _check(self._ctx, L.lib().edxr_render_mesh(self._ctx, C.byref(cm)))
+
+    # Renderer::GetBackBuffer (Core/Renderer.cpp:360-363): RGBA8, bottom-up rows.
+    def get_back_buffer(self):
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        _check(self._ctx, L.lib().edxr_read_color(self._ctx, out.ctypes.data, out.nbytes))
+        return out
+
+    def read_depth(self):
+        out = np.empty((self.height, self.width), np.float32)
+        _check(self._ctx, L.lib().edxr_read_depth(self._ctx, out.ctypes.data, out.size))
+        return out
+
+    def read_triangle_ids(self):
+        out = np.empty((self.height, self.width), np.uint32)
+        _check(self._ctx, L.lib().edxr_read_triangle_ids(self._ctx, out.ctypes.data, out.size))
+        return out
+
+    def set_profiling(self, on):
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_PROFILE, 1 if on else 0))
+
+    def profile(self, reset=False):
+        """Summed per-stage milliseconds (HIP events) and the frame count."""
+        ms = (C.c_double * len(L.STAGES))()
+        n = C.c_uint64()
+        _check(self._ctx, L.lib().edxr_profile_read(self._ctx, ms, C.byref(n)))
+        if reset:
+            _check(self._ctx, L.lib().edxr_profile_reset(self._ctx))
+        return dict(zip(L.STAGES, list(ms))), n.value
+
+    def device_buffers(self):
+        c, d, i = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(self._ctx, L.lib().edxr_device_buffers(self._ctx, C.byref(c), C.byref(d), C.byref(i)))
+        return c.value, d.value, i.value
+
+    def stats(self):
+        s = L.Stats()
+        _check(self._ctx, L.lib().edxr_get_stats(self._ctx, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in L.Stats._fields_}
+
+    def debug_records(self):
+        """Setup records in submission order, (N, 16) uint32 (see edxr_debug_read_records)."""
+        n = L.lib().edxr_debug_read_records(self._ctx, None, 0)
+        if n < 0:
+            _check(self._ctx, int(n))
+        out = np.zeros((max(n, 1), 16), np.uint32)
+        n2 = L.lib().edxr_debug_read_records(self._ctx, out.ctypes.data, n)
+        if n2 < 0:
+            _check(self._ctx, int(n2))
+        return out[:n]
+
+    def debug_bins(self):
+        """(offsets[num_tiles+1], refs (N,3) = tri_id, sub, flags)."""
+        ntiles = ((self.width + 31) // 32) * ((self.height + 31) // 32)
+        offs = np.zeros(ntiles + 1, np.uint64)
+        n = L.lib().edxr_debug_read_bins(self._ctx, offs.ctypes.data, None, 0)
+        if n < 0:
+            _check(self._ctx, int(n))
+        refs = np.zeros((max(n, 1), 3), np.uint32)
+        n2 = L.lib().edxr_debug_read_bins(self._ctx, offs.ctypes.data, refs.ctypes.data, n)
+        if n2 < 0:
+            _check(self._ctx, int(n2))
+        return offs, refs[:n]

@@ -1,0 +1,194 @@
+/*
+ * edxraster.h -- C ABI of the MI355X (gfx950) EDXRaster pipeline.
+ *
+ * Each entry point replaces one member of the reference's C++ renderer API
+ * (EDXRaster/Core/Renderer.h:37-50 and the shader objects of Core/Shader.h);
+ * the replaced member is named in the comment above each declaration.
+ *
+ * Conventions
+ *   - Status: 0 = OK, negative = error (EDXR_E_*); edxr_last_error() gives text.
+ *     No C++ exception crosses this boundary.
+ *   - Matrices are row-major float[16], column-vector convention
+ *     (out_r = sum_c m[4r+c] * v_c), the convention of EDXUtil's
+ *     Matrix::TransformPoint used by the reference.
+ *   - Output images use GetBackBuffer's layout (Core/FrameBuffer.h:55-58):
+ *     W x H, row (H-1-y) holds raster row y (y grows downwards on screen).
+ *   - One context = one renderer = one HIP stream on the current device.  The
+ *     context owns its device memory; callers own every host array they pass.
+ *     Calls on one context are serialised; separate contexts are independent.
+ *   - There is no CPU fallback: every render call runs HIP kernels and fails
+ *     with EDXR_E_HIP when no usable gfx950 device is present.
+ */
+#ifndef EDXRASTER_H
+#define EDXRASTER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDXR_VERSION_MAJOR 0
+#define EDXR_VERSION_MINOR 1
+
+enum edxr_status {
+    EDXR_OK = 0,
+    EDXR_E_INVALID = -1,   /* bad argument */
+    EDXR_E_HIP = -2,       /* HIP runtime error or no device */
+    EDXR_E_NOMEM = -3,     /* device allocation failed */
+    EDXR_E_INDEX = -4,     /* an index >= num_vertices */
+    EDXR_E_LIMIT = -5,     /* size beyond a documented limit */
+    EDXR_E_STATE = -6      /* call out of order (e.g. backward before forward) */
+};
+
+enum edxr_shader_kind {
+    /* LambertianAlbedoPixelShader (Core/Shader.h:209-244) with a constant
+     * texture, as Mesh::LoadSphere/LoadPlane bind (Utils/Mesh.cpp:44,66). */
+    EDXR_SHADER_LAMBERT_ALBEDO = 0,
+    /* BlinnPhongPixelShader (Core/Shader.h:246-282): albedo 1, shininess 200. */
+    EDXR_SHADER_BLINN_PHONG_REF = 1,
+    /* Vertex-colour Blinn-Phong: diffuse * interpolated RGB + specular. */
+    EDXR_SHADER_VC_BLINN_PHONG = 2
+};
+
+/* Pixel-shader parameters.  The reference values are ambient 0.2, kd 3,
+ * ks 3, shininess 200 (Core/Shader.h:264,275-278). */
+typedef struct edxr_material {
+    float ambient;
+    float kd;
+    float ks;
+    float shininess;     /* integer values use exponentiation by squaring */
+    float albedo[3];     /* constant albedo of EDXR_SHADER_LAMBERT_ALBEDO */
+    int32_t clamp_specular; /* 0 = reference (pow of the unclamped N.H) */
+} edxr_material;
+
+enum edxr_state_key {
+    /* Renderer::SetHierarchicalRasterize (Core/Renderer.h:49); default 1. */
+    EDXR_STATE_HIERARCHICAL = 1,
+    /* Bin chunks used by the order-preserving binning (0 = automatic). */
+    EDXR_STATE_BIN_CHUNKS = 2,
+    /* 1 = record HIP events around each pipeline stage of every frame. */
+    EDXR_STATE_PROFILE = 3
+};
+
+/* Pipeline stages timed when EDXR_STATE_PROFILE is on. */
+enum edxr_stage {
+    EDXR_STAGE_SETUP = 0,       /* vertex transform, outcodes, setup, clipping */
+    EDXR_STAGE_BIN_COUNT = 1,
+    EDXR_STAGE_BIN_SCAN = 2,
+    EDXR_STAGE_BIN_SCATTER = 3,
+    EDXR_STAGE_RASTER = 4,      /* rasterisation, depth test, shading, write-back */
+    EDXR_NUM_STAGES = 5
+};
+
+enum edxr_memory { EDXR_MEM_HOST = 0, EDXR_MEM_DEVICE = 1 };
+
+/* One mesh (Utils/Mesh.h:17-69, Utils/InputBuffer.h:43-194), SoA fp32.
+ * Normals and colours may be NULL for EDXR_SHADER_LAMBERT_ALBEDO /
+ * EDXR_SHADER_BLINN_PHONG_REF (they are then read as (0,0,0)). */
+typedef struct edxr_mesh {
+    const float *px, *py, *pz;
+    const float *nx, *ny, *nz;
+    const float *cr, *cg, *cb;
+    const uint32_t *indices;     /* 3 per triangle (IndexBuffer) */
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    int32_t memory;              /* EDXR_MEM_HOST or EDXR_MEM_DEVICE */
+} edxr_mesh;
+
+/* Per-frame counters (filled after every render). */
+typedef struct edxr_stats {
+    uint64_t triangles_in;
+    uint64_t trivially_rejected;
+    uint64_t clipped;            /* straddled at least one plane */
+    uint64_t near_clipped;       /* straddled the near plane */
+    uint64_t setup_records;      /* triangles (and clipped pieces) after setup */
+    uint64_t refs;               /* tile references (N_ref) */
+    uint64_t clip_vertices;      /* vertices created by clipping */
+    uint32_t retries;            /* re-runs after a capacity overflow */
+    uint32_t bin_chunks;
+} edxr_stats;
+
+typedef struct edxr_ctx edxr_ctx;
+
+/* Renderer::Initialize (Core/Renderer.cpp:22-62).  w, h <= 8192 and the
+ * tile grid (32x32-pixel tiles) holds at most 16384 tiles. */
+int edxr_create(edxr_ctx** out, uint32_t width, uint32_t height);
+/* Renderer::~Renderer. */
+void edxr_destroy(edxr_ctx* ctx);
+/* Renderer::Resize (Core/Renderer.cpp:64-83). */
+int edxr_resize(edxr_ctx* ctx, uint32_t width, uint32_t height);
+/* Renderer::SetTransform (Core/Renderer.cpp:85-92): MVP = proj * mv and
+ * MV^-1 are derived on the host. */
+int edxr_set_transform(edxr_ctx* ctx, const float mv[16], const float proj[16], const float raster[16]);
+/* Light direction passed to PixelShader::Shade (reference: the constant
+ * (1,1,-1) at Core/Renderer.cpp:290).  Normalised by the shader. */
+int edxr_set_light(edxr_ctx* ctx, const float dir[3]);
+/* Replaces Renderer's pixel-shader object (Core/Renderer.cpp:41). */
+int edxr_set_pixel_shader(edxr_ctx* ctx, int32_t kind, const edxr_material* material);
+/* Fills the reference's shader constants for a kind. */
+void edxr_default_material(int32_t kind, edxr_material* out);
+/* RenderStates flags (Core/RenderStates.h:55-61). */
+int edxr_set_state(edxr_ctx* ctx, int32_t key, int32_t value);
+/* Renderer::RenderMesh (Core/Renderer.cpp:100-118).  Synchronous: returns
+ * after the frame is complete on the device. */
+int edxr_render_mesh(edxr_ctx* ctx, const edxr_mesh* mesh);
+/* Renderer::GetBackBuffer (Core/Renderer.cpp:360-363), copied to the caller:
+ * RGBA8, W*H*4 bytes, bottom-up rows; background 0, alpha 255 where drawn. */
+int edxr_read_color(edxr_ctx* ctx, uint8_t* rgba, size_t bytes);
+/* Final depth (fp32, cleared to 1.0), same layout. */
+int edxr_read_depth(edxr_ctx* ctx, float* depth, size_t count);
+/* Winning input-triangle index per pixel (0xFFFFFFFF = none), same layout. */
+int edxr_read_triangle_ids(edxr_ctx* ctx, uint32_t* ids, size_t count);
+/* Device addresses of the three frame buffers (valid until resize/destroy). */
+int edxr_device_buffers(edxr_ctx* ctx, void** color, void** depth, void** ids);
+int edxr_get_stats(edxr_ctx* ctx, edxr_stats* out);
+/* Profiling: summed milliseconds per stage (EDXR_NUM_STAGES values) and the
+ * number of frames measured since the last reset. */
+int edxr_profile_read(edxr_ctx* ctx, double* stage_ms, uint64_t* frames);
+int edxr_profile_reset(edxr_ctx* ctx);
+/* The HIP stream (hipStream_t) the context launches on. */
+void* edxr_stream(edxr_ctx* ctx);
+const char* edxr_last_error(const edxr_ctx* ctx);
+
+/* ---- introspection for parity tests (no counterpart in the reference) ---- */
+/* Setup records in submission order: per record 16 uint32 words
+ * {x0,y0,x1,y1,x2,y2, inv_det, z0,z1,z2, tri_id, sub, corners, v0,v1,v2}
+ * (floats as bit patterns).  Returns the record count or a negative error;
+ * writes at most cap records. */
+int64_t edxr_debug_read_records(edxr_ctx* ctx, uint32_t* out, uint64_t cap);
+/* Per-tile reference lists: offsets[num_tiles+1] and per reference
+ * {tri_id, sub, flags} (flags: bit0-2 accept edge, bit3 trivial, bit4 big).
+ * Returns the reference count or a negative error. */
+int64_t edxr_debug_read_bins(edxr_ctx* ctx, uint64_t* offsets, uint32_t* refs3, uint64_t cap);
+
+/* ---- scenes (Mesh::LoadSphere / LoadPlane stand-ins; host memory) ---- */
+typedef struct edxr_host_mesh {
+    float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb;
+    uint32_t* indices;
+    uint32_t num_vertices, num_triangles;
+} edxr_host_mesh;
+
+/* UV sphere: (stacks+1)*(slices+1) vertices, 2*slices*stacks triangles,
+ * outward faces clockwise on screen; colours U[0,1] from SplitMix64(seed). */
+int edxr_scene_sphere(edxr_host_mesh* out, uint32_t slices, uint32_t stacks, float radius,
+                      const float center[3], uint64_t color_seed);
+/* n x n heightfield on [-1,1]^2 (y up), 8-octave value noise * amplitude. */
+int edxr_scene_heightfield(edxr_host_mesh* out, uint32_t n, float amplitude, uint64_t height_seed,
+                           uint64_t color_seed);
+/* 16x16 grid of spheres (segments x segments), radii U[0.5,2], one mesh. */
+int edxr_scene_sphere_grid(edxr_host_mesh* out, uint32_t grid, uint32_t segments, uint64_t seed);
+/* Appends b to a (a's indices unchanged, b's shifted). */
+int edxr_scene_merge(edxr_host_mesh* a, const edxr_host_mesh* b);
+void edxr_scene_free(edxr_host_mesh* m);
+/* Left-handed look-at view matrix and D3D-style projection (z in [0,1]),
+ * raster matrix x_r = (x+1)W/2, y_r = (1-y)H/2.  Computed in double. */
+void edxr_look_at(const float eye[3], const float at[3], const float up[3], float out[16]);
+void edxr_perspective(float fov_y_deg, float aspect, float znear, float zfar, float out[16]);
+void edxr_raster_matrix(uint32_t width, uint32_t height, float out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
