@@ -92,6 +92,17 @@ SIGNATURES = [
 _lib = None
 
 
+def _share_torch_runtime():
+    """PyTorch wheels bundle a HIP runtime whose soname (libamdhip64.so.7) is
+    the one this library links against.  Importing torch first makes the
+    library bind to that copy, so the process holds a single HIP runtime; the
+    other order makes torch load a second runtime, which then sees no device."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Loads libedxraster.so once; raises if it has not been built."""
     global _lib
@@ -99,6 +110,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 "libedxraster.so is not built (run __graft_entry__.build() or python edxraster_amd/build.py)")
+        _share_torch_runtime()
         l = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(l, name)

@@ -1,8 +1,8 @@
-"""Build recipe for the gfx950 library and the CPU oracle.
+"""Build recipe for the gfx950 library.
 
 ``build_library()`` compiles ``csrc/*.hip`` and ``csrc/*.cpp`` with hipcc for
 gfx950 into ``edxraster_amd/libedxraster.so`` (in-tree, so the built library
-travels with the source tree).  ``build_oracle()`` runs ``oracle/Makefile``.
+travels with the source tree).
 """
 import os
 import subprocess
@@ -43,14 +43,6 @@ def build_library(verbose=False):
     return LIB
 
 
-def build_oracle(verbose=False):
-    cmd = ["make", "-C", os.path.join(ROOT, "oracle"), "-s"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-
-
 if __name__ == "__main__":
     build_library(verbose=True)
-    build_oracle(verbose=True)
     print("built", LIB, file=sys.stderr)

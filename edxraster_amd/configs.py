@@ -1,7 +1,7 @@
 """Synthetic workloads: seeded scenes, cameras and resolutions.
 
-Every scene is generated on the host with SplitMix64 seeds, so the HIP
-pipeline and the CPU oracle receive identical bytes.  Projection: D3D-style
+Every scene is generated on the host with SplitMix64 seeds, so every consumer
+receives identical bytes.  Projection: D3D-style
 (z in [0,1]), left-handed, vertical fov 60 degrees, near 0.1, far 100.  Light
 (1,1,-1) (EDXRaster/Core/Renderer.cpp:290), vertex-colour Blinn-Phong with the
 reference constants (ambient 0.2, diffuse 3/pi, specular x3) and shininess 32.

@@ -127,9 +127,13 @@ struct edxr_ctx {
     // pipeline buffers
     DevBuf<RasterRec> rrec;
     DevBuf<ShadeRec> srec;
-    DevBuf<uint32_t> desc, clipQueue, binCounts, tileTotals, tileStart, refs, counters, color, ids;
+    DevBuf<uint32_t> desc, clipQueue, binCounts, tileTotals, tileStart, refs, counters, color, ids, work, tileItem;
     DevBuf<float> depth;
-    DevBuf<float> extra[10];
+    DevBuf<uint64_t> partials;
+    DevBuf<uint16_t> slotRefs;
+    DevBuf<uint32_t> groupSums;
+    uint32_t numCUs = 256;
+    DevBuf<ShadeVert> verts;  // V input + extraVertCap clip-created shading vertices
     uint32_t clipQueueCap = 0, extraRecCap = 0, extraVertCap = 0, refCap = 0;
     uint32_t* hostCounters = nullptr;
 
@@ -165,16 +169,20 @@ int set_size(edxr_ctx* c, uint32_t w, uint32_t h) {
     hipError_t e;
     if ((e = c->color.ensure(P)) != hipSuccess || (e = c->depth.ensure(P)) != hipSuccess ||
         (e = c->ids.ensure(P)) != hipSuccess || (e = c->tileTotals.ensure(c->nTiles)) != hipSuccess ||
-        (e = c->tileStart.ensure(c->nTiles + 1)) != hipSuccess)
+        (e = c->tileStart.ensure(c->nTiles + 1)) != hipSuccess ||
+        (e = c->tileItem.ensure(c->nTiles)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(frame buffers)");
     c->rendered = false;
     return EDXR_OK;
 }
 
+// Binning chunks: about 1024 triangles each so the count/scatter grids fill
+// the GPU, capped so the chunk x tile count matrix stays within 64 MB.
 uint32_t choose_chunks(const edxr_ctx* c, uint32_t T) {
-    if (c->binChunksSetting) return std::max(1u, std::min(c->binChunksSetting, std::max(1u, T)));
-    uint32_t C = (T + 8191) / 8192;
-    return std::max(1u, std::min(C, 1024u));
+    if (c->binChunksSetting) return std::max(1u, std::min(std::min(c->binChunksSetting, kMaxChunks), std::max(1u, T)));
+    const uint32_t byMatrix = (uint32_t)std::max<uint64_t>(1, (64ull << 20) / (4ull * c->nTiles));
+    const uint32_t C = (T + 1023) / 1024;
+    return std::max(1u, std::min(std::min(C, byMatrix), kMaxChunks));
 }
 
 }  // namespace
@@ -207,6 +215,11 @@ int edxr_create(edxr_ctx** out, uint32_t width, uint32_t height) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hostCounters, sizeof(uint32_t) * kCntSlots, hipHostMallocDefault);
     if (e == hipSuccess) e = c->counters.ensure(kCntSlots);
+    if (e == hipSuccess) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
+            c->numCUs = (uint32_t)cus;
+    }
     if (e != hipSuccess) {
         edxr_destroy(c);
         return EDXR_E_HIP;
@@ -228,10 +241,13 @@ void edxr_destroy(edxr_ctx* c) {
     c->rrec.release();
     c->srec.release();
     for (DevBuf<uint32_t>* b : {&c->desc, &c->clipQueue, &c->binCounts, &c->tileTotals, &c->tileStart, &c->refs,
-                                &c->counters, &c->color, &c->ids})
+                                &c->counters, &c->color, &c->ids, &c->work, &c->tileItem})
         b->release();
     c->depth.release();
-    for (auto& b : c->extra) b.release();
+    c->partials.release();
+    c->slotRefs.release();
+    c->groupSums.release();
+    c->verts.release();
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
     for (auto& ev : c->events)
         if (ev) (void)hipEventDestroy(ev);
@@ -366,28 +382,40 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
         if ((uint64_t)T + c->extraRecCap > kRecordMask) return c->fail(EDXR_E_LIMIT, "record capacity exceeded");
         if ((e = c->rrec.ensure((size_t)T + c->extraRecCap)) != hipSuccess ||
             (e = c->srec.ensure((size_t)T + c->extraRecCap)) != hipSuccess || (e = c->desc.ensure(T)) != hipSuccess ||
+            (e = c->slotRefs.ensure(T)) != hipSuccess ||
+            (e = c->groupSums.ensure((size_t)((P.num_chunks + kChunkGroup - 1) / kChunkGroup) * c->nTiles)) !=
+                hipSuccess ||
             (e = c->clipQueue.ensure(c->clipQueueCap)) != hipSuccess ||
             (e = c->binCounts.ensure((size_t)P.num_chunks * c->nTiles)) != hipSuccess ||
-            (e = c->refs.ensure(c->refCap)) != hipSuccess)
+            (e = c->refs.ensure(c->refCap)) != hipSuccess ||
+            (e = c->work.ensure((size_t)c->nTiles + c->refCap / kSegRefs + 2)) != hipSuccess ||
+            (e = c->partials.ensure(c->work.cap * kTileSize * kTileSize)) != hipSuccess)
             return c->hip_fail(e, "hipMalloc(pipeline)");
-        for (auto& b : c->extra)
-            if ((e = b.ensure(c->extraVertCap)) != hipSuccess) return c->hip_fail(e, "hipMalloc(clip vertices)");
+        if ((e = c->verts.ensure((size_t)V + c->extraVertCap)) != hipSuccess)
+            return c->hip_fail(e, "hipMalloc(vertices)");
 
         DeviceBuffers B;
         B.rrec = c->rrec.p;
         B.srec = c->srec.p;
         B.desc = c->desc.p;
+        B.slot_refs = c->slotRefs.p;
+        B.group_sums = c->groupSums.p;
         B.clip_queue = c->clipQueue.p;
         B.clip_queue_cap = c->clipQueueCap;
         B.extra_rec_cap = c->extraRecCap;
-        B.extra = ExtraVerts{c->extra[0].p, c->extra[1].p, c->extra[2].p, c->extra[3].p, c->extra[4].p,
-                             c->extra[5].p, c->extra[6].p, c->extra[7].p, c->extra[8].p, c->extra[9].p,
-                             c->extraVertCap};
+        B.verts = c->verts.p;
+        B.extra_vert_cap = c->extraVertCap;
         B.bin_counts = c->binCounts.p;
         B.tile_totals = c->tileTotals.p;
         B.tile_start = c->tileStart.p;
         B.refs = c->refs.p;
         B.ref_cap = c->refCap;
+        B.work = c->work.p;
+        B.work_cap = (uint32_t)c->work.cap;
+        B.tile_item = c->tileItem.p;
+        B.partials = c->partials.p;
+        // Persistent raster grid: enough resident workgroups per CU to hide latency.
+        B.raster_blocks = c->numCUs * 8;
         B.counters = c->counters.p;
         B.color = c->color.p;
         B.depth = c->depth.p;
@@ -404,11 +432,16 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
         if (ov & kOvPolygon) return c->fail(EDXR_E_LIMIT, "clip polygon exceeded 9 vertices");
         if (!ov) {
             if (h[kCntBadIndex]) return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices");
+            uint64_t rejected = 0, mainRecords = 0;
+            for (uint32_t s = 0; s < kStripes; s++) {
+                rejected += h[kCntStriped + kStRejected * kStripes + s];
+                mainRecords += h[kCntStriped + kStMainRecords * kStripes + s];
+            }
             c->stats.triangles_in = T;
-            c->stats.trivially_rejected = h[kCntRejected];
+            c->stats.trivially_rejected = rejected;
             c->stats.clipped = h[kCntClipped];
             c->stats.near_clipped = h[kCntNearClipped];
-            c->stats.setup_records = (uint64_t)h[kCntMainRecords] + h[kCntExtraRecords];
+            c->stats.setup_records = mainRecords + h[kCntExtraRecords];
             c->stats.refs = h[kCntTotalRefs];
             c->stats.clip_vertices = h[kCntExtraVertices];
             c->stats.retries = attempt;
@@ -514,8 +547,10 @@ int64_t edxr_debug_read_records(edxr_ctx* c, uint32_t* out, uint64_t cap) {
     const uint64_t nrec = (uint64_t)T + c->extraRecCap;
     std::vector<RasterRec> rr(nrec);
     std::vector<ShadeRec> sr(nrec);
+    std::vector<uint32_t> idx(3ull * T);
     hipError_t e = hipSuccess;
     if (T) e = hipMemcpy(desc.data(), c->desc.p, 4ull * T, hipMemcpyDeviceToHost);
+    if (T && e == hipSuccess) e = hipMemcpy(idx.data(), c->lastMesh.idx, 12ull * T, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(rr.data(), c->rrec.p, sizeof(RasterRec) * nrec, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(sr.data(), c->srec.p, sizeof(ShadeRec) * nrec, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return c->hip_fail(e, "debug_read_records");
@@ -525,7 +560,8 @@ int64_t edxr_debug_read_records(edxr_ctx* c, uint32_t* out, uint64_t cap) {
         for (uint32_t k = 0; k < cnt; k++, n++) {
             if (n >= cap || !out) continue;
             const RasterRec& r = rr[first + k];
-            const ShadeRec& s = sr[first + k];
+            // Unclipped triangles keep their input vertex ids (no ShadeRec).
+            const ShadeRec s = (r.order & 1u) ? sr[first + k] : ShadeRec{idx[3 * t], idx[3 * t + 1], idx[3 * t + 2], t};
             uint32_t* o = out + 16 * n;
             o[0] = (uint32_t)r.x0; o[1] = (uint32_t)r.y0; o[2] = (uint32_t)r.x1; o[3] = (uint32_t)r.y1;
             o[4] = (uint32_t)r.x2; o[5] = (uint32_t)r.y2;
@@ -564,7 +600,7 @@ int64_t edxr_debug_read_bins(edxr_ctx* c, uint64_t* offsets, uint32_t* refs3, ui
             const uint32_t p = refs[i], ri = p & kRecordMask;
             const uint32_t acc = (p >> kRecordBits) & 7u;
             const uint32_t big = p >> 31;
-            refs3[3 * i] = sr[ri].tri_id;
+            refs3[3 * i] = rr[ri].order >> 4;
             refs3[3 * i + 1] = (rr[ri].info >> 12) & 7u;
             refs3[3 * i + 2] = acc | ((acc == 7u) ? 8u : 0u) | (big ? 16u : 0u);
         }

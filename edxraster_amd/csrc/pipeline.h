@@ -14,6 +14,10 @@ constexpr uint32_t kMaxDim = 8192;
 constexpr uint32_t kRecordBits = 28;         // record index bits in a tile reference
 constexpr uint32_t kRecordMask = (1u << kRecordBits) - 1;
 constexpr int kMaxPoly = 16;                 // clip polygon capacity (convex input needs <= 9)
+constexpr uint32_t kSegRefs = 2048;          // tile references per raster work item
+constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
+constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
+constexpr uint32_t kMaxChunks = 4096;
 
 enum ShaderKind { kLambertAlbedo = 0, kBlinnPhongRef = 1, kVcBlinnPhong = 2 };
 
@@ -39,11 +43,17 @@ struct MeshPtrs {
     const uint32_t* idx;
 };
 
-// Vertices created by clipping (Clipper.h:136-151), SoA.
-struct ExtraVerts {
-    float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb, *inv_w;
-    uint32_t capacity;
+// Post-transform vertex for shading (ProjectedVertex, Core/Shader.h:13-25,
+// with RGB instead of UV): one 48-byte AoS record so a vertex fetch touches
+// one cache line instead of nine SoA planes.  Indices < V are the input
+// vertices (written by k_vertex); V + e are vertices created by clipping
+// (Core/Clipper.h:136-151).
+struct __attribute__((aligned(16))) ShadeVert {
+    float px, py, pz, nx;
+    float ny, nz, cr, cg;
+    float cb, inv_w, pad0, pad1;
 };
+static_assert(sizeof(ShadeVert) == 48, "ShadeVert layout");
 
 // Rasterisation part of a setup record (RasterTriangle, Core/RasterTriangle.h:13-23).
 struct __attribute__((aligned(16))) RasterRec {
@@ -51,13 +61,21 @@ struct __attribute__((aligned(16))) RasterRec {
     int32_t x2, y2;
     float inv_det, z0;
     float z1, z2;
-    uint32_t info;        // corners (12 bits) | sub << 12
-    uint32_t pad;
+    uint32_t info;        // corners (12 bits) | fan index << 12
+    uint32_t order;       // submission order: tri << 4 | survivor << 1 | clipped
 };
 static_assert(sizeof(RasterRec) == 48, "RasterRec layout");
 
-// Shading part: unified vertex ids (< V input, >= V clip-created) and the
-// input triangle index.
+// Submission order of a record: input triangle, then the index among the
+// triangle's surviving fan pieces (Core/Clipper.h:155-170).  Unclipped
+// triangles live in record slot `tri`; clipped ones at desc[tri] >> 3 + survivor.
+__host__ __device__ inline uint32_t make_order(uint32_t tri, uint32_t survivor, bool clipped) {
+    return (tri << 4) | (survivor << 1) | (clipped ? 1u : 0u);
+}
+
+// Shading part of a clipped piece: unified vertex ids (< V input, >= V
+// clip-created) and the input triangle index.  Unclipped triangles take their
+// vertex ids from the index buffer.
 struct __attribute__((aligned(16))) ShadeRec {
     uint32_t v0, v1, v2, tri_id;
 };
@@ -73,8 +91,16 @@ enum CounterSlot {
     kCntNearClipped,
     kCntMainRecords,
     kCntTotalRefs,
-    kCntSlots = 16
+    kCntWorkItems,       // raster work items (tile segments)
+    kCntWorkNext,        // next work item to claim
+    kCntStriped = 16,    // high-rate counters: kStripes slots each, summed on the host
+    kCntSlots = kCntStriped + 2 * 32
 };
+
+// Counters bumped by many workgroups are striped over kStripes addresses so
+// the atomics do not serialise on one cache line.
+constexpr uint32_t kStripes = 32;
+enum StripedCounter { kStRejected = 0, kStMainRecords = 1 };
 
 enum OverflowBit {
     kOvClipQueue = 1,
@@ -91,12 +117,20 @@ struct DeviceBuffers {
     uint32_t* clip_queue;
     uint32_t clip_queue_cap;
     uint32_t extra_rec_cap;
-    ExtraVerts extra;
+    ShadeVert* verts;      // V + extra_vert_cap shading vertices
+    uint32_t extra_vert_cap;
+    uint16_t* slot_refs;   // per input triangle: tile references (saturating)
     uint32_t* bin_counts;  // num_chunks * num_tiles, scanned in place
+    uint32_t* group_sums;  // (num_chunks / kChunkGroup) * num_tiles
     uint32_t* tile_totals;
     uint32_t* tile_start;  // num_tiles + 1
     uint32_t* refs;
     uint32_t ref_cap;
+    uint32_t* work;        // raster work items: tile | segment << 14
+    uint32_t work_cap;
+    uint32_t* tile_item;   // per tile: index of its first work item
+    uint64_t* partials;    // work_cap * 1024 (depth, order) keys, one tile image per work item
+    uint32_t raster_blocks;
     uint32_t* counters;
     uint32_t* color;
     float* depth;

@@ -92,7 +92,6 @@ __device__ __forceinline__ bool setup_record(const FrameParams& P, float ax, flo
     r.inv_det = 1.0f / (float)det;
     const uint32_t c0 = corner_pair(B0, C0), c1 = corner_pair(B1, C1), c2 = corner_pair(B2, C2);
     r.info = (c0 & 3u) | ((c1 & 3u) << 2) | ((c2 & 3u) << 4) | ((c0 >> 8) << 6) | ((c1 >> 8) << 8) | ((c2 >> 8) << 10);
-    r.pad = 0;
     return true;
 }
 
@@ -106,8 +105,40 @@ __device__ __forceinline__ uint32_t wave_count(bool p) { return (uint32_t)__popc
 // Counters
 // ---------------------------------------------------------------------------
 __global__ void k_reset(uint32_t* counters) {
-    if (threadIdx.x < kCntSlots) counters[threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < kCntSlots; i += blockDim.x) counters[i] = 0;
 }
+
+// ---------------------------------------------------------------------------
+// Vertex processing (Core/Renderer.cpp:120-127): one lane per vertex, SoA
+// input read coalesced, one 48-byte shading record out.  invW = 1/w of
+// DefaultVertexShader's MVP transform (Core/Shader.h:45) as the z-divide loop
+// computes it (Core/Renderer.cpp:144).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_vertex(FrameParams P, MeshPtrs M, ShadeVert* __restrict__ verts) {
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= P.num_vertices) return;
+    ShadeVert o;
+    o.px = M.px[v];
+    o.py = M.py[v];
+    o.pz = M.pz[v];
+    o.nx = M.nx[v];
+    o.ny = M.ny[v];
+    o.nz = M.nz[v];
+    o.cr = M.cr ? M.cr[v] : 0.0f;
+    o.cg = M.cg ? M.cg[v] : 0.0f;
+    o.cb = M.cb ? M.cb[v] : 0.0f;
+    const float w = ((P.mvp[12] * o.px + P.mvp[13] * o.py) + P.mvp[14] * o.pz) + P.mvp[15] * 1.0f;
+    o.inv_w = 1.0f / w;
+    o.pad0 = 0.0f;
+    o.pad1 = 0.0f;
+    verts[v] = o;
+}
+
+namespace {
+__device__ __forceinline__ float4 vertex_pos(const ShadeVert* verts, uint32_t v) {
+    return *reinterpret_cast<const float4*>(&verts[v].px);  // (px, py, pz, nx)
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Setup: one lane per input triangle.
@@ -126,9 +157,10 @@ __global__ __launch_bounds__(kBlock) void k_setup(FrameParams P, MeshPtrs M, Dev
             bad = true;
         } else {
             // DefaultVertexShader::Execute: projectedPos = MVP * (p, 1).
-            const float4 c0 = xform4(P.mvp, M.px[i0], M.py[i0], M.pz[i0], 1.0f);
-            const float4 c1 = xform4(P.mvp, M.px[i1], M.py[i1], M.pz[i1], 1.0f);
-            const float4 c2 = xform4(P.mvp, M.px[i2], M.py[i2], M.pz[i2], 1.0f);
+            const float4 p0 = vertex_pos(B.verts, i0), p1 = vertex_pos(B.verts, i1), p2 = vertex_pos(B.verts, i2);
+            const float4 c0 = xform4(P.mvp, p0.x, p0.y, p0.z, 1.0f);
+            const float4 c1 = xform4(P.mvp, p1.x, p1.y, p1.z, 1.0f);
+            const float4 c2 = xform4(P.mvp, p2.x, p2.y, p2.z, 1.0f);
             const uint32_t k0 = clip_code(c0), k1 = clip_code(c1), k2 = clip_code(c2);
             if (k0 | k1 | k2) {
                 if (k0 & k1 & k2) rejected = true;
@@ -154,8 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(FrameParams P, MeshPtrs M, Dev
     if (valid) {
         uint32_t d = 0;
         if (main) {
+            rec.order = make_order(t, 0, false);
             B.rrec[t] = rec;
-            B.srec[t] = ShadeRec{i0, i1, i2, t};
             d = (t << 3) | 1u;
         } else if (queued) {
             const uint32_t slot = qbase + (uint32_t)__popcll(qmask & lanemask_lt());
@@ -164,11 +196,22 @@ __global__ __launch_bounds__(kBlock) void k_setup(FrameParams P, MeshPtrs M, Dev
         }
         B.desc[t] = d;
     }
+    // Per-workgroup totals, one striped atomic per counter per workgroup.
+    __shared__ uint32_t wgCount[3];
+    if (threadIdx.x < 3) wgCount[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t nrej = wave_count(rejected), nbad = wave_count(bad), nmain = wave_count(main);
     if (__lane_id() == 0) {
-        if (nrej) atomicAdd(&B.counters[kCntRejected], nrej);
-        if (nbad) atomicAdd(&B.counters[kCntBadIndex], nbad);
-        if (nmain) atomicAdd(&B.counters[kCntMainRecords], nmain);
+        if (nrej) atomicAdd(&wgCount[0], nrej);
+        if (nbad) atomicAdd(&wgCount[1], nbad);
+        if (nmain) atomicAdd(&wgCount[2], nmain);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t stripe = blockIdx.x & (kStripes - 1);
+        if (wgCount[0]) atomicAdd(&B.counters[kCntStriped + kStRejected * kStripes + stripe], wgCount[0]);
+        if (wgCount[1]) atomicAdd(&B.counters[kCntBadIndex], wgCount[1]);
+        if (wgCount[2]) atomicAdd(&B.counters[kCntStriped + kStMainRecords * kStripes + stripe], wgCount[2]);
     }
 }
 
@@ -219,7 +262,10 @@ __global__ __launch_bounds__(64) void k_clip(FrameParams P, MeshPtrs M, DeviceBu
         const uint32_t t = B.clip_queue[q];
         const uint32_t vi[3] = {M.idx[3 * (size_t)t], M.idx[3 * (size_t)t + 1], M.idx[3 * (size_t)t + 2]};
         float4 c[3];
-        for (int k = 0; k < 3; k++) c[k] = xform4(P.mvp, M.px[vi[k]], M.py[vi[k]], M.pz[vi[k]], 1.0f);
+        for (int k = 0; k < 3; k++) {
+            const float4 p = vertex_pos(B.verts, vi[k]);
+            c[k] = xform4(P.mvp, p.x, p.y, p.z, 1.0f);
+        }
         const uint32_t k0 = clip_code(c[0]), k1 = clip_code(c[1]), k2 = clip_code(c[2]);
         const uint32_t planeCode = (k0 ^ k1) | (k1 ^ k2) | (k2 ^ k0);
         atomicAdd(&B.counters[kCntClipped], 1u);
@@ -287,7 +333,7 @@ __global__ __launch_bounds__(64) void k_clip(FrameParams P, MeshPtrs M, DeviceBu
         uint32_t vbase = 0;
         if (nNew) {
             vbase = atomicAdd(&B.counters[kCntExtraVertices], (uint32_t)nNew);
-            if (vbase + (uint32_t)nNew > B.extra.capacity) {
+            if (vbase + (uint32_t)nNew > B.extra_vert_cap) {
                 atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvExtraVertices);
                 B.desc[t] = 0;
                 continue;
@@ -306,19 +352,24 @@ __global__ __launch_bounds__(64) void k_clip(FrameParams P, MeshPtrs M, DeviceBu
                 vid[j] = P.num_vertices + xi;
                 pos = pp[cur][j];
                 const float3 w = ww[cur][j];
-                auto comb = [&](const float* a) {
-                    return (w.x * a[vi[0]] + w.y * a[vi[1]]) + w.z * a[vi[2]];
-                };
-                B.extra.px[xi] = comb(M.px);
-                B.extra.py[xi] = comb(M.py);
-                B.extra.pz[xi] = comb(M.pz);
-                B.extra.nx[xi] = M.nx ? comb(M.nx) : 0.0f;
-                B.extra.ny[xi] = M.ny ? comb(M.ny) : 0.0f;
-                B.extra.nz[xi] = M.nz ? comb(M.nz) : 0.0f;
-                B.extra.cr[xi] = M.cr ? comb(M.cr) : 0.0f;
-                B.extra.cg[xi] = M.cg ? comb(M.cg) : 0.0f;
-                B.extra.cb[xi] = M.cb ? comb(M.cb) : 0.0f;
-                B.extra.inv_w[xi] = 1.0f / pos.w;
+                const ShadeVert& a = B.verts[vi[0]];
+                const ShadeVert& b = B.verts[vi[1]];
+                const ShadeVert& cc = B.verts[vi[2]];
+                auto comb = [&](float fa, float fb, float fc) { return (w.x * fa + w.y * fb) + w.z * fc; };
+                ShadeVert o;
+                o.px = comb(a.px, b.px, cc.px);
+                o.py = comb(a.py, b.py, cc.py);
+                o.pz = comb(a.pz, b.pz, cc.pz);
+                o.nx = comb(a.nx, b.nx, cc.nx);
+                o.ny = comb(a.ny, b.ny, cc.ny);
+                o.nz = comb(a.nz, b.nz, cc.nz);
+                o.cr = comb(a.cr, b.cr, cc.cr);
+                o.cg = comb(a.cg, b.cg, cc.cg);
+                o.cb = comb(a.cb, b.cb, cc.cb);
+                o.inv_w = 1.0f / pos.w;
+                o.pad0 = 0.0f;
+                o.pad1 = 0.0f;
+                B.verts[vid[j]] = o;
             }
             hx[j] = pos.x / pos.w;
             hy[j] = pos.y / pos.w;
@@ -348,6 +399,7 @@ __global__ __launch_bounds__(64) void k_clip(FrameParams P, MeshPtrs M, DeviceBu
             r.z1 = zz[k - 1];
             r.z2 = zz[k];
             r.info |= (uint32_t)(k - 2) << 12;
+            r.order = make_order(t, s, true);
             B.rrec[first + s] = r;
             B.srec[first + s] = ShadeRec{vid[0], vid[k - 1], vid[k], t};
             s++;
@@ -493,73 +545,147 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(FrameParams P, DeviceBuffe
     for (uint32_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
         RefEnum e;
         e.init(B.desc[s]);
-        uint32_t tile, payload;
-        while (e.next(P, B.rrec, tile, payload)) atomicAdd(&hist[tile], 1u);
+        uint32_t tile, payload, R = 0;
+        while (e.next(P, B.rrec, tile, payload)) {
+            atomicAdd(&hist[tile], 1u);
+            R++;
+        }
+        // Saturated counts are recomputed by the scatter.
+        B.slot_refs[s] = (uint16_t)min(R, (uint32_t)kSlotRefsSaturated);
     }
     __syncthreads();
     uint32_t* row = B.bin_counts + (size_t)chunk * P.num_tiles;
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) row[i] = hist[i];
 }
 
-// Column scan: for 64 tiles per block, exclusive prefix over chunks (in place)
-// and the per-tile totals.
-__global__ __launch_bounds__(kBlock) void k_bin_colscan(FrameParams P, DeviceBuffers B) {
-    __shared__ uint32_t part[kBlock / kWave][kWave];
+// The chunk x tile count matrix is scanned down each tile's column in three
+// steps over blocks of 64 tiles x kChunkGroup chunks (lanes = tiles, each wave
+// a quarter of the group's chunks), so every step has C/64 x tiles/64 blocks.
+namespace {
+// Per-wave partial sums of one (64-tile, chunk-group) block, combined in LDS;
+// returns this wave's exclusive base within the group and the group total.
+__device__ __forceinline__ void colgroup_sums(const FrameParams& P, const DeviceBuffers& B, uint32_t tile,
+                                              uint32_t c0, uint32_t c1, uint32_t (*part)[kWave], uint32_t& waveBase,
+                                              uint32_t& groupTotal) {
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
-    const uint32_t tile = blockIdx.x * kWave + lane;
-    const bool ok = tile < P.num_tiles;
-    const uint32_t C = P.num_chunks, per = (C + 3) / 4;
-    const uint32_t c0 = min(C, wave * per), c1 = min(C, c0 + per);
     uint32_t sum = 0;
-    if (ok)
+    if (tile < P.num_tiles)
         for (uint32_t c = c0; c < c1; c++) sum += B.bin_counts[(size_t)c * P.num_tiles + tile];
     part[wave][lane] = sum;
     __syncthreads();
-    uint32_t run = 0, total = 0;
+    waveBase = 0;
+    groupTotal = 0;
     for (uint32_t w = 0; w < kBlock / kWave; w++) {
-        if (w < wave) run += part[w][lane];
-        total += part[w][lane];
+        if (w < wave) waveBase += part[w][lane];
+        groupTotal += part[w][lane];
     }
-    if (ok) {
-        for (uint32_t c = c0; c < c1; c++) {
-            const size_t o = (size_t)c * P.num_tiles + tile;
-            const uint32_t v = B.bin_counts[o];
-            B.bin_counts[o] = run;
-            run += v;
-        }
-        if (wave == 0) B.tile_totals[tile] = total;
+}
+__device__ __forceinline__ void colgroup_range(const FrameParams& P, uint32_t& c0, uint32_t& c1) {
+    const uint32_t wave = threadIdx.x / kWave, per = kChunkGroup / (kBlock / kWave);
+    c0 = min(P.num_chunks, blockIdx.y * kChunkGroup + wave * per);
+    c1 = min(P.num_chunks, c0 + per);
+}
+}  // namespace
+
+// Step 1: per (tile, chunk group) column sums.
+__global__ __launch_bounds__(kBlock) void k_bin_colsum(FrameParams P, DeviceBuffers B) {
+    __shared__ uint32_t part[kBlock / kWave][kWave];
+    const uint32_t tile = blockIdx.x * kWave + __lane_id();
+    uint32_t c0, c1, waveBase, groupTotal;
+    colgroup_range(P, c0, c1);
+    colgroup_sums(P, B, tile, c0, c1, part, waveBase, groupTotal);
+    if (threadIdx.x < kWave && tile < P.num_tiles) B.group_sums[(size_t)blockIdx.y * P.num_tiles + tile] = groupTotal;
+}
+
+// Step 2: per tile, exclusive scan over the chunk groups and the tile total.
+__global__ __launch_bounds__(kBlock) void k_bin_groupscan(FrameParams P, DeviceBuffers B) {
+    const uint32_t tile = blockIdx.x * kBlock + threadIdx.x;
+    if (tile >= P.num_tiles) return;
+    const uint32_t G = (P.num_chunks + kChunkGroup - 1) / kChunkGroup;
+    uint32_t run = 0;
+    for (uint32_t g = 0; g < G; g++) {
+        uint32_t* p = &B.group_sums[(size_t)g * P.num_tiles + tile];
+        const uint32_t v = *p;
+        *p = run;
+        run += v;
+    }
+    B.tile_totals[tile] = run;
+}
+
+// Step 3: rewrite the counts in place as offsets within the tile's list.
+__global__ __launch_bounds__(kBlock) void k_bin_colscan(FrameParams P, DeviceBuffers B) {
+    __shared__ uint32_t part[kBlock / kWave][kWave];
+    const uint32_t tile = blockIdx.x * kWave + __lane_id();
+    uint32_t c0, c1, waveBase, groupTotal;
+    colgroup_range(P, c0, c1);
+    colgroup_sums(P, B, tile, c0, c1, part, waveBase, groupTotal);
+    if (tile >= P.num_tiles) return;
+    uint32_t run = B.group_sums[(size_t)blockIdx.y * P.num_tiles + tile] + waveBase;
+    for (uint32_t c = c0; c < c1; c++) {
+        const size_t o = (size_t)c * P.num_tiles + tile;
+        const uint32_t v = B.bin_counts[o];
+        B.bin_counts[o] = run;
+        run += v;
     }
 }
 
-// Exclusive scan of the tile totals (num_tiles <= 16384, one block of 1024).
-__global__ __launch_bounds__(1024) void k_bin_tilescan(FrameParams P, DeviceBuffers B) {
-    __shared__ uint32_t wsum[16];
-    const uint32_t per = (P.num_tiles + 1023) / 1024;
-    const uint32_t t0 = min(P.num_tiles, threadIdx.x * per), t1 = min(P.num_tiles, t0 + per);
-    uint32_t local = 0;
-    for (uint32_t t = t0; t < t1; t++) local += B.tile_totals[t];
+namespace {
+// Exclusive scan over the 1024 threads of a block (16 waves).
+__device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* wsum, uint32_t& total) {
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
-    uint32_t x = local;
+    uint32_t x = v;
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t y = __shfl_up(x, d);
         if (lane >= (uint32_t)d) x += y;
     }
     if (lane == kWave - 1) wsum[wave] = x;
     __syncthreads();
-    uint32_t base = 0, total = 0;
+    uint32_t base = 0;
+    total = 0;
     for (uint32_t w = 0; w < 16; w++) {
         if (w < wave) base += wsum[w];
         total += wsum[w];
     }
-    uint32_t run = base + x - local;
+    __syncthreads();
+    return base + x - v;
+}
+
+// Raster work items of a tile with n references (an empty tile has none).
+__device__ __forceinline__ uint32_t tile_segments(uint32_t n) { return (n + kSegRefs - 1) / kSegRefs; }
+}  // namespace
+
+// Exclusive scan of the tile totals (num_tiles <= 16384, one block of 1024),
+// and the raster work list: one item per (tile, segment of <= kSegRefs refs).
+__global__ __launch_bounds__(1024) void k_bin_tilescan(FrameParams P, DeviceBuffers B) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t per = (P.num_tiles + 1023) / 1024;
+    const uint32_t t0 = min(P.num_tiles, threadIdx.x * per), t1 = min(P.num_tiles, t0 + per);
+    uint32_t localRefs = 0, localSegs = 0;
     for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t n = B.tile_totals[t];
+        localRefs += n;
+        localSegs += tile_segments(n);
+    }
+    uint32_t total, totalSegs;
+    uint32_t run = scan1024(localRefs, wsum, total);
+    uint32_t seg = scan1024(localSegs, wsum, totalSegs);
+    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap;
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t n = B.tile_totals[t];
         B.tile_start[t] = run;
-        run += B.tile_totals[t];
+        run += n;
+        if (fits) {
+            const uint32_t ns = tile_segments(n);
+            B.tile_item[t] = seg;
+            for (uint32_t s = 0; s < ns; s++) B.work[seg + s] = t | (s << 14);
+            seg += ns;
+        }
     }
     if (threadIdx.x == 0) {
         B.tile_start[P.num_tiles] = total;
         B.counters[kCntTotalRefs] = total;
-        if (total > B.ref_cap) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvRefs);
+        B.counters[kCntWorkItems] = totalSegs;
+        if (!fits) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvRefs);
     }
 }
 
@@ -583,8 +709,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(FrameParams P, DeviceBuf
         const uint32_t desc = s < s1 ? B.desc[s] : 0u;
         RefEnum e;
         e.init(desc);
-        uint32_t R = 0, tile, payload;
-        while (e.next(P, B.rrec, tile, payload)) R++;
+        uint32_t R = s < s1 ? B.slot_refs[s] : 0u, tile, payload;
+        if (R == kSlotRefsSaturated) {
+            R = 0;
+            while (e.next(P, B.rrec, tile, payload)) R++;
+        }
         uint32_t total;
         const uint32_t base = block_exclusive_scan(R, wsum, total);
         e.init(desc);
@@ -643,7 +772,7 @@ struct RTri {
     int B0, C0, B1, C1, B2, C2;
     int pb0, pb1, pb2;   // pixel-level top-left bias (Core/RasterTriangle.h:296-299)
     float inv_det, z0, z1, z2;
-    uint32_t info;
+    uint32_t info, order;
 };
 
 __device__ __forceinline__ int pixel_bias(int ax, int ay, int bx, int by) {
@@ -661,6 +790,7 @@ __device__ __forceinline__ void load_rtri(const RasterRec& r, RTri& t) {
     t.inv_det = r.inv_det;
     t.z0 = r.z0; t.z1 = r.z1; t.z2 = r.z2;
     t.info = r.info;
+    t.order = r.order;
 }
 
 // Ordered depth key: (depth, submission order) min = the winner of the
@@ -680,26 +810,30 @@ __device__ __forceinline__ void bary_depth(const RTri& t, int px, int py, float&
     d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
 }
 
-// Depth-tests the 4 pixels of the quad at (qx, qy); edges tested when useEdges.
-__device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
-                                           uint32_t pos, uint64_t* keys) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int x = qx + (k & 1), y = qy + (k >> 1);
-        const int px = x * 16 + 8, py = y * 16 + 8;
-        if (useEdges) {
-            const int e0 = t.B0 * (px - t.x0) + t.C0 * (py - t.y0) + t.pb0;
-            const int e1 = t.B1 * (px - t.x1) + t.C1 * (py - t.y1) + t.pb1;
-            const int e2 = t.B2 * (px - t.x2) + t.C2 * (py - t.y2) + t.pb2;
-            if ((e0 | e1 | e2) < 0) continue;
-        }
-        float l0, l1, d;
-        bary_depth(t, px, py, l0, l1, d);
-        if (!(d <= 1.0f)) continue;   // never passes d <= D with D <= 1 (NaN included)
-        const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - pos);
-        uint64_t* slot = &keys[(y - tminY) * kTileSize + (x - tminX)];
-        if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+// Depth-tests pixel (x, y): edge test at the centre (16x+8, 16y+8) when
+// useEdges (Core/Rasterizer.h:145-162), then the (depth, order) key update.
+__device__ __forceinline__ void pixel_depth(const RTri& t, int x, int y, bool useEdges, int tminX, int tminY,
+                                            uint64_t* keys) {
+    const int px = x * 16 + 8, py = y * 16 + 8;
+    if (useEdges) {
+        const int e0 = t.B0 * (px - t.x0) + t.C0 * (py - t.y0) + t.pb0;
+        const int e1 = t.B1 * (px - t.x1) + t.C1 * (py - t.y1) + t.pb1;
+        const int e2 = t.B2 * (px - t.x2) + t.C2 * (py - t.y2) + t.pb2;
+        if ((e0 | e1 | e2) < 0) return;
     }
+    float l0, l1, d;
+    bary_depth(t, px, py, l0, l1, d);
+    if (!(d <= 1.0f)) return;   // never passes d <= D with D <= 1 (NaN included)
+    const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - t.order);
+    uint64_t* slot = &keys[(y - tminY) * kTileSize + (x - tminX)];
+    if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+}
+
+// The 4 pixels of the 2x2 quad at (qx, qy) (one SSE lane each in the reference).
+__device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
+                                           uint64_t* keys) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) pixel_depth(t, qx + (k & 1), qy + (k >> 1), useEdges, tminX, tminY, keys);
 }
 
 __device__ __forceinline__ float pow_decl(float x, int n, float s) {
@@ -719,34 +853,10 @@ __device__ __forceinline__ uint32_t to_byte(float c) {
     return (uint32_t)(int)(c * 255.0f + 0.5f);
 }
 
-struct VAttr {
-    float px, py, pz, nx, ny, nz, cr, cg, cb, inv_w;
-};
-
-__device__ __forceinline__ VAttr fetch_vertex(const FrameParams& P, const MeshPtrs& M, const ExtraVerts& X,
-                                              uint32_t v) {
-    VAttr a;
-    if (v < P.num_vertices) {
-        a.px = M.px[v]; a.py = M.py[v]; a.pz = M.pz[v];
-        a.nx = M.nx[v]; a.ny = M.ny[v]; a.nz = M.nz[v];
-        if (M.cr) { a.cr = M.cr[v]; a.cg = M.cg[v]; a.cb = M.cb[v]; }
-        else { a.cr = a.cg = a.cb = 0.0f; }
-        const float w = ((P.mvp[12] * a.px + P.mvp[13] * a.py) + P.mvp[14] * a.pz) + P.mvp[15] * 1.0f;
-        a.inv_w = 1.0f / w;
-    } else {
-        const uint32_t e = v - P.num_vertices;
-        a.px = X.px[e]; a.py = X.py[e]; a.pz = X.pz[e];
-        a.nx = X.nx[e]; a.ny = X.ny[e]; a.nz = X.nz[e];
-        a.cr = X.cr[e]; a.cg = X.cg[e]; a.cb = X.cb[e];
-        a.inv_w = X.inv_w[e];
-    }
-    return a;
-}
-
 // Fragment::Interpolate (Core/Shader.h:142-170) + pixel shader
 // (Core/Shader.h:209-282) + Color4b::FromFloats.
-__device__ __forceinline__ uint32_t shade_pixel(const FrameParams& P, float b0, float b1, const VAttr& v0,
-                                                const VAttr& v1, const VAttr& v2) {
+__device__ __forceinline__ uint32_t shade_pixel(const FrameParams& P, float b0, float b1, const ShadeVert& v0,
+                                                const ShadeVert& v1, const ShadeVert& v2) {
     float b2 = 1.0f - b0 - b1;
     b0 *= v0.inv_w;
     b1 *= v1.inv_w;
@@ -799,38 +909,160 @@ __device__ __forceinline__ uint32_t shade_pixel(const FrameParams& P, float b0, 
 
 }  // namespace
 
-__global__ __launch_bounds__(kBlock) void k_raster(FrameParams P, MeshPtrs M, DeviceBuffers B) {
-    __shared__ uint64_t keys[kTileSize * kTileSize];
-    __shared__ uint32_t bigq[kBlock];
-    __shared__ uint32_t nbig;
-    if (B.counters[kCntOverflow] & kOvRefs) return;
-    const uint32_t tile = blockIdx.x;
+namespace {
+
+// Large references for one wave: the lanes are the 8x8 quads of a 16x16-pixel
+// quadrant and the wave walks the tile's four quadrants.  Covers
+// TrivialAcceptTriangle over the tile, CoarseRasterize's stepped quadrant
+// tests and FineRasterize with the bbox clamp (Core/Rasterizer.h:32-200,310-353).
+__device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RTri& t, uint32_t pay,
+                                                  int tminX, int tminY, int tmaxX, int tmaxY, uint64_t* keys) {
+    const uint32_t lane = __lane_id();
+    const int lx = (int)(lane & 7u) * 2, ly = (int)(lane >> 3) * 2;
+    const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
+    const bool big = (pay >> 31) != 0;
+    if (trivial) {
+        // TrivialAcceptTriangle over the tile (Core/Rasterizer.h:310-353).
+        for (int q = 0; q < 4; q++) {
+            const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
+            if (qx <= tmaxX - 1 && qy <= tmaxY - 1) quad_depth(t, qx, qy, false, tminX, tminY, keys);
+        }
+    } else if (big && P.hierarchical) {
+        // CoarseRasterize (Core/Rasterizer.h:32-111): stepped corner tests per 16x16 quadrant.
+        const int baseX = tminX << 4, baseY = tminY << 4;
+        const int off = (kTileSize << 4) - 1, stepSize = kTileSize << 3;
+        const int sb0 = scalar_bias(t.x0, t.y0, t.x1, t.y1);
+        const int sb1 = scalar_bias(t.x1, t.y1, t.x2, t.y2);
+        const int sb2 = scalar_bias(t.x2, t.y2, t.x0, t.y0);
+        auto ef = [&](int e, uint32_t c) {
+            const int cxp = baseX + (int)(c & 1u) * off, cyp = baseY + (int)(c >> 1) * off;
+            if (e == 0) return t.B0 * (cxp - t.x0) + t.C0 * (cyp - t.y0) + sb0;
+            if (e == 1) return t.B1 * (cxp - t.x1) + t.C1 * (cyp - t.y1) + sb1;
+            return t.B2 * (cxp - t.x2) + t.C2 * (cyp - t.y2) + sb2;
+        };
+        // GenStepVectors (Core/RasterTriangle.h:192-255), lane q of the vector.
+        auto step = [&](uint32_t c, int Bk, int Ck, int q) {
+            int v;
+            if (c == 0) v = (q == 0) ? 0 : (q == 1) ? Bk : (q == 2) ? Ck : Bk + Ck;
+            else if (c == 1) v = (q == 0) ? -Bk : (q == 1) ? 0 : (q == 2) ? Ck - Bk : Ck;
+            else if (c == 2) v = (q == 0) ? -Ck : (q == 1) ? Bk - Ck : (q == 2) ? 0 : Bk;
+            else v = (q == 0) ? -Bk - Ck : (q == 1) ? -Ck : (q == 2) ? -Bk : 0;
+            return stepSize * v;
+        };
+        const uint32_t r0 = t.info & 3u, r1 = (t.info >> 2) & 3u, r2 = (t.info >> 4) & 3u;
+        const uint32_t a0 = (t.info >> 6) & 3u, a1 = (t.info >> 8) & 3u, a2 = (t.info >> 10) & 3u;
+        const int rj0 = ef(0, r0), rj1 = ef(1, r1), rj2 = ef(2, r2);
+        const bool ae0 = (pay >> kRecordBits) & 1u, ae1 = (pay >> (kRecordBits + 1)) & 1u,
+                   ae2 = (pay >> (kRecordBits + 2)) & 1u;
+        const int ac0 = ae0 ? 0 : ef(0, a0), ac1 = ae1 ? 0 : ef(1, a1), ac2 = ae2 ? 0 : ef(2, a2);
+        const int fminX = min(t.x0, min(t.x1, t.x2)) >> 4, fmaxX = max(t.x0, max(t.x1, t.x2)) >> 4;
+        const int fminY = min(t.y0, min(t.y1, t.y2)) >> 4, fmaxY = max(t.y0, max(t.y1, t.y2)) >> 4;
+        for (int q = 0; q < 4; q++) {
+            const bool rej = (rj0 + step(r0, t.B0, t.C0, q) < 0) || (rj1 + step(r1, t.B1, t.C1, q) < 0) ||
+                             (rj2 + step(r2, t.B2, t.C2, q) < 0);
+            if (rej) continue;
+            const bool acc = (ae0 || ac0 + step(a0, t.B0, t.C0, q) >= 0) &&
+                             (ae1 || ac1 + step(a1, t.B1, t.C1, q) >= 0) &&
+                             (ae2 || ac2 + step(a2, t.B2, t.C2, q) >= 0);
+            const int bminX = (q & 1) ? tminX + 16 : tminX, bmaxX = (q & 1) ? tmaxX : tminX + 16;
+            const int bminY = (q >> 1) ? tminY + 16 : tminY, bmaxY = (q >> 1) ? tmaxY : tminY + 16;
+            const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
+            if (acc) {
+                if (qx >= bminX && qx <= bmaxX - 1 && qy >= bminY && qy <= bmaxY - 1)
+                    quad_depth(t, qx, qy, false, tminX, tminY, keys);
+            } else {
+                int minX = max(bminX, fminX), minY = max(bminY, fminY);
+                const int maxX = min(bmaxX - 1, fmaxX), maxY = min(bmaxY - 1, fmaxY);
+                minX -= minX % 2;
+                minY -= minY % 2;
+                if (qx >= minX && qx <= maxX && qy >= minY && qy <= maxY)
+                    quad_depth(t, qx, qy, true, tminX, tminY, keys);
+            }
+        }
+    } else {
+        // FineRasterize over the tile with the bbox clamp: the lanes cover the
+        // triangle's quad bbox, 64 quads per pass.
+        int minX = max(tminX, min(t.x0, min(t.x1, t.x2)) >> 4);
+        const int maxX = min(tmaxX - 1, max(t.x0, max(t.x1, t.x2)) >> 4);
+        int minY = max(tminY, min(t.y0, min(t.y1, t.y2)) >> 4);
+        const int maxY = min(tmaxY - 1, max(t.y0, max(t.y1, t.y2)) >> 4);
+        minX -= minX % 2;
+        minY -= minY % 2;
+        if (maxX < minX || maxY < minY) return;
+        const int nqx = (maxX - minX) / 2 + 1, nq = nqx * ((maxY - minY) / 2 + 1);
+        for (int k = (int)lane; k - (int)lane < nq; k += kWave)
+            if (k < nq) quad_depth(t, minX + 2 * (k % nqx), minY + 2 * (k / nqx), true, tminX, tminY, keys);
+    }
+}
+
+// The record of lane l, made wave-uniform.
+__device__ __forceinline__ RTri broadcast_rtri(const RTri& t, int l) {
+    RTri u;
+    u.x0 = __builtin_amdgcn_readlane(t.x0, l);
+    u.y0 = __builtin_amdgcn_readlane(t.y0, l);
+    u.x1 = __builtin_amdgcn_readlane(t.x1, l);
+    u.y1 = __builtin_amdgcn_readlane(t.y1, l);
+    u.x2 = __builtin_amdgcn_readlane(t.x2, l);
+    u.y2 = __builtin_amdgcn_readlane(t.y2, l);
+    u.B0 = u.y0 - u.y1; u.C0 = u.x1 - u.x0;
+    u.B1 = u.y1 - u.y2; u.C1 = u.x2 - u.x1;
+    u.B2 = u.y2 - u.y0; u.C2 = u.x0 - u.x2;
+    u.pb0 = __builtin_amdgcn_readlane(t.pb0, l);
+    u.pb1 = __builtin_amdgcn_readlane(t.pb1, l);
+    u.pb2 = __builtin_amdgcn_readlane(t.pb2, l);
+    u.inv_det = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t.inv_det), l));
+    u.z0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t.z0), l));
+    u.z1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t.z1), l));
+    u.z2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t.z2), l));
+    u.info = __builtin_amdgcn_readlane(t.info, l);
+    u.order = __builtin_amdgcn_readlane(t.order, l);
+    return u;
+}
+
+// One raster work item: references [seg * kSegRefs, (seg + 1) * kSegRefs) of a
+// tile.  Each wave streams its own 64-reference slices through the tile's LDS
+// key buffer (no workgroup barrier per slice): small triangles take one lane
+// each, large ones the whole wave.  The item's keys go to partials[workIndex].
+__device__ __forceinline__ void raster_item(const FrameParams& P, const DeviceBuffers& B, uint32_t workIndex,
+                                            uint64_t* keys) {
+    const uint32_t workItem = B.work[workIndex];
+    const uint32_t tile = workItem & 0x3FFFu, seg = workItem >> 14;
     const int tx = (int)(tile % P.tiles_x), ty = (int)(tile / P.tiles_x);
     const int tminX = tx * kTileSize, tminY = ty * kTileSize;
     const int tmaxX = min(tminX + kTileSize, (int)P.width), tmaxY = min(tminY + kTileSize, (int)P.height);
     for (uint32_t i = threadIdx.x; i < kTileSize * kTileSize; i += kBlock) keys[i] = bg_key();
     const uint32_t start = B.tile_start[tile];
     const uint32_t n = B.tile_start[tile + 1] - start;
+    const uint32_t segEnd = min(n, (seg + 1) * kSegRefs);
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     __syncthreads();
 
-    for (uint32_t b = 0; b < n; b += kBlock) {
-        if (threadIdx.x == 0) nbig = 0;
-        __syncthreads();
-        const uint32_t p = b + threadIdx.x;
-        if (p < n) {
-            const uint32_t pay = B.refs[start + p];
-            const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
-            const bool big = (pay >> 31) != 0;
-            bool large = trivial || (big && P.hierarchical);
-            if (!large) {
-                RTri t;
-                load_rtri(B.rrec[pay & kRecordMask], t);
-                // FineRasterize_SingleSample bbox (Core/Rasterizer.h:132-140).
-                int minX = max(tminX, min(t.x0, min(t.x1, t.x2)) >> 4);
-                const int maxX = min(tmaxX - 1, max(t.x0, max(t.x1, t.x2)) >> 4);
-                int minY = max(tminY, min(t.y0, min(t.y1, t.y2)) >> 4);
-                const int maxY = min(tmaxY - 1, max(t.y0, max(t.y1, t.y2)) >> 4);
+    for (uint32_t b = seg * kSegRefs + wave * kWave; b < segEnd; b += kBlock) {
+        const uint32_t p = b + lane;
+        const bool valid = p < segEnd;
+        const uint32_t pay = valid ? B.refs[start + p] : 0u;
+        const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
+        const bool big = (pay >> 31) != 0;
+        bool large = valid && (trivial || (big && P.hierarchical));
+        // Every lane loads its own record; large ones are broadcast below.
+        RTri t;
+        if (valid) load_rtri(B.rrec[pay & kRecordMask], t);
+        if (valid && !large) {
+            const int bx0 = min(t.x0, min(t.x1, t.x2)) >> 4, bx1 = max(t.x0, max(t.x1, t.x2)) >> 4;
+            const int by0 = min(t.y0, min(t.y1, t.y2)) >> 4, by1 = max(t.y0, max(t.y1, t.y2)) >> 4;
+            // FineRasterize_SingleSample bbox (Core/Rasterizer.h:132-140).
+            int minX = max(tminX, bx0);
+            const int maxX = min(tmaxX - 1, bx1);
+            int minY = max(tminY, by0);
+            const int maxY = min(tmaxY - 1, by1);
+            if (bx1 - bx0 <= 1 && by1 - by0 <= 1) {
+                // Pixel bbox of at most 2x2: test only its pixels.  The
+                // reference's even-aligned quads add pixels whose centres lie
+                // outside the bbox, hence outside the triangle, and with edges
+                // this short no edge function can wrap.
+                for (int y = minY; y <= maxY; y++)
+                    for (int x = minX; x <= maxX; x++) pixel_depth(t, x, y, true, tminX, tminY, keys);
+            } else {
                 minX -= minX % 2;
                 minY -= minY % 2;
                 if (maxX >= minX && maxY >= minY) {
@@ -838,128 +1070,110 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams P, MeshPtrs M, De
                     if (nqx * nqy <= kSmallQuads) {
                         for (int qy = 0; qy < nqy; qy++)
                             for (int qx = 0; qx < nqx; qx++)
-                                quad_depth(t, minX + 2 * qx, minY + 2 * qy, true, tminX, tminY, p, keys);
+                                quad_depth(t, minX + 2 * qx, minY + 2 * qy, true, tminX, tminY, keys);
                     } else {
                         large = true;
                     }
                 }
             }
-            if (large) bigq[atomicAdd(&nbig, 1u)] = p;
         }
-        __syncthreads();
-        const uint32_t nb = nbig;
-        for (uint32_t e = wave; e < nb; e += kBlock / kWave) {
-            const uint32_t pp = __builtin_amdgcn_readfirstlane(bigq[e]);
-            const uint32_t pay = __builtin_amdgcn_readfirstlane(B.refs[start + pp]);
-            RTri t;
-            load_rtri(B.rrec[pay & kRecordMask], t);
-            const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
-            const bool big = (pay >> 31) != 0;
-            const int lx = (int)(lane & 7u) * 2, ly = (int)(lane >> 3) * 2;
-            if (trivial) {
-                // TrivialAcceptTriangle over the tile (Core/Rasterizer.h:310-353).
-                for (int q = 0; q < 4; q++) {
-                    const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
-                    if (qx <= tmaxX - 1 && qy <= tmaxY - 1) quad_depth(t, qx, qy, false, tminX, tminY, pp, keys);
-                }
-            } else if (big && P.hierarchical) {
-                // CoarseRasterize (Core/Rasterizer.h:32-111): stepped corner tests per 16x16 quadrant.
-                const int baseX = tminX << 4, baseY = tminY << 4;
-                const int off = (kTileSize << 4) - 1, stepSize = kTileSize << 3;
-                const int sb0 = scalar_bias(t.x0, t.y0, t.x1, t.y1);
-                const int sb1 = scalar_bias(t.x1, t.y1, t.x2, t.y2);
-                const int sb2 = scalar_bias(t.x2, t.y2, t.x0, t.y0);
-                auto ef = [&](int e, uint32_t c) {
-                    const int cxp = baseX + (int)(c & 1u) * off, cyp = baseY + (int)(c >> 1) * off;
-                    if (e == 0) return t.B0 * (cxp - t.x0) + t.C0 * (cyp - t.y0) + sb0;
-                    if (e == 1) return t.B1 * (cxp - t.x1) + t.C1 * (cyp - t.y1) + sb1;
-                    return t.B2 * (cxp - t.x2) + t.C2 * (cyp - t.y2) + sb2;
-                };
-                // GenStepVectors (Core/RasterTriangle.h:192-255), lane q of the vector.
-                auto step = [&](uint32_t c, int Bk, int Ck, int q) {
-                    int v;
-                    if (c == 0) v = (q == 0) ? 0 : (q == 1) ? Bk : (q == 2) ? Ck : Bk + Ck;
-                    else if (c == 1) v = (q == 0) ? -Bk : (q == 1) ? 0 : (q == 2) ? Ck - Bk : Ck;
-                    else if (c == 2) v = (q == 0) ? -Ck : (q == 1) ? Bk - Ck : (q == 2) ? 0 : Bk;
-                    else v = (q == 0) ? -Bk - Ck : (q == 1) ? -Ck : (q == 2) ? -Bk : 0;
-                    return stepSize * v;
-                };
-                const uint32_t r0 = t.info & 3u, r1 = (t.info >> 2) & 3u, r2 = (t.info >> 4) & 3u;
-                const uint32_t a0 = (t.info >> 6) & 3u, a1 = (t.info >> 8) & 3u, a2 = (t.info >> 10) & 3u;
-                const int rj0 = ef(0, r0), rj1 = ef(1, r1), rj2 = ef(2, r2);
-                const bool ae0 = (pay >> kRecordBits) & 1u, ae1 = (pay >> (kRecordBits + 1)) & 1u,
-                           ae2 = (pay >> (kRecordBits + 2)) & 1u;
-                const int ac0 = ae0 ? 0 : ef(0, a0), ac1 = ae1 ? 0 : ef(1, a1), ac2 = ae2 ? 0 : ef(2, a2);
-                const int fminX = min(t.x0, min(t.x1, t.x2)) >> 4, fmaxX = max(t.x0, max(t.x1, t.x2)) >> 4;
-                const int fminY = min(t.y0, min(t.y1, t.y2)) >> 4, fmaxY = max(t.y0, max(t.y1, t.y2)) >> 4;
-                for (int q = 0; q < 4; q++) {
-                    const bool rej = (rj0 + step(r0, t.B0, t.C0, q) < 0) || (rj1 + step(r1, t.B1, t.C1, q) < 0) ||
-                                     (rj2 + step(r2, t.B2, t.C2, q) < 0);
-                    if (rej) continue;
-                    const bool acc = (ae0 || ac0 + step(a0, t.B0, t.C0, q) >= 0) &&
-                                     (ae1 || ac1 + step(a1, t.B1, t.C1, q) >= 0) &&
-                                     (ae2 || ac2 + step(a2, t.B2, t.C2, q) >= 0);
-                    const int bminX = (q & 1) ? tminX + 16 : tminX, bmaxX = (q & 1) ? tmaxX : tminX + 16;
-                    const int bminY = (q >> 1) ? tminY + 16 : tminY, bmaxY = (q >> 1) ? tmaxY : tminY + 16;
-                    const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
-                    if (acc) {
-                        if (qx >= bminX && qx <= bmaxX - 1 && qy >= bminY && qy <= bmaxY - 1)
-                            quad_depth(t, qx, qy, false, tminX, tminY, pp, keys);
-                    } else {
-                        int minX = max(bminX, fminX), minY = max(bminY, fminY);
-                        const int maxX = min(bmaxX - 1, fmaxX), maxY = min(bmaxY - 1, fmaxY);
-                        minX -= minX % 2;
-                        minY -= minY % 2;
-                        if (qx >= minX && qx <= maxX && qy >= minY && qy <= maxY)
-                            quad_depth(t, qx, qy, true, tminX, tminY, pp, keys);
-                    }
-                }
-            } else {
-                // FineRasterize over the tile with the bbox clamp.
-                int minX = max(tminX, min(t.x0, min(t.x1, t.x2)) >> 4);
-                const int maxX = min(tmaxX - 1, max(t.x0, max(t.x1, t.x2)) >> 4);
-                int minY = max(tminY, min(t.y0, min(t.y1, t.y2)) >> 4);
-                const int maxY = min(tmaxY - 1, max(t.y0, max(t.y1, t.y2)) >> 4);
-                minX -= minX % 2;
-                minY -= minY % 2;
-                for (int q = 0; q < 4; q++) {
-                    const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
-                    if (qx >= minX && qx <= maxX && qy >= minY && qy <= maxY)
-                        quad_depth(t, qx, qy, true, tminX, tminY, pp, keys);
-                }
-            }
+        // Large references: the whole wave takes them one at a time, the
+        // owner lane's record broadcast with readlane (no memory round trip).
+        uint64_t lm = __ballot(large);
+        while (lm) {
+            const int l = __ffsll((unsigned long long)lm) - 1;
+            lm &= lm - 1;
+            const uint32_t ppay = __builtin_amdgcn_readlane(pay, l);
+            raster_large_wave(P, broadcast_rtri(t, l), ppay, tminX, tminY, tmaxX, tmaxY, keys);
         }
-        __syncthreads();
     }
+    __syncthreads();
+    uint64_t* g = B.partials + (size_t)workIndex * (kTileSize * kTileSize);
+    for (uint32_t i = threadIdx.x; i < kTileSize * kTileSize; i += kBlock) g[i] = keys[i];
+}
 
-    // Deferred shading of the winner and one write-back per pixel.
-    const int row = (int)(threadIdx.x >> 3), xg = (int)(threadIdx.x & 7u) * 4;
-    const int y = tminY + row;
-    if (y >= (int)P.height) return;
-    const ExtraVerts X = B.extra;
-    for (int i = 0; i < 4; i++) {
-        const int x = tminX + xg + i;
-        if (x >= (int)P.width) continue;
-        const uint64_t key = keys[row * kTileSize + xg + i];
-        const size_t o = (size_t)(P.height - 1 - y) * P.width + x;
-        if (key == bg_key()) {
-            B.color[o] = 0u;
-            B.depth[o] = 1.0f;
-            B.ids[o] = 0xFFFFFFFFu;
-            continue;
-        }
-        const uint32_t pos = 0xFFFFFFFEu - (uint32_t)key;
-        const uint32_t ri = B.refs[start + pos] & kRecordMask;
-        RTri t;
-        load_rtri(B.rrec[ri], t);
-        const ShadeRec s = B.srec[ri];
-        float l0, l1, d;
-        bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
-        const VAttr v0 = fetch_vertex(P, M, X, s.v0), v1 = fetch_vertex(P, M, X, s.v1), v2 = fetch_vertex(P, M, X, s.v2);
-        B.color[o] = shade_pixel(P, l0, l1, v0, v1, v2);
-        B.depth[o] = d;
-        B.ids[o] = s.tri_id;
+}  // namespace
+
+// Persistent raster kernel: workgroups pull (tile, segment) items from the
+// list built by k_bin_tilescan until it is exhausted.
+__global__ __launch_bounds__(kBlock) void k_raster(FrameParams P, DeviceBuffers B) {
+    __shared__ uint64_t keys[kTileSize * kTileSize];
+    __shared__ uint32_t sItem;
+    if (B.counters[kCntOverflow] & kOvRefs) return;
+    const uint32_t nItems = B.counters[kCntWorkItems];
+    for (;;) {
+        if (threadIdx.x == 0) sItem = atomicAdd(&B.counters[kCntWorkNext], 1u);
+        __syncthreads();
+        const uint32_t item = sItem;
+        if (item >= nItems) break;
+        raster_item(P, B, item, keys);
+        __syncthreads();
     }
+}
+
+// Deferred shading, one pixel per thread and one workgroup per tile: the
+// winner is the minimum (depth, order) key over the tile's segments (the
+// sequential d <= D test is an associative minimum), then perspective-correct
+// interpolation, the pixel shader and one write of colour, depth and
+// triangle ID (Core/Renderer.cpp:272-350).
+__global__ __launch_bounds__(1024) void k_shade(FrameParams P, MeshPtrs M, DeviceBuffers B) {
+    constexpr uint32_t kPix = kTileSize * kTileSize;
+    if (B.counters[kCntOverflow] & kOvRefs) return;
+    const uint32_t tile = blockIdx.x;
+    const int x = (int)(tile % P.tiles_x) * kTileSize + (int)(threadIdx.x & 31u);
+    const int y = (int)(tile / P.tiles_x) * kTileSize + (int)(threadIdx.x >> 5);
+    if (x >= (int)P.width || y >= (int)P.height) return;
+    const uint32_t start = B.tile_start[tile];
+    const uint32_t n = B.tile_start[tile + 1] - start;
+    uint64_t key = bg_key();
+    if (n) {
+        const uint32_t nseg = tile_segments(n);
+        const uint64_t* g = B.partials + (size_t)B.tile_item[tile] * kPix + threadIdx.x;
+        key = g[0];
+        uint32_t s = 1;
+        for (; s + 4 <= nseg; s += 4) {  // four independent loads in flight
+            const uint64_t a = g[(size_t)s * kPix], b = g[(size_t)(s + 1) * kPix];
+            const uint64_t c = g[(size_t)(s + 2) * kPix], e = g[(size_t)(s + 3) * kPix];
+            const uint64_t ab = a < b ? a : b, ce = c < e ? c : e;
+            const uint64_t m = ab < ce ? ab : ce;
+            key = m < key ? m : key;
+        }
+        for (; s < nseg; s++) {
+            const uint64_t v = g[(size_t)s * kPix];
+            key = v < key ? v : key;
+        }
+    }
+    const size_t o = (size_t)(P.height - 1 - y) * P.width + x;
+    if (key == bg_key()) {
+        B.color[o] = 0u;
+        B.depth[o] = 1.0f;
+        B.ids[o] = 0xFFFFFFFFu;
+        return;
+    }
+    // The key's low word is the winner's submission order (make_order).
+    const uint32_t order = 0xFFFFFFFEu - (uint32_t)key;
+    const uint32_t tri = order >> 4;
+    uint32_t v0i, v1i, v2i, ri;
+    if (order & 1u) {
+        ri = (B.desc[tri] >> 3) + ((order >> 1) & 7u);
+        const ShadeRec s = B.srec[ri];
+        v0i = s.v0;
+        v1i = s.v1;
+        v2i = s.v2;
+    } else {
+        ri = tri;
+        v0i = M.idx[3 * (size_t)tri];
+        v1i = M.idx[3 * (size_t)tri + 1];
+        v2i = M.idx[3 * (size_t)tri + 2];
+    }
+    RTri t;
+    load_rtri(B.rrec[ri], t);
+    float l0, l1, d;
+    bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
+    const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
+    B.color[o] = shade_pixel(P, l0, l1, v0, v1, v2);
+    B.depth[o] = d;
+    B.ids[o] = tri;
 }
 
 // ---------------------------------------------------------------------------
@@ -986,6 +1200,8 @@ hipError_t launch_frame(const FrameParams& P, const MeshPtrs& M, const DeviceBuf
     };
     mark(0);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, B.counters);
+    if (P.num_vertices)
+        hipLaunchKernelGGL(k_vertex, dim3((P.num_vertices + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, M, B.verts);
     if (P.num_triangles) {
         hipLaunchKernelGGL(k_setup, dim3((P.num_triangles + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, M, B);
         const uint32_t clipBlocks = max(1u, min(1024u, (B.clip_queue_cap + 63) / 64));
@@ -994,12 +1210,16 @@ hipError_t launch_frame(const FrameParams& P, const MeshPtrs& M, const DeviceBuf
     mark(1);
     hipLaunchKernelGGL(k_bin_count, dim3(P.num_chunks), dim3(kBlock), lds, s, P, B);
     mark(2);
-    hipLaunchKernelGGL(k_bin_colscan, dim3((P.num_tiles + kWave - 1) / kWave), dim3(kBlock), 0, s, P, B);
+    const dim3 colGrid((P.num_tiles + kWave - 1) / kWave, (P.num_chunks + kChunkGroup - 1) / kChunkGroup);
+    hipLaunchKernelGGL(k_bin_colsum, colGrid, dim3(kBlock), 0, s, P, B);
+    hipLaunchKernelGGL(k_bin_groupscan, dim3((P.num_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, B);
+    hipLaunchKernelGGL(k_bin_colscan, colGrid, dim3(kBlock), 0, s, P, B);
     hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, P, B);
     mark(3);
     hipLaunchKernelGGL(k_bin_scatter, dim3(P.num_chunks), dim3(kBlock), lds, s, P, B);
     mark(4);
-    hipLaunchKernelGGL(k_raster, dim3(P.num_tiles), dim3(kBlock), 0, s, P, M, B);
+    hipLaunchKernelGGL(k_raster, dim3(B.raster_blocks), dim3(kBlock), 0, s, P, B);
+    hipLaunchKernelGGL(k_shade, dim3(P.num_tiles), dim3(1024), 0, s, P, M, B);
     mark(5);
     return hipGetLastError();
 }

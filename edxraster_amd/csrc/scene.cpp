@@ -3,7 +3,7 @@
 // Stand-ins for Mesh::LoadSphere / Mesh::LoadPlane (EDXRaster/Utils/Mesh.cpp:36-70),
 // which delegate to EDXUtil's ObjMesh (not available).  Geometry is computed in
 // double and rounded once to fp32; random numbers come from SplitMix64 so every
-// consumer (HIP pipeline, CPU oracle, tests, benchmark) sees identical bytes.
+// consumer (HIP pipeline, CPU reference, tests, benchmark) sees identical bytes.
 // Winding: front faces are clockwise on screen (det > 0 in
 // RasterTriangle::Setup, EDXRaster/Core/RasterTriangle.h:49-51).
 #include <algorithm>

@@ -95,6 +95,26 @@ def test_shader_kinds_parity(built):
         r.close()
 
 
+FULL = [
+    ("grid1m_1920x1080", lambda: configs.sphere_grid_1080p()),
+    ("hf10m_3840x2160", lambda: configs.heightfield_4k()),
+]
+
+
+@pytest.mark.parametrize("name,make", FULL, ids=[w[0] for w in FULL])
+def test_full_size_parity(built, name, make):
+    """The benchmark workloads at full size against the oracle (16 host threads)."""
+    w = make()
+    r, o, ost = render_both(w, threads=16)
+    info = assert_frame_parity(r, o, name)
+    st = r.stats()
+    assert st["setup_records"] == ost["setup_records"]
+    assert st["refs"] == ost["refs"]
+    assert st["clipped"] == ost["clipped"] and st["near_clipped"] == ost["near_clipped"]
+    print(name, info, st)
+    r.close()
+
+
 def test_clipping_stats_and_parity(built):
     w = configs.sphere_grid_1080p(640, 360, grid=4, segments=24)
     r, o, ost = render_both(w)

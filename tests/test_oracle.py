@@ -1,0 +1,445 @@
+"""Pins the CPU oracle (the parity reference of the HIP pipeline).
+
+The reference ships no tests, fixtures or golden vectors, so the oracle is
+pinned by (1) an independent brute-force restatement (tests/brute.py), (2)
+known answers of the coverage rule, (3) FP64 closed forms of depth,
+perspective-correct interpolation, shading and clipping, and (4) binning
+invariants.  Each check is shown to catch a planted mistake (oracle
+`mutation` switch): 1 inverted top-left bias, 2 LESS instead of LEQUAL,
+3 affine interpolation, 4 transposed MVP product, 5 near-plane t sign,
+6 wrong edge in a barycentric.
+"""
+import numpy as np
+import pytest
+
+from edxraster_amd import configs, Mesh, look_at, perspective, raster_matrix
+from oracle import oracle as O
+import brute
+
+F = np.float32
+
+
+def mesh_from_pixels(tris, width, height, z=0.5, colors=None):
+    """Triangles given in raster pixel coordinates (y down) with identity MV and
+    projection; exact because every coordinate is a multiple of 1/16."""
+    pts = np.asarray(tris, np.float64).reshape(-1, 2)
+    n = pts.shape[0]
+    px = (pts[:, 0] * 2.0 / width - 1.0).astype(F)
+    py = (1.0 - pts[:, 1] * 2.0 / height).astype(F)
+    pz = np.full(n, z, F) if np.isscalar(z) else np.asarray(z, F)
+    nrm = np.zeros(n, F)
+    if colors is None:
+        colors = np.full((n, 3), 0.5, F)
+    colors = np.asarray(colors, F)
+    return Mesh(px, py, pz, nrm, nrm, np.full(n, -1.0, F), colors[:, 0], colors[:, 1], colors[:, 2],
+                np.arange(n, dtype=np.uint32))
+
+
+def identity_render(o, mesh, w, h, **kw):
+    I = np.eye(4, dtype=F)
+    return o.render(mesh, w, h, I, I, raster_matrix(w, h), **kw)
+
+
+def rand_scene(seed, ntri=40):
+    """Random triangles inside the frustum of a camera at (0,0,-4)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-0.9, 0.9, (ntri, 3))
+    c[:, 2] = rng.uniform(-0.8, 1.5, ntri)
+    v = c[:, None, :] + rng.normal(0, 0.35, (ntri, 3, 3))
+    v = v.reshape(-1, 3).astype(F)
+    n = rng.normal(0, 1, v.shape).astype(F)
+    col = rng.uniform(0, 1, v.shape).astype(F)
+    mesh = Mesh(v[:, 0], v[:, 1], v[:, 2], n[:, 0], n[:, 1], n[:, 2], col[:, 0], col[:, 1], col[:, 2],
+                np.arange(v.shape[0], dtype=np.uint32))
+    return mesh
+
+
+def scenes():
+    out = []
+    w, h = 64, 48
+    mv, p, r = configs.camera(w, h, (0.0, 0.0, -4.0), (0.0, 0.0, 0.0))
+    for seed in (1, 2, 3):
+        out.append(("random%d" % seed, rand_scene(seed), w, h, mv, p, r))
+    sph = Mesh.load_sphere(16, 8, 1.0, (0, 0, 0), color_seed=9)
+    mv, p, r = configs.camera(w, h, (0.0, 0.4, -3.0), (0.0, 0.0, 0.0))
+    out.append(("sphere16x8", sph, w, h, mv, p, r))
+    hf = Mesh.load_heightfield(10, 0.2, height_seed=4, color_seed=5)
+    mv, p, r = configs.camera(80, 72, (0.0, 2.0, -2.8), (0.0, 0.0, 0.0))
+    out.append(("heightfield10", hf, 80, 72, mv, p, r))
+    # Big triangles over several 32x32 tiles (tile-level accept, coarse path).
+    big = rand_scene(7, ntri=6)
+    for a in (big.px, big.py):
+        a *= F(1.6)
+    mv, p, r = configs.camera(96, 80, (0.0, 0.0, -6.0), (0.0, 0.0, 0.0))
+    out.append(("big", big, 96, 80, mv, p, r))
+    return out
+
+
+SCENES = scenes()
+
+
+def oracle_frame(mesh, w, h, mv, p, r, shader=2, **kw):
+    o = O.Oracle()
+    o.render(mesh, w, h, mv, p, r, shader=shader, shininess=32.0, **kw)
+    return o
+
+
+def eye_of(mv):
+    inv = np.zeros(16, F)
+    O.lib().oracle_mat_inverse(np.ascontiguousarray(mv, F).reshape(16).ctypes.data_as(O.F),
+                               inv.ctypes.data_as(O.F))
+    inv = inv.reshape(4, 4)
+    x, y, z, w = (((inv[k, 0] * F(0) + inv[k, 1] * F(0)) + inv[k, 2] * F(0)) + inv[k, 3] * F(1) for k in range(4))
+    return [x / w, y / w, z / w]
+
+
+# ---------------------------------------------------------------- brute force
+@pytest.mark.parametrize("name,mesh,w,h,mv,p,r", SCENES, ids=[s[0] for s in SCENES])
+def test_oracle_equals_brute_force(built, name, mesh, w, h, mv, p, r):
+    mvp = O.mat_mul(p, mv)
+    ids, D, L0, L1, s = brute.rasterize(mesh, w, h, mvp, r)
+    assert (ids != 0xFFFFFFFF).sum() > 20, "scene covers too few pixels"
+    col = brute.shade(mesh, ids, L0, L1, s, eye_of(mv), (1.0, 1.0, -1.0), 2, 32.0)
+    for hier in (1, 0):
+        for threads in (1, 3):
+            o = oracle_frame(mesh, w, h, mv, p, r, hierarchical=hier, threads=threads)
+            assert o.render(mesh, w, h, mv, p, r, shader=2, shininess=32.0, hierarchical=hier,
+                            threads=threads)["clipped"] == 0, "brute force covers unclipped scenes only"
+            np.testing.assert_array_equal(o.ids(), brute.to_bottom_up(ids))
+            np.testing.assert_array_equal(o.depth().view(np.uint32), brute.to_bottom_up(D).view(np.uint32))
+            np.testing.assert_array_equal(o.color(), brute.to_bottom_up(col))
+
+
+def test_brute_force_catches_transposed_mvp(built):
+    name, mesh, w, h, mv, p, r = SCENES[0]
+    ids, *_ = brute.rasterize(mesh, w, h, O.mat_mul(p, mv), r)
+    o = oracle_frame(mesh, w, h, mv, p, r, mutation=4)
+    assert not np.array_equal(o.ids(), brute.to_bottom_up(ids))
+
+
+def test_mat_mul_and_inverse(built):
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=(4, 4)).astype(F)
+    b = rng.normal(size=(4, 4)).astype(F)
+    ab = O.mat_mul(a, b)
+    np.testing.assert_allclose(ab, a.astype(np.float64) @ b.astype(np.float64), rtol=1e-5, atol=1e-5)
+    inv = np.zeros(16, F)
+    O.lib().oracle_mat_inverse(np.ascontiguousarray(a).reshape(16).ctypes.data_as(O.F), inv.ctypes.data_as(O.F))
+    np.testing.assert_allclose(inv.reshape(4, 4), np.linalg.inv(a.astype(np.float64)), rtol=1e-5, atol=1e-5)
+
+
+# ---------------------------------------------------------------- known answers
+RECT = [[(2.5, 1.5), (6.5, 1.5), (2.5, 4.5)], [(6.5, 1.5), (6.5, 4.5), (2.5, 4.5)]]
+
+
+def covered(o):
+    return o.ids()[::-1] != 0xFFFFFFFF  # back to raster rows
+
+
+def test_top_left_rule_known_answer(built):
+    """A rectangle whose edges pass through pixel centres: left/top edges are
+    inclusive, right/bottom exclusive (D3D top-left rule, RasterTriangle.h:296-299)."""
+    w = h = 64
+    expect = np.zeros((h, w), bool)
+    expect[1:4, 2:6] = True
+    o = O.Oracle()
+    identity_render(o, mesh_from_pixels(RECT, w, h), w, h)
+    np.testing.assert_array_equal(covered(o), expect)
+    # Each piece alone: disjoint, union = rectangle (watertight shared diagonal).
+    masks = []
+    for tri in RECT:
+        identity_render(o, mesh_from_pixels([tri], w, h), w, h)
+        masks.append(covered(o))
+    assert not (masks[0] & masks[1]).any()
+    np.testing.assert_array_equal(masks[0] | masks[1], expect)
+    # The planted inverted bias moves the covered set.
+    identity_render(o, mesh_from_pixels(RECT, w, h), w, h, mutation=1)
+    assert not np.array_equal(covered(o), expect)
+
+
+def test_pixel_aligned_rectangle(built):
+    w = h = 64
+    tris = [[(3, 5), (20, 5), (3, 17)], [(20, 5), (20, 17), (3, 17)]]
+    expect = np.zeros((h, w), bool)
+    expect[5:17, 3:20] = True
+    o = O.Oracle()
+    identity_render(o, mesh_from_pixels(tris, w, h), w, h)
+    np.testing.assert_array_equal(covered(o), expect)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_watertight_shared_edges(built, seed):
+    """Random fans around a centre: every centre inside the polygon is covered
+    by exactly one triangle."""
+    rng = np.random.default_rng(seed)
+    w = h = 96
+    c = rng.uniform(40, 56, 2)
+    k = 7
+    # Angles increase clockwise on the y-down screen, so (c, p_i, p_i+1) is front-facing.
+    ang = (np.arange(k) + rng.uniform(-0.3, 0.3, k)) * (2 * np.pi / k)
+    rad = rng.uniform(15, 40, k)
+    pts = np.stack([c[0] + rad * np.cos(ang), c[1] + rad * np.sin(ang)], 1)
+    pts = np.round(pts * 16) / 16
+    tris = [[tuple(c), tuple(pts[i]), tuple(pts[(i + 1) % k])] for i in range(k)]
+    o = O.Oracle()
+    count = np.zeros((h, w), int)
+    for t in tris:
+        identity_render(o, mesh_from_pixels([t], w, h), w, h)
+        count += covered(o)
+    identity_render(o, mesh_from_pixels(tris, w, h), w, h)
+    assert count.max() == 1
+    np.testing.assert_array_equal(covered(o), count == 1)
+
+
+def test_integer_translation_invariance(built):
+    w = h = 96
+    rng = np.random.default_rng(5)
+    tri = np.round(rng.uniform(10, 60, (3, 2)) * 16) / 16
+    e1, e2 = tri[1] - tri[0], tri[2] - tri[0]
+    tri = tri if e1[0] * e2[1] - e1[1] * e2[0] > 0 else tri[[0, 2, 1]]
+    o = O.Oracle()
+    identity_render(o, mesh_from_pixels([tri.tolist()], w, h), w, h)
+    base = covered(o)
+    assert base.sum() > 10
+    for dx, dy in ((3, 0), (0, 5), (7, 11)):
+        identity_render(o, mesh_from_pixels([(tri + [dx, dy]).tolist()], w, h), w, h)
+        np.testing.assert_array_equal(covered(o)[dy:, dx:], base[:h - dy, :w - dx])
+
+
+def test_depth_tie_goes_to_later_triangle(built):
+    """Overlapping triangles at z = 0 (every interpolated depth is exactly 0):
+    d <= D lets the later one win (FrameBuffer.cpp:64); LESS (planted) keeps
+    the first."""
+    w = h = 64
+    t0 = [(4, 4), (40, 4), (4, 40)]
+    t1 = [(10, 6), (44, 6), (10, 44)]
+    mesh = mesh_from_pixels([t0, t1], w, h, z=0.0)
+    o = O.Oracle()
+    identity_render(o, mesh, w, h)
+    ids = o.ids()[::-1]
+    assert (ids[10:20, 12:20] == 1).all()
+    identity_render(o, mesh, w, h, mutation=2)
+    ids2 = o.ids()[::-1]
+    assert (ids2[10:20, 12:20] == 0).all()
+
+
+def test_depth_one_wins_and_beyond_loses(built):
+    """D clears to 1.0 and the test is d <= D (FrameBuffer.cpp:64,103): inside
+    the triangle a centre is drawn iff its interpolated depth is <= 1.0; at
+    z = 1 many centres interpolate to exactly 1.0 and are drawn."""
+    w = h = 32
+    mesh = mesh_from_pixels([[(2, 2), (28, 2), (2, 28)]], w, h, z=1.0)
+    o = O.Oracle()
+    identity_render(o, mesh, w, h)
+    I = np.eye(4, dtype=F)
+    ids, D, *_ = brute.rasterize(mesh, w, h, I, raster_matrix(w, h))
+    s = brute.setup(mesh, I, raster_matrix(w, h))
+    ys, xs = np.mgrid[0:h, 0:w]
+    px, py = xs * 16 + 8, ys * 16 + 8
+    X, Y, B, C = s["X"][0], s["Y"][0], [b[0] for b in s["B"]], [c[0] for c in s["C"]]
+    l0 = brute.wrap32(B[1] * (px - X[2]) + C[1] * (py - Y[2])).astype(F) * s["inv_det"][0]
+    l1 = brute.wrap32(B[2] * (px - X[2]) + C[2] * (py - Y[2])).astype(F) * s["inv_det"][0]
+    d = (l0 * F(1) + l1 * F(1)) + ((F(1) - l0) - l1) * F(1)
+    inside = np.ones((h, w), bool)
+    for k in range(3):
+        a, c = k, (k + 1) % 3
+        bias = -1 if (Y[c] > Y[a]) or (Y[a] == Y[c] and X[a] > X[c]) else 0
+        inside &= brute.wrap32(B[k] * (px - X[a]) + C[k] * (py - Y[a]) + bias) >= 0
+    assert (inside & (d == 1)).any()
+    np.testing.assert_array_equal(covered(o), inside & (d <= 1))
+    # z = 1 + 2^-23 is rejected by the far-plane outcode (z > w) and never drawn.
+    identity_render(o, mesh_from_pixels([[(2, 2), (20, 2), (2, 20)]], w, h, z=1.0000001), w, h)
+    assert covered(o).sum() == 0
+
+
+def test_culling_of_back_faces_and_degenerates(built):
+    w = h = 64
+    o = O.Oracle()
+    st = identity_render(o, mesh_from_pixels([[(4, 4), (4, 30), (30, 4)]], w, h), w, h)
+    assert st["culled"] == 1 and covered(o).sum() == 0
+    st = identity_render(o, mesh_from_pixels([[(4, 4), (20, 20), (36, 36)]], w, h), w, h)
+    assert st["culled"] == 1
+
+
+# ---------------------------------------------------------------- FP64 closed forms
+def fp64_reference(o, mesh, mv, p, r):
+    """Per covered pixel: FP64 depth, perspective-correct attributes and colour
+    of the oracle's winner (visibility taken from the oracle)."""
+    ids = o.ids()[::-1].astype(np.int64)
+    H, W = ids.shape
+    m = ids != 0xFFFFFFFF
+    ys, xs = np.nonzero(m)
+    t = ids[m]
+    mvp = O.mat_mul(p, mv).astype(np.float64)
+    idx = mesh.indices.reshape(-1, 3).astype(np.int64)[t]
+    P = np.stack([mesh.px, mesh.py, mesh.pz, np.ones_like(mesh.px)], 1).astype(np.float64)
+    clip = P @ mvp.T
+    rr = np.asarray(r, np.float64)
+    ndc = clip[:, :3] / clip[:, 3:4]
+    scr = np.concatenate([ndc, np.ones((ndc.shape[0], 1))], 1) @ rr.T
+    X = np.trunc(scr[:, 0].astype(F) * 16).astype(np.float64)[idx]
+    Y = np.trunc(scr[:, 1].astype(F) * 16).astype(np.float64)[idx]
+    px, py = xs * 16.0 + 8, ys * 16.0 + 8
+    B1, C1 = Y[:, 1] - Y[:, 2], X[:, 2] - X[:, 1]
+    B2, C2 = Y[:, 2] - Y[:, 0], X[:, 0] - X[:, 2]
+    det = C2 * B1 - C1 * B2
+    l0 = (B1 * (px - X[:, 2]) + C1 * (py - Y[:, 2])) / det
+    l1 = (B2 * (px - X[:, 2]) + C2 * (py - Y[:, 2])) / det
+    l2 = 1.0 - l0 - l1
+    z = (clip[:, 2] / clip[:, 3])[idx]
+    d = l0 * z[:, 0] + l1 * z[:, 1] + l2 * z[:, 2]
+    iw = (1.0 / clip[:, 3])[idx]
+    b = np.stack([l0 * iw[:, 0], l1 * iw[:, 1], l2 * iw[:, 2]], 1)
+    b /= b.sum(1, keepdims=True)
+    attr = lambda a: (b * np.asarray(a, np.float64)[idx]).sum(1)
+    return {"ys": ys, "xs": xs, "depth": d, "b": b, "attr": attr, "P": [attr(mesh.px), attr(mesh.py), attr(mesh.pz)],
+            "N": [attr(mesh.nx), attr(mesh.ny), attr(mesh.nz)], "C": [attr(mesh.cr), attr(mesh.cg), attr(mesh.cb)]}
+
+
+def fp64_colour(ref, eye, light, shininess, ambient=0.2, kd=3.0, ks=3.0):
+    N = np.stack(ref["N"], 1)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    L = np.asarray(light, np.float64) / np.linalg.norm(light)
+    V = np.asarray(eye, np.float64)[None] - np.stack(ref["P"], 1)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    Hh = L[None] + V
+    Hh /= np.linalg.norm(Hh, axis=1, keepdims=True)
+    diffuse = (np.maximum(N @ L, 0) + ambient) * kd / np.pi
+    spec = ((N * Hh).sum(1)) ** shininess * ks
+    c = diffuse[:, None] * np.stack(ref["C"], 1) + spec[:, None]
+    return np.trunc(np.clip(c, 0, 1) * 255 + 0.5)
+
+
+@pytest.mark.parametrize("name,mesh,w,h,mv,p,r", SCENES, ids=[s[0] for s in SCENES])
+def test_depth_and_shading_fp64(built, name, mesh, w, h, mv, p, r):
+    o = oracle_frame(mesh, w, h, mv, p, r)
+    ref = fp64_reference(o, mesh, mv, p, r)
+    d32 = o.depth()[::-1][ref["ys"], ref["xs"]].astype(np.float64)
+    np.testing.assert_allclose(d32, ref["depth"], rtol=0, atol=2e-6)
+    c = fp64_colour(ref, eye_of(mv), (1, 1, -1), 32.0)
+    got = o.color()[::-1][ref["ys"], ref["xs"], :3].astype(np.float64)
+    assert np.abs(got - c).max() <= 1.0
+
+
+def test_fp64_depth_catches_wrong_barycentric(built):
+    name, mesh, w, h, mv, p, r = SCENES[3]
+    o = oracle_frame(mesh, w, h, mv, p, r, mutation=6)
+    ref = fp64_reference(o, mesh, mv, p, r)
+    d32 = o.depth()[::-1][ref["ys"], ref["xs"]].astype(np.float64)
+    assert np.abs(d32 - ref["depth"]).max() > 1e-4
+
+
+def floor_scene(w=96, h=72, near_cross=False):
+    """A two-triangle floor receding in depth, colours 0 near and 1 far (strong
+    perspective), optionally crossing the near plane."""
+    z0 = -3.0 if near_cross else 0.5
+    v = np.array([[-3, 0, z0], [3, 0, z0], [-3, 0, 30], [3, 0, 30]], F)
+    col = np.array([[0, 0, 0], [0, 0, 0], [1, 1, 1], [1, 1, 1]], F)
+    n = np.tile(np.array([0, 1, 0], F), (4, 1))
+    mesh = Mesh(v[:, 0], v[:, 1], v[:, 2], n[:, 0], n[:, 1], n[:, 2], col[:, 0], col[:, 1], col[:, 2],
+                np.array([0, 2, 1, 1, 2, 3], np.uint32))
+    if near_cross:  # eye 0.03 above the floor: the near-plane rectangle cuts it
+        mv, p, r = configs.camera(w, h, (0.0, 0.03, -1.0), (0.0, 0.02, 10.0))
+    else:
+        mv, p, r = configs.camera(w, h, (0.0, 1.0, -2.0), (0.0, 0.0, 6.0))
+    return mesh, w, h, mv, p, r
+
+
+def test_perspective_interpolation_fp64_and_affine_mutation(built):
+    mesh, w, h, mv, p, r = floor_scene()
+    o = oracle_frame(mesh, w, h, mv, p, r)
+    ref = fp64_reference(o, mesh, mv, p, r)
+    assert len(ref["ys"]) > 500
+    got = o.color()[::-1][ref["ys"], ref["xs"], :3].astype(np.float64)
+    c = fp64_colour(ref, eye_of(mv), (1, 1, -1), 32.0)
+    assert np.abs(got - c).max() <= 1.0
+    o3 = oracle_frame(mesh, w, h, mv, p, r, mutation=3)
+    got3 = o3.color()[::-1][ref["ys"], ref["xs"], :3].astype(np.float64)
+    assert np.abs(got3 - c).max() > 10.0
+
+
+def polygon_coverage_fp64(mesh, w, h, mv, p, r):
+    """Pixels whose centres lie inside the floor clipped in FP64 against z >= 0."""
+    mvp = O.mat_mul(p, mv).astype(np.float64)
+    P = np.stack([mesh.px, mesh.py, mesh.pz, np.ones_like(mesh.px)], 1).astype(np.float64) @ mvp.T
+    quad = P[[0, 1, 3, 2]]
+    out = []
+    for i in range(4):
+        a, b = quad[i], quad[(i + 1) % 4]
+        if a[2] >= 0:
+            out.append(a)
+        if (a[2] >= 0) != (b[2] >= 0):
+            t = a[2] / (a[2] - b[2])
+            out.append(a + (b - a) * t)
+    poly = np.array(out)
+    ndc = poly[:, :2] / poly[:, 3:4]
+    rr = np.asarray(r, np.float64)
+    sx = rr[0, 0] * ndc[:, 0] + rr[0, 3]
+    sy = rr[1, 1] * ndc[:, 1] + rr[1, 3]
+    ys, xs = np.mgrid[0:h, 0:w] + 0.5
+    inside = np.ones((h, w), bool)
+    n = len(sx)
+    area = 0.0
+    for i in range(n):
+        j = (i + 1) % n
+        area += sx[i] * sy[j] - sx[j] * sy[i]
+    sgn = np.sign(area)
+    for i in range(n):
+        j = (i + 1) % n
+        e = (sx[j] - sx[i]) * (ys - sy[i]) - (sy[j] - sy[i]) * (xs - sx[i])
+        inside &= sgn * e > -1e-9
+    return inside
+
+
+def test_near_plane_clipping_fp64_and_mutation(built):
+    mesh, w, h, mv, p, r = floor_scene(near_cross=True)
+    o = oracle_frame(mesh, w, h, mv, p, r)
+    st_near = o.render(mesh, w, h, mv, p, r)
+    assert st_near["near_clipped"] >= 1
+    exp = polygon_coverage_fp64(mesh, w, h, mv, p, r)
+    got = covered(o)
+    # Only pixels whose centres are within one pixel of the polygon boundary may differ.
+    diff = exp ^ got
+    assert diff.sum() <= 0.02 * exp.sum() + 2 * (w + h)
+    o5 = oracle_frame(mesh, w, h, mv, p, r, mutation=5)
+    assert (exp ^ covered(o5)).sum() > 4 * diff.sum() + 50
+
+
+# ---------------------------------------------------------------- binning
+@pytest.mark.parametrize("name,mesh,w,h,mv,p,r", SCENES, ids=[s[0] for s in SCENES])
+def test_binning_invariants(built, name, mesh, w, h, mv, p, r):
+    o = oracle_frame(mesh, w, h, mv, p, r)
+    offs, refs = o.bins()
+    tx = (w + 31) // 32
+    for t in range(len(offs) - 1):
+        lst = refs[offs[t]:offs[t + 1]]
+        key = lst[:, 0].astype(np.int64) * 8 + lst[:, 1]
+        assert (np.diff(key) > 0).all(), "per-tile order must be ascending submission order"
+    ids = o.ids()[::-1]
+    ys, xs = np.nonzero(ids != 0xFFFFFFFF)
+    for y, x in zip(ys, xs):
+        t = (y // 32) * tx + x // 32
+        assert ids[y, x] in set(refs[offs[t]:offs[t + 1], 0].tolist())
+
+
+# ---------------------------------------------------------------- scenes
+def test_scene_sizes_and_winding(built):
+    s = Mesh.load_sphere(64, 32, 1.0, (0, 0, 0), color_seed=1)
+    assert (s.num_vertices, s.num_triangles) == (2145, 4096)
+    hf = Mesh.load_heightfield(11, 0.2, 3, 33)
+    assert (hf.num_vertices, hf.num_triangles) == (121, 200)
+    g = Mesh.sphere_grid(2, 8, 2)
+    assert (g.num_vertices, g.num_triangles) == (4 * 81, 4 * 128)
+    # Front faces face the camera: every visible sphere triangle's geometric
+    # normal (v1 - v0) x (v2 - v0) points outward and toward the eye.
+    w = configs.small_sphere(200, 150)
+    o = O.Oracle()
+    o.render_workload(w)
+    ids = np.unique(o.ids())
+    ids = ids[ids != 0xFFFFFFFF].astype(np.int64)
+    v = w.mesh.positions().astype(np.float64)[w.mesh.indices.reshape(-1, 3)[ids]]
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    cen = v.mean(1)
+    to_eye = np.array([0.0, 0.0, -3.0]) - cen
+    assert len(ids) > 100
+    assert ((n * cen).sum(1) > 0).all() and ((n * to_eye).sum(1) > 0).all()
