@@ -1,0 +1,219 @@
+/*
+ * edxraster.hpp -- header-only C++ facade over the C ABI (edxraster.h) that
+ * re-creates the reference's host API so EDXRaster-style code keeps its shape:
+ *
+ *   EDX::RasterRenderer::Renderer     Core/Renderer.h:15-59
+ *   EDX::RasterRenderer::PixelShader  Core/Shader.h:173-282 (shader objects)
+ *   EDX::RasterRenderer::Mesh         Utils/Mesh.h:17-69 (SoA instead of AoS)
+ *   EDX::Matrix                       EDXUtil's Matrix (row-major, column vectors)
+ *
+ * The reference reports no errors; this facade throws std::runtime_error with
+ * edxr_last_error() text when the C ABI returns a negative status.
+ */
+#ifndef EDXRASTER_HPP
+#define EDXRASTER_HPP
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "edxraster.h"
+
+namespace EDX {
+
+/* Row-major 4x4, column-vector convention: out_r = sum_c m[r][c] * v_c. */
+struct Matrix {
+    float m[4][4];
+    static Matrix Identity() {
+        Matrix r{};
+        for (int i = 0; i < 4; i++) r.m[i][i] = 1.0f;
+        return r;
+    }
+    static Matrix LookAt(const float eye[3], const float at[3], const float up[3]) {
+        Matrix r;
+        edxr_look_at(eye, at, up, &r.m[0][0]);
+        return r;
+    }
+    static Matrix Perspective(float fovYDeg, float aspect, float znear, float zfar) {
+        Matrix r;
+        edxr_perspective(fovYDeg, aspect, znear, zfar, &r.m[0][0]);
+        return r;
+    }
+    static Matrix ToRaster(uint32_t width, uint32_t height) {
+        Matrix r;
+        edxr_raster_matrix(width, height, &r.m[0][0]);
+        return r;
+    }
+};
+
+namespace RasterRenderer {
+
+/* Pixel shader objects (Core/Shader.h:173-282) as shading descriptors. */
+class PixelShader {
+  public:
+    virtual ~PixelShader() {}
+    int32_t Kind() const { return kind_; }
+    edxr_material Material() const { return material_; }
+
+  protected:
+    explicit PixelShader(int32_t kind) : kind_(kind) { edxr_default_material(kind, &material_); }
+    int32_t kind_;
+    edxr_material material_;
+};
+
+/* LambertianAlbedoPixelShader with a constant texture (Core/Shader.h:209-244). */
+class LambertianAlbedoPixelShader : public PixelShader {
+  public:
+    explicit LambertianAlbedoPixelShader(float r = 0.9f, float g = 0.9f, float b = 0.9f)
+        : PixelShader(EDXR_SHADER_LAMBERT_ALBEDO) {
+        material_.albedo[0] = r;
+        material_.albedo[1] = g;
+        material_.albedo[2] = b;
+    }
+};
+
+/* BlinnPhongPixelShader (Core/Shader.h:246-282). */
+class BlinnPhongPixelShader : public PixelShader {
+  public:
+    BlinnPhongPixelShader() : PixelShader(EDXR_SHADER_BLINN_PHONG_REF) {}
+};
+
+/* Vertex-colour Blinn-Phong: diffuse * interpolated RGB + specular. */
+class VertexColorBlinnPhongPixelShader : public PixelShader {
+  public:
+    explicit VertexColorBlinnPhongPixelShader(float shininess = 32.0f) : PixelShader(EDXR_SHADER_VC_BLINN_PHONG) {
+        material_.shininess = shininess;
+    }
+};
+
+/* Mesh (Utils/Mesh.h:17-69): SoA fp32 positions, normals, colours, uint32 indices. */
+class Mesh {
+  public:
+    Mesh() { std::memset(&m_, 0, sizeof(m_)); }
+    ~Mesh() { Release(); }
+    Mesh(const Mesh&) = delete;
+    Mesh& operator=(const Mesh&) = delete;
+
+    /* Mesh::LoadSphere (Utils/Mesh.cpp:53-70); colours seeded. */
+    void LoadSphere(const float center[3], float radius, int slices = 64, int stacks = 64, uint64_t colorSeed = 1) {
+        Release();
+        check(edxr_scene_sphere(&m_, (uint32_t)slices, (uint32_t)stacks, radius, center, colorSeed));
+    }
+    /* A seeded noise heightfield on [-1,1]^2 (stands in for Mesh::LoadPlane). */
+    void LoadHeightfield(uint32_t n, float amplitude, uint64_t heightSeed, uint64_t colorSeed) {
+        Release();
+        check(edxr_scene_heightfield(&m_, n, amplitude, heightSeed, colorSeed));
+    }
+    void Release() { edxr_scene_free(&m_); }
+
+    uint32_t GetVertexCount() const { return m_.num_vertices; }
+    uint32_t GetTriangleCount() const { return m_.num_triangles; }
+    edxr_mesh View() const {
+        edxr_mesh v;
+        v.px = m_.px; v.py = m_.py; v.pz = m_.pz;
+        v.nx = m_.nx; v.ny = m_.ny; v.nz = m_.nz;
+        v.cr = m_.cr; v.cg = m_.cg; v.cb = m_.cb;
+        v.indices = m_.indices;
+        v.num_vertices = m_.num_vertices;
+        v.num_triangles = m_.num_triangles;
+        v.memory = EDXR_MEM_HOST;
+        return v;
+    }
+
+  private:
+    static void check(int rc) {
+        if (rc != EDXR_OK) throw std::runtime_error("edxraster: mesh generation failed (" + std::to_string(rc) + ")");
+    }
+    edxr_host_mesh m_;
+};
+
+/* Renderer (Core/Renderer.h:15-59) on one MI355X. */
+class Renderer {
+  public:
+    Renderer() = default;
+    ~Renderer() {
+        if (ctx_) edxr_destroy(ctx_);
+    }
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+
+    void Initialize(uint32_t width, uint32_t height) {
+        if (ctx_) edxr_destroy(ctx_);
+        ctx_ = nullptr;
+        const int rc = edxr_create(&ctx_, width, height);
+        if (rc != EDXR_OK) throw std::runtime_error("edxraster: edxr_create failed (" + std::to_string(rc) + ")");
+        width_ = width;
+        height_ = height;
+    }
+    void Resize(uint32_t width, uint32_t height) {
+        check(edxr_resize(ctx_, width, height));
+        width_ = width;
+        height_ = height;
+    }
+    void SetTransform(const Matrix& modelView, const Matrix& proj, const Matrix& toRaster) {
+        check(edxr_set_transform(ctx_, &modelView.m[0][0], &proj.m[0][0], &toRaster.m[0][0]));
+    }
+    void SetLight(float x, float y, float z) {
+        const float d[3] = {x, y, z};
+        check(edxr_set_light(ctx_, d));
+    }
+    void SetPixelShader(const PixelShader& ps) {
+        const edxr_material m = ps.Material();
+        check(edxr_set_pixel_shader(ctx_, ps.Kind(), &m));
+    }
+    void SetHierarchicalRasterize(bool on) { check(edxr_set_state(ctx_, EDXR_STATE_HIERARCHICAL, on ? 1 : 0)); }
+    /* Single-sample only: any other level is rejected. */
+    void SetMSAAMode(int sampleCountLog2) {
+        if (sampleCountLog2 != 0) throw std::runtime_error("edxraster: MSAA is not implemented");
+    }
+    void RenderMesh(const Mesh& mesh) {
+        const edxr_mesh v = mesh.View();
+        check(edxr_render_mesh(ctx_, &v));
+        backBufferValid_ = false;
+    }
+    void RenderMesh(const edxr_mesh& mesh) {
+        check(edxr_render_mesh(ctx_, &mesh));
+        backBufferValid_ = false;
+    }
+    /* Renderer::GetBackBuffer: RGBA8, W x H, bottom-up rows; valid until the
+     * next RenderMesh or Resize. */
+    const uint8_t* GetBackBuffer() {
+        if (!backBufferValid_) {
+            backBuffer_.resize((size_t)width_ * height_ * 4);
+            check(edxr_read_color(ctx_, backBuffer_.data(), backBuffer_.size()));
+            backBufferValid_ = true;
+        }
+        return backBuffer_.data();
+    }
+    void ReadDepth(std::vector<float>& out) {
+        out.resize((size_t)width_ * height_);
+        check(edxr_read_depth(ctx_, out.data(), out.size()));
+    }
+    void ReadTriangleIds(std::vector<uint32_t>& out) {
+        out.resize((size_t)width_ * height_);
+        check(edxr_read_triangle_ids(ctx_, out.data(), out.size()));
+    }
+    edxr_stats GetStats() {
+        edxr_stats s;
+        check(edxr_get_stats(ctx_, &s));
+        return s;
+    }
+    uint32_t GetWidth() const { return width_; }
+    uint32_t GetHeight() const { return height_; }
+
+  private:
+    void check(int rc) {
+        if (rc != EDXR_OK) throw std::runtime_error(std::string("edxraster: ") + edxr_last_error(ctx_));
+    }
+    edxr_ctx* ctx_ = nullptr;
+    uint32_t width_ = 0, height_ = 0;
+    std::vector<uint8_t> backBuffer_;
+    bool backBufferValid_ = false;
+};
+
+}  // namespace RasterRenderer
+}  // namespace EDX
+
+#endif
