@@ -25,6 +25,9 @@
  *                          clamp is fminf(fmaxf(c, 0), 1), so NaN gives 0
  *   IntSSE(EDX_INFINITY)   INT_MAX
  *   BoolSSE true           all ones (-1 when added to an IntSSE)
+ * Parity unpinned: these declared EDXUtil semantics (and the LoadSphere /
+ * LoadPlane stand-in topology) have no check against EDXUtil itself, which is
+ * absent; the tests pin them only against closed forms and brute force.
  * Additions beyond the reference: a triangle-ID buffer (input triangle index,
  * clipped pieces inherit it, background 0xFFFFFFFF), a vertex-colour
  * Blinn-Phong shader and a settable light direction.  Pixels outside the
