@@ -556,7 +556,8 @@ int64_t edxr_debug_read_records(edxr_ctx* c, uint32_t* out, uint64_t cap) {
     if (e != hipSuccess) return c->hip_fail(e, "debug_read_records");
     uint64_t n = 0;
     for (uint32_t t = 0; t < T; t++) {
-        const uint32_t first = desc[t] >> 3, cnt = desc[t] & 7u;
+        const bool compact = (desc[t] & kDescCompact) != 0;
+        const uint32_t first = compact ? t : desc[t] >> 3, cnt = compact ? 1u : desc[t] & 7u;
         for (uint32_t k = 0; k < cnt; k++, n++) {
             if (n >= cap || !out) continue;
             const RasterRec& r = rr[first + k];

@@ -18,6 +18,10 @@ constexpr uint32_t kSegRefs = 2048;          // tile references per raster work 
 constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
 constexpr uint32_t kMaxChunks = 4096;
+// Per-triangle descriptor: bit 31 set = compact small triangle (record in slot
+// t; tile bbox minTX | minTY << 14 | dx << 28 | dy << 29, dx, dy in {0, 1});
+// otherwise (first record << 3) | record count (0..7).
+constexpr uint32_t kDescCompact = 1u << 31;
 
 enum ShaderKind { kLambertAlbedo = 0, kBlinnPhongRef = 1, kVcBlinnPhong = 2 };
 
@@ -113,7 +117,7 @@ enum OverflowBit {
 struct DeviceBuffers {
     RasterRec* rrec;
     ShadeRec* srec;
-    uint32_t* desc;        // per input triangle: (first record << 3) | count
+    uint32_t* desc;        // per input triangle, see kDescCompact
     uint32_t* clip_queue;
     uint32_t clip_queue_cap;
     uint32_t extra_rec_cap;

@@ -64,14 +64,17 @@ __device__ __forceinline__ int snap(float a) {
     return (int)s;
 }
 
-// Trivial reject/accept corner choice (Core/RasterTriangle.h:66-150).
+// Trivial reject/accept corner choice (Core/RasterTriangle.h:66-150).  The
+// reference branches on slope = float(B) / float(C) >= 0; int->float keeps the
+// sign and the quotient of nonzero ints cannot underflow, so the test is a sign
+// test: C == 0 gives +-inf (or NaN for B == 0), B == 0 with C != 0 gives +-0.
 __device__ __forceinline__ uint32_t corner_pair(int B, int C) {
-    const float slope = (float)B / (float)C;
+    const bool slopeNonNeg = (C == 0) ? (B > 0) : (B == 0 || ((B > 0) == (C > 0)));
     uint32_t rej, acc;
-    if (slope >= 0.0f) {
-        if ((float)C >= 0.0f) { rej = 3; acc = 0; } else { rej = 0; acc = 3; }
+    if (slopeNonNeg) {
+        if (C >= 0) { rej = 3; acc = 0; } else { rej = 0; acc = 3; }
     } else {
-        if ((float)C >= 0.0f) { rej = 2; acc = 1; } else { rej = 1; acc = 2; }
+        if (C >= 0) { rej = 2; acc = 1; } else { rej = 1; acc = 2; }
     }
     return rej | (acc << 8);
 }
@@ -82,9 +85,18 @@ __device__ __forceinline__ bool setup_record(const FrameParams& P, float ax, flo
     const float4 A = xform4(P.raster, ax, ay, az, 1.0f);
     const float4 Bv = xform4(P.raster, bx, by, bz, 1.0f);
     const float4 Cv = xform4(P.raster, cx, cy, cz, 1.0f);
-    const int x0 = snap(A.x / A.w), y0 = snap(A.y / A.w);
-    const int x1 = snap(Bv.x / Bv.w), y1 = snap(Bv.y / Bv.w);
-    const int x2 = snap(Cv.x / Cv.w), y2 = snap(Cv.y / Cv.w);
+    int x0, y0, x1, y1, x2, y2;
+    // An affine raster matrix gives w == 1 and v / 1 == v exactly; the wave
+    // skips the six divides only when every lane has w == 1.
+    if (__all(A.w == 1.0f && Bv.w == 1.0f && Cv.w == 1.0f)) {
+        x0 = snap(A.x); y0 = snap(A.y);
+        x1 = snap(Bv.x); y1 = snap(Bv.y);
+        x2 = snap(Cv.x); y2 = snap(Cv.y);
+    } else {
+        x0 = snap(A.x / A.w); y0 = snap(A.y / A.w);
+        x1 = snap(Bv.x / Bv.w); y1 = snap(Bv.y / Bv.w);
+        x2 = snap(Cv.x / Cv.w); y2 = snap(Cv.y / Cv.w);
+    }
     const int B0 = y0 - y1, C0 = x1 - x0, B1 = y1 - y2, C1 = x2 - x1, B2 = y2 - y0, C2 = x0 - x2;
     const int det = C2 * B1 - C1 * B2;
     if (det <= 0) return false;
@@ -138,6 +150,21 @@ namespace {
 __device__ __forceinline__ float4 vertex_pos(const ShadeVert* verts, uint32_t v) {
     return *reinterpret_cast<const float4*>(&verts[v].px);  // (px, py, pz, nx)
 }
+
+// Descriptor of an unclipped triangle's record in slot t.  When its clamped
+// tile bbox spans at most 2x2 tiles, binning lists it in every bbox tile with
+// no corner test (Core/Renderer.cpp:168-180), so the bbox alone (packed here)
+// enumerates its references; otherwise (t << 3) | 1.
+__device__ __forceinline__ uint32_t compact_desc(const FrameParams& P, const RasterRec& r, uint32_t t) {
+    const int Shift = kTileLog2 + 4;
+    const int minTX = max(0, min(r.x0, min(r.x1, r.x2)) >> Shift);
+    const int maxTX = min((int)P.tiles_x - 1, max(r.x0, max(r.x1, r.x2)) >> Shift);
+    const int minTY = max(0, min(r.y0, min(r.y1, r.y2)) >> Shift);
+    const int maxTY = min((int)P.tiles_y - 1, max(r.y0, max(r.y1, r.y2)) >> Shift);
+    const int dx = maxTX - minTX, dy = maxTY - minTY;
+    if (dx < 0 || dy < 0 || dx > 1 || dy > 1) return (t << 3) | 1u;
+    return kDescCompact | (uint32_t)minTX | ((uint32_t)minTY << 14) | ((uint32_t)dx << 28) | ((uint32_t)dy << 29);
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -188,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(FrameParams P, MeshPtrs M, Dev
         if (main) {
             rec.order = make_order(t, 0, false);
             B.rrec[t] = rec;
-            d = (t << 3) | 1u;
+            d = compact_desc(P, rec, t);
         } else if (queued) {
             const uint32_t slot = qbase + (uint32_t)__popcll(qmask & lanemask_lt());
             if (slot < B.clip_queue_cap) B.clip_queue[slot] = t;
@@ -474,12 +501,25 @@ struct RefEnum {
     bool loaded;
     BinTri b;
 
-    __device__ __forceinline__ void init(uint32_t desc) {
-        first = desc >> 3;
-        count = desc & 7u;
+    // desc of slot s: either a compact small-triangle descriptor (the tile
+    // bbox; the record is slot s) or (first record << 3) | count.
+    __device__ __forceinline__ void init(uint32_t desc, uint32_t s) {
         r = 0;
         k = 0;
-        loaded = false;
+        if (desc & kDescCompact) {
+            first = s;
+            count = 1;
+            loaded = true;
+            b.minTX = (int)(desc & 0x3FFFu);
+            b.minTY = (int)((desc >> 14) & 0x3FFFu);
+            b.w = 1 + (int)((desc >> 28) & 1u);
+            b.h = 1 + (int)((desc >> 29) & 1u);
+            b.small = true;
+        } else {
+            first = desc >> 3;
+            count = desc & 7u;
+            loaded = false;
+        }
     }
     // Returns false when exhausted.
     __device__ __forceinline__ bool next(const FrameParams& P, const RasterRec* rrec, uint32_t& tile,
@@ -544,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(FrameParams P, DeviceBuffe
     const uint32_t s1 = min(P.num_triangles, s0 + P.chunk_slots);
     for (uint32_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
         RefEnum e;
-        e.init(B.desc[s]);
+        e.init(B.desc[s], s);
         uint32_t tile, payload, R = 0;
         while (e.next(P, B.rrec, tile, payload)) {
             atomicAdd(&hist[tile], 1u);
@@ -708,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(FrameParams P, DeviceBuf
         const uint32_t s = rb + threadIdx.x;
         const uint32_t desc = s < s1 ? B.desc[s] : 0u;
         RefEnum e;
-        e.init(desc);
+        e.init(desc, s);
         uint32_t R = s < s1 ? B.slot_refs[s] : 0u, tile, payload;
         if (R == kSlotRefsSaturated) {
             R = 0;
@@ -716,7 +756,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(FrameParams P, DeviceBuf
         }
         uint32_t total;
         const uint32_t base = block_exclusive_scan(R, wsum, total);
-        e.init(desc);
+        e.init(desc, s);
         for (uint32_t win = 0; win < total; win += kBlock) {
             // Emit this thread's refs that fall in [win, win + 256).
             const uint32_t lo = max(base, win), hi = min(base + R, win + kBlock);
