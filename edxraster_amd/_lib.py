@@ -24,6 +24,9 @@ SHADER_VC_BLINN_PHONG = 2
 STATE_HIERARCHICAL = 1
 STATE_BIN_CHUNKS = 2
 STATE_PROFILE = 3
+STATE_GRAPH = 4
+STATE_TRIS_PER_CHUNK = 5
+STATE_RASTER_BLOCKS_PER_CU = 6
 STAGES = ("setup", "bin_count", "bin_scan", "bin_scatter", "raster")
 
 MEM_HOST = 0
@@ -48,6 +51,17 @@ class Stats(C.Structure):
     _fields_ = [("triangles_in", C.c_uint64), ("trivially_rejected", C.c_uint64), ("clipped", C.c_uint64),
                 ("near_clipped", C.c_uint64), ("setup_records", C.c_uint64), ("refs", C.c_uint64),
                 ("clip_vertices", C.c_uint64), ("retries", C.c_uint32), ("bin_chunks", C.c_uint32)]
+
+
+class Job(C.Structure):
+    _fields_ = [("mesh", Mesh), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("mv", C.c_float * 16), ("proj", C.c_float * 16), ("raster", C.c_float * 16),
+                ("light", C.c_float * 3), ("shader", C.c_int32), ("material", Material),
+                ("hierarchical", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tris_per_chunk", C.c_uint32), ("raster_blocks_per_cu", C.c_uint32), ("frame_ms", C.c_double)]
 
 
 class HostMesh(C.Structure):
@@ -77,6 +91,11 @@ SIGNATURES = [
     ("edxr_profile_reset", C.c_int, [C.c_void_p]),
     ("edxr_stream", C.c_void_p, [C.c_void_p]),
     ("edxr_last_error", C.c_char_p, [C.c_void_p]),
+    ("edxr_render_batch", C.c_int, [C.c_void_p, C.POINTER(Job), C.c_uint32]),
+    ("edxr_batch_read", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("edxr_batch_device_buffers", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                            C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
+    ("edxr_autotune", C.c_int, [C.c_void_p, C.POINTER(Job), C.c_uint32, C.POINTER(Tuning)]),
     ("edxr_debug_read_records", C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("edxr_debug_read_bins", C.c_int64, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("edxr_scene_sphere", C.c_int, [C.POINTER(HostMesh), C.c_uint32, C.c_uint32, C.c_float, F, C.c_uint64]),

@@ -35,6 +35,12 @@ class Workload:
         m.shininess = self.shininess
         return m
 
+    def job(self, mesh=None, hierarchical=True):
+        """edxr_job of this workload for a batch (mesh: Mesh or edxr_mesh)."""
+        from .renderer import Renderer
+        return Renderer.make_job(self.mesh if mesh is None else mesh, self.width, self.height, self.mv, self.proj,
+                                 self.raster, self.light, self.shader, self.material(), hierarchical)
+
     def apply(self, renderer):
         renderer.set_transform(self.mv, self.proj, self.raster)
         renderer.set_light(self.light)
@@ -82,6 +88,39 @@ def heightfield_small(n=64, width=320, height=240, seed=7):
     mesh = Mesh.load_heightfield(n, 0.2, height_seed=seed, color_seed=seed + 30)
     mv, p, r = camera(width, height, (0.0, 1.2, -1.6), (0.0, 0.0, 0.0))
     return Workload("heightfield_%d_%dx%d" % (n, width, height), width, height, mesh, mv, p, r)
+
+
+def _unit(rng):
+    v = rng.normal(size=3)
+    return v / np.linalg.norm(v)
+
+
+def batch_jobs(n_jobs=256, seed_base=4000, tmin=1e3, tmax=5e5, resolutions=(256, 512, 1024)):
+    """Independent render jobs of uneven sizes.  Job j draws from a PCG64
+    stream seeded seed_base + j: a UV sphere (r = 1, slices = 2s, stacks = s,
+    T = 4 s^2) or an n x n heightfield (T = 2 (n-1)^2) with p = 0.5, the target
+    triangle count log-uniform in [tmin, tmax], a square resolution from
+    `resolutions`, the eye uniform on the radius-3 sphere looking at the
+    origin and the light direction uniform on S^2."""
+    jobs = []
+    for j in range(n_jobs):
+        rng = np.random.Generator(np.random.PCG64(seed_base + j))
+        sphere = rng.random() < 0.5
+        target = float(np.exp(rng.uniform(np.log(tmin), np.log(tmax))))
+        res = int(resolutions[rng.integers(len(resolutions))])
+        d = _unit(rng)
+        light = tuple(float(x) for x in _unit(rng))
+        if sphere:
+            s = max(2, int(round(np.sqrt(target / 4.0))))
+            mesh = Mesh.load_sphere(2 * s, s, 1.0, (0.0, 0.0, 0.0), color_seed=seed_base + j)
+        else:
+            n = max(3, int(round(1.0 + np.sqrt(target / 2.0))))
+            mesh = Mesh.load_heightfield(n, 0.2, height_seed=seed_base + j, color_seed=seed_base + 1000 + j)
+        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.99 else (0.0, 0.0, 1.0)
+        mv, p, r = camera(res, res, 3.0 * d, (0.0, 0.0, 0.0), up)
+        jobs.append(Workload("job%d_%s_%dx%d" % (j, "sphere" if sphere else "heightfield", res, res), res, res,
+                             mesh, mv, p, r, light=light, info={"seed": seed_base + j}))
+    return jobs
 
 
 WORKLOADS = {

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -103,6 +104,45 @@ struct DevBuf {
 
 bool is_integer_exponent(float s) { return s >= 0.0f && s <= 4096.0f && s == std::floor(s); }
 
+// Pinned host staging buffer (hipHostMalloc), grown on demand.
+template <class T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault);
+        if (e == hipSuccess) cap = std::max<size_t>(n, 1);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Host description of one job before placement in the batch arrays.
+struct JobInput {
+    MeshPtrs mesh;
+    uint32_t V, T, W, H;
+    float mvp[16], raster[16], eye[3], light[3];
+    edxr_material mat;
+    int32_t shader, hierarchical;
+};
+
+// A batch laid out for launch_frame: job table, batch sizes, block -> job
+// tables (vertex blocks | setup blocks | chunks | tiles) and buffer sizes.
+struct Plan {
+    std::vector<FrameParams> jobs;
+    BatchParams Q{};
+    std::vector<uint32_t> tables;
+    uint64_t countEntries = 0, groupEntries = 0, pixels = 0;
+};
+
 }  // namespace
 
 struct edxr_ctx {
@@ -116,7 +156,9 @@ struct edxr_ctx {
     edxr_material mat;
     int32_t hierarchical = 1;
     uint32_t binChunksSetting = 0;
+    uint32_t trisPerChunk = 0, rasterPerCU = 0;
     int32_t profile = 0;
+    int32_t useGraph = 0;
     hipEvent_t events[kNumStages + 1] = {};
     double stageMs[kNumStages] = {};
     uint64_t profiledFrames = 0;
@@ -124,6 +166,7 @@ struct edxr_ctx {
     // owned copies of host meshes
     DevBuf<float> mesh[9];
     DevBuf<uint32_t> meshIdx;
+    DevBuf<uint8_t> batchMesh;
     // pipeline buffers
     DevBuf<RasterRec> rrec;
     DevBuf<ShadeRec> srec;
@@ -132,12 +175,27 @@ struct edxr_ctx {
     DevBuf<uint64_t> partials;
     DevBuf<uint16_t> slotRefs;
     DevBuf<uint32_t> groupSums;
+    DevBuf<ShadeVert> verts;  // shading vertices + extraVertCap clip-created ones
+    DevBuf<FrameParams> jobsDev;
+    DevBuf<uint32_t> jobTables;
+    HostBuf<FrameParams> jobsHost;
+    HostBuf<uint32_t> tablesHost;
+    // batch outputs
+    DevBuf<uint32_t> bcolor, bids;
+    DevBuf<float> bdepth;
+    std::vector<uint64_t> batchPix;
+    std::vector<uint32_t> batchW, batchH;
+    bool batchRendered = false;
+
     uint32_t numCUs = 256;
-    DevBuf<ShadeVert> verts;  // V input + extraVertCap clip-created shading vertices
     uint32_t clipQueueCap = 0, extraRecCap = 0, extraVertCap = 0, refCap = 0;
     uint32_t* hostCounters = nullptr;
 
-    // last frame
+    // captured frame
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<uint8_t> graphKey;
+
+    // last single frame
     bool rendered = false;
     uint32_t lastT = 0, lastV = 0, lastChunks = 0;
     MeshPtrs lastMesh{};
@@ -150,16 +208,22 @@ struct edxr_ctx {
     }
     int hip_fail(hipError_t e, const char* what) {
         err = std::string(what) + ": " + hipGetErrorString(e);
-        return EDXR_E_HIP;
+        return e == hipErrorOutOfMemory ? EDXR_E_NOMEM : EDXR_E_HIP;
     }
 };
 
 namespace {
 
+bool tile_grid(uint32_t w, uint32_t h, uint32_t& tx, uint32_t& ty) {
+    if (w == 0 || h == 0 || w > kMaxDim || h > kMaxDim) return false;
+    tx = (w + kTileSize - 1) >> kTileLog2;
+    ty = (h + kTileSize - 1) >> kTileLog2;
+    return (uint64_t)tx * ty <= kMaxTiles;
+}
+
 int set_size(edxr_ctx* c, uint32_t w, uint32_t h) {
-    if (w == 0 || h == 0 || w > kMaxDim || h > kMaxDim) return c->fail(EDXR_E_LIMIT, "resolution out of range");
-    const uint32_t tx = (w + kTileSize - 1) >> kTileLog2, ty = (h + kTileSize - 1) >> kTileLog2;
-    if ((uint64_t)tx * ty > kMaxTiles) return c->fail(EDXR_E_LIMIT, "more than 16384 tiles");
+    uint32_t tx, ty;
+    if (!tile_grid(w, h, tx, ty)) return c->fail(EDXR_E_LIMIT, "resolution out of range (<= 8192, <= 16384 tiles)");
     c->W = w;
     c->H = h;
     c->tilesX = tx;
@@ -168,21 +232,347 @@ int set_size(edxr_ctx* c, uint32_t w, uint32_t h) {
     const size_t P = (size_t)w * h;
     hipError_t e;
     if ((e = c->color.ensure(P)) != hipSuccess || (e = c->depth.ensure(P)) != hipSuccess ||
-        (e = c->ids.ensure(P)) != hipSuccess || (e = c->tileTotals.ensure(c->nTiles)) != hipSuccess ||
-        (e = c->tileStart.ensure(c->nTiles + 1)) != hipSuccess ||
-        (e = c->tileItem.ensure(c->nTiles)) != hipSuccess)
+        (e = c->ids.ensure(P)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(frame buffers)");
     c->rendered = false;
     return EDXR_OK;
 }
 
-// Binning chunks: about 1024 triangles each so the count/scatter grids fill
-// the GPU, capped so the chunk x tile count matrix stays within 64 MB.
-uint32_t choose_chunks(const edxr_ctx* c, uint32_t T) {
-    if (c->binChunksSetting) return std::max(1u, std::min(std::min(c->binChunksSetting, kMaxChunks), std::max(1u, T)));
-    const uint32_t byMatrix = (uint32_t)std::max<uint64_t>(1, (64ull << 20) / (4ull * c->nTiles));
-    const uint32_t C = (T + 1023) / 1024;
+// Binning chunks of a job: about trisPerChunk (1024) triangles each so the
+// count/scatter grids fill the GPU, capped so the job's chunk x tile count
+// matrix stays within 64 MB.
+uint32_t choose_chunks(const edxr_ctx* c, uint32_t T, uint32_t nTiles, bool single) {
+    if (T == 0) return 0;
+    if (single && c->binChunksSetting) return std::max(1u, std::min(std::min(c->binChunksSetting, kMaxChunks), T));
+    const uint32_t tpc = c->trisPerChunk ? c->trisPerChunk : 1024;
+    const uint32_t byMatrix = (uint32_t)std::max<uint64_t>(1, (64ull << 20) / (4ull * nTiles));
+    const uint32_t C = (T + tpc - 1) / tpc;
     return std::max(1u, std::min(std::min(C, byMatrix), kMaxChunks));
+}
+
+int check_mesh(edxr_ctx* c, const edxr_mesh* m, int32_t shader) {
+    if (!m->px || !m->py || !m->pz || (m->num_triangles && !m->indices))
+        return c->fail(EDXR_E_INVALID, "mesh positions/indices missing");
+    if (!m->nx || !m->ny || !m->nz) return c->fail(EDXR_E_INVALID, "mesh normals missing");
+    if (shader == EDXR_SHADER_VC_BLINN_PHONG && (!m->cr || !m->cg || !m->cb))
+        return c->fail(EDXR_E_INVALID, "vertex colours required by EDXR_SHADER_VC_BLINN_PHONG");
+    if (m->memory != EDXR_MEM_HOST && m->memory != EDXR_MEM_DEVICE) return c->fail(EDXR_E_INVALID, "bad mesh memory");
+    return EDXR_OK;
+}
+
+// Per-job constants; the light is normalised on the host as Math::Normalize
+// in the pixel shader does (Core/Shader.h:221).
+FrameParams job_params(const JobInput& in, uint32_t tilesX, uint32_t tilesY) {
+    FrameParams P;
+    memset(&P, 0, sizeof(P));
+    memcpy(P.mvp, in.mvp, 64);
+    memcpy(P.raster, in.raster, 64);
+    memcpy(P.eye, in.eye, 12);
+    const float* L = in.light;
+    const float s = 1.0f / sqrtf((L[0] * L[0] + L[1] * L[1]) + L[2] * L[2]);
+    P.light[0] = L[0] * s;
+    P.light[1] = L[1] * s;
+    P.light[2] = L[2] * s;
+    P.ambient = in.mat.ambient;
+    P.kd = in.mat.kd;
+    P.ks = in.mat.ks;
+    P.shininess = in.mat.shininess;
+    memcpy(P.albedo, in.mat.albedo, 12);
+    P.shader = in.shader;
+    P.shin_int = is_integer_exponent(in.mat.shininess) ? (int32_t)in.mat.shininess : -1;
+    P.clamp_spec = in.mat.clamp_specular;
+    P.hierarchical = in.hierarchical;
+    P.width = in.W;
+    P.height = in.H;
+    P.tiles_x = tilesX;
+    P.tiles_y = tilesY;
+    P.num_tiles = tilesX * tilesY;
+    P.num_vertices = in.V;
+    P.num_triangles = in.T;
+    P.mesh = in.mesh;
+    return P;
+}
+
+// Places the jobs in the batch arrays.  Vertex and triangle slots start on
+// kBlock boundaries so each block of k_vertex / k_setup belongs to one job.
+int build_plan(edxr_ctx* c, const std::vector<JobInput>& in, Plan& pl) {
+    const bool single = in.size() == 1;
+    pl.jobs.clear();
+    pl.tables.clear();
+    uint64_t vert = 0, slot = 0, tile = 0, chunk = 0;
+    pl.countEntries = pl.groupEntries = pl.pixels = 0;
+    BatchParams& Q = pl.Q;
+    Q = BatchParams{};
+    for (const JobInput& j : in) {
+        uint32_t tx, ty;
+        if (!tile_grid(j.W, j.H, tx, ty)) return c->fail(EDXR_E_LIMIT, "job resolution out of range");
+        FrameParams P = job_params(j, tx, ty);
+        vert = (vert + kJobAlign - 1) / kJobAlign * kJobAlign;
+        slot = (slot + kJobAlign - 1) / kJobAlign * kJobAlign;
+        P.vert_base = (uint32_t)vert;
+        P.slot_base = (uint32_t)slot;
+        P.tile_base = (uint32_t)tile;
+        P.chunk_base = (uint32_t)chunk;
+        P.num_chunks = choose_chunks(c, j.T, P.num_tiles, single);
+        P.chunk_slots = P.num_chunks ? (j.T + P.num_chunks - 1) / P.num_chunks : 0;
+        P.count_base = pl.countEntries;
+        P.group_base = pl.groupEntries;
+        P.pix_base = pl.pixels;
+        vert += j.V;
+        slot += j.T;
+        tile += P.num_tiles;
+        chunk += P.num_chunks;
+        pl.countEntries += (uint64_t)P.num_chunks * P.num_tiles;
+        pl.groupEntries += (uint64_t)((P.num_chunks + kChunkGroup - 1) / kChunkGroup) * P.num_tiles;
+        pl.pixels += (uint64_t)j.W * j.H;
+        Q.max_job_tiles = std::max(Q.max_job_tiles, P.num_tiles);
+        Q.max_job_chunks = std::max(Q.max_job_chunks, P.num_chunks);
+        pl.jobs.push_back(P);
+        if (vert > 0xF0000000ull || slot + 65536 > kRecordMask) return c->fail(EDXR_E_LIMIT, "too many triangles");
+    }
+    if (tile > kMaxBatchTiles) return c->fail(EDXR_E_LIMIT, "batch exceeds 262144 tiles");
+    Q.num_jobs = (uint32_t)in.size();
+    Q.total_verts = (uint32_t)vert;
+    Q.total_slots = (uint32_t)slot;
+    Q.total_tiles = (uint32_t)tile;
+    Q.total_chunks = (uint32_t)chunk;
+    Q.vert_blocks = (uint32_t)((vert + kJobAlign - 1) / kJobAlign);
+    Q.slot_blocks = (uint32_t)((slot + kJobAlign - 1) / kJobAlign);
+    if (!single) {
+        pl.tables.resize((size_t)Q.vert_blocks + Q.slot_blocks + Q.total_chunks + Q.total_tiles);
+        uint32_t* vb = pl.tables.data();
+        uint32_t* sb = vb + Q.vert_blocks;
+        uint32_t* cj = sb + Q.slot_blocks;
+        uint32_t* tj = cj + Q.total_chunks;
+        for (uint32_t k = 0; k < Q.num_jobs; k++) {
+            const FrameParams& P = pl.jobs[k];
+            for (uint32_t b = P.vert_base / kJobAlign; b < (P.vert_base + P.num_vertices + kJobAlign - 1) / kJobAlign; b++)
+                vb[b] = k;
+            for (uint32_t b = P.slot_base / kJobAlign; b < (P.slot_base + P.num_triangles + kJobAlign - 1) / kJobAlign; b++)
+                sb[b] = k;
+            for (uint32_t i = 0; i < P.num_chunks; i++) cj[P.chunk_base + i] = k;
+            for (uint32_t i = 0; i < P.num_tiles; i++) tj[P.tile_base + i] = k;
+        }
+    }
+    return EDXR_OK;
+}
+
+// Bytes of the launch layout: a captured graph is replayed while they repeat.
+std::vector<uint8_t> launch_key(const BatchParams& Q, const DeviceBuffers& B, const FrameParams* jobs) {
+    std::vector<uint8_t> k(sizeof(Q) + sizeof(B) + sizeof(jobs));
+    memcpy(k.data(), &Q, sizeof(Q));
+    memcpy(k.data() + sizeof(Q), &B, sizeof(B));
+    memcpy(k.data() + sizeof(Q) + sizeof(B), &jobs, sizeof(jobs));
+    return k;
+}
+
+void drop_graph(edxr_ctx* c) {
+    if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
+    c->graphExec = nullptr;
+    c->graphKey.clear();
+}
+
+// Runs a planned batch (sized, launched, checked; re-run with larger buffers
+// after a capacity overflow).  For a single job the job table is written by
+// the reset kernel, otherwise it is uploaded from pinned memory.
+int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids) {
+    hipError_t e;
+    BatchParams& Q = pl.Q;
+    const bool single = Q.num_jobs == 1;
+    const bool graph = c->useGraph != 0;
+    if ((e = c->jobsDev.ensure(Q.num_jobs)) != hipSuccess ||
+        (e = c->tileTotals.ensure(Q.total_tiles)) != hipSuccess ||
+        (e = c->tileStart.ensure((size_t)Q.total_tiles + 1)) != hipSuccess ||
+        (e = c->tileItem.ensure(Q.total_tiles)) != hipSuccess)
+        return c->hip_fail(e, "hipMalloc(job tables)");
+    if (!single || graph) {
+        if ((e = c->jobsHost.ensure(Q.num_jobs)) != hipSuccess) return c->hip_fail(e, "hipHostMalloc(jobs)");
+        memcpy(c->jobsHost.p, pl.jobs.data(), sizeof(FrameParams) * Q.num_jobs);
+        if ((e = hipMemcpyAsync(c->jobsDev.p, c->jobsHost.p, sizeof(FrameParams) * Q.num_jobs,
+                                hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+            return c->hip_fail(e, "hipMemcpyAsync(jobs)");
+    }
+    if (!pl.tables.empty()) {
+        if ((e = c->jobTables.ensure(pl.tables.size())) != hipSuccess ||
+            (e = c->tablesHost.ensure(pl.tables.size())) != hipSuccess)
+            return c->hip_fail(e, "hipMalloc(job tables)");
+        memcpy(c->tablesHost.p, pl.tables.data(), 4 * pl.tables.size());
+        if ((e = hipMemcpyAsync(c->jobTables.p, c->tablesHost.p, 4 * pl.tables.size(), hipMemcpyHostToDevice,
+                                c->stream)) != hipSuccess)
+            return c->hip_fail(e, "hipMemcpyAsync(job tables)");
+        Q.vblock_job = c->jobTables.p;
+        Q.sblock_job = Q.vblock_job + Q.vert_blocks;
+        Q.chunk_job = Q.sblock_job + Q.slot_blocks;
+        Q.tile_job = Q.chunk_job + Q.total_chunks;
+    }
+    const uint32_t T = Q.total_slots, V = Q.total_verts;
+    if (!c->clipQueueCap) c->clipQueueCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16));
+    if (!c->extraRecCap) c->extraRecCap = 4096;
+    if (!c->extraVertCap) c->extraVertCap = 8192;
+    const uint32_t refWant = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * Q.total_tiles + 1024, 0xFFFFFFF0ull);
+    c->refCap = std::max(c->refCap, refWant);
+
+    c->stats = edxr_stats{};
+    for (uint32_t attempt = 0; attempt < 8; attempt++) {
+        if ((uint64_t)T + c->extraRecCap > kRecordMask) return c->fail(EDXR_E_LIMIT, "record capacity exceeded");
+        if ((e = c->rrec.ensure((size_t)T + c->extraRecCap)) != hipSuccess ||
+            (e = c->srec.ensure((size_t)T + c->extraRecCap)) != hipSuccess ||
+            (e = c->desc.ensure(std::max<uint32_t>(T, 1))) != hipSuccess ||
+            (e = c->slotRefs.ensure(std::max<uint32_t>(T, 1))) != hipSuccess ||
+            (e = c->groupSums.ensure(std::max<uint64_t>(pl.groupEntries, 1))) != hipSuccess ||
+            (e = c->clipQueue.ensure(c->clipQueueCap)) != hipSuccess ||
+            (e = c->binCounts.ensure(std::max<uint64_t>(pl.countEntries, 1))) != hipSuccess ||
+            (e = c->refs.ensure(c->refCap)) != hipSuccess ||
+            (e = c->work.ensure((size_t)Q.total_tiles + c->refCap / kSegRefs + 2)) != hipSuccess ||
+            (e = c->partials.ensure(c->work.cap * kTileSize * kTileSize)) != hipSuccess ||
+            (e = c->verts.ensure((size_t)V + c->extraVertCap)) != hipSuccess)
+            return c->hip_fail(e, "hipMalloc(pipeline)");
+
+        DeviceBuffers B;
+        B.rrec = c->rrec.p;
+        B.srec = c->srec.p;
+        B.desc = c->desc.p;
+        B.slot_refs = c->slotRefs.p;
+        B.group_sums = c->groupSums.p;
+        B.clip_queue = c->clipQueue.p;
+        B.clip_queue_cap = c->clipQueueCap;
+        B.extra_rec_cap = c->extraRecCap;
+        B.verts = c->verts.p;
+        B.extra_vert_cap = c->extraVertCap;
+        B.bin_counts = c->binCounts.p;
+        B.tile_totals = c->tileTotals.p;
+        B.tile_start = c->tileStart.p;
+        B.refs = c->refs.p;
+        B.ref_cap = c->refCap;
+        B.work = c->work.p;
+        B.work_cap = (uint32_t)c->work.cap;
+        B.tile_item = c->tileItem.p;
+        B.partials = c->partials.p;
+        // Persistent raster grid: enough resident workgroups per CU to hide latency.
+        B.raster_blocks = c->numCUs * (c->rasterPerCU ? c->rasterPerCU : 8);
+        B.counters = c->counters.p;
+        B.color = color;
+        B.depth = depth;
+        B.ids = ids;
+
+        const bool prof = c->profile && !graph;
+        if (graph) {
+            const std::vector<uint8_t> key = launch_key(Q, B, c->jobsDev.p);
+            if (!c->graphExec || key != c->graphKey) {
+                drop_graph(c);
+                hipGraph_t g = nullptr;
+                if ((e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                    return c->hip_fail(e, "hipStreamBeginCapture");
+                const hipError_t le = launch_frame(Q, c->jobsDev.p, nullptr, B, c->stream, nullptr);
+                e = hipStreamEndCapture(c->stream, &g);
+                if (le != hipSuccess) e = le;
+                if (e == hipSuccess) e = hipGraphInstantiate(&c->graphExec, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    drop_graph(c);
+                    return c->hip_fail(e, "graph capture");
+                }
+                c->graphKey = key;
+            }
+            if ((e = hipGraphLaunch(c->graphExec, c->stream)) != hipSuccess) return c->hip_fail(e, "hipGraphLaunch");
+        } else {
+            const FrameParams* job0 = single ? &pl.jobs[0] : nullptr;
+            if ((e = launch_frame(Q, c->jobsDev.p, job0, B, c->stream, prof ? c->events : nullptr)) != hipSuccess)
+                return c->hip_fail(e, "kernel launch");
+        }
+        if ((e = hipMemcpyAsync(c->hostCounters, c->counters.p, sizeof(uint32_t) * kCntSlots, hipMemcpyDeviceToHost,
+                                c->stream)) != hipSuccess)
+            return c->hip_fail(e, "hipMemcpyAsync(counters)");
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
+        const uint32_t* h = c->hostCounters;
+        const uint32_t ov = h[kCntOverflow];
+        if (ov & kOvPolygon) return c->fail(EDXR_E_LIMIT, "clip polygon exceeded 9 vertices");
+        if (!ov) {
+            if (h[kCntBadIndex]) return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices");
+            uint64_t rejected = 0, mainRecords = 0, tin = 0;
+            for (uint32_t s = 0; s < kStripes; s++) {
+                rejected += h[kCntStriped + kStRejected * kStripes + s];
+                mainRecords += h[kCntStriped + kStMainRecords * kStripes + s];
+            }
+            for (const FrameParams& P : pl.jobs) tin += P.num_triangles;
+            c->stats.triangles_in = tin;
+            c->stats.trivially_rejected = rejected;
+            c->stats.clipped = h[kCntClipped];
+            c->stats.near_clipped = h[kCntNearClipped];
+            c->stats.setup_records = mainRecords + h[kCntExtraRecords];
+            c->stats.refs = h[kCntTotalRefs];
+            c->stats.clip_vertices = h[kCntExtraVertices];
+            c->stats.retries = attempt;
+            c->stats.bin_chunks = Q.total_chunks;
+            if (prof) {
+                for (int k = 0; k < kNumStages; k++) {
+                    float ms = 0.0f;
+                    if (hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]) == hipSuccess) c->stageMs[k] += ms;
+                }
+                c->profiledFrames++;
+            }
+            return EDXR_OK;
+        }
+        auto grow = [](uint32_t need) {
+            return (uint32_t)std::min<uint64_t>((uint64_t)need + need / 4 + 1024, 0xFFFFFFF0ull);
+        };
+        if (ov & kOvClipQueue) c->clipQueueCap = grow(h[kCntClipQueue]);
+        if (ov & kOvExtraRecords) c->extraRecCap = grow(std::max(h[kCntExtraRecords], 2 * c->extraRecCap));
+        if (ov & kOvExtraVertices) c->extraVertCap = grow(std::max(h[kCntExtraVertices], 2 * c->extraVertCap));
+        if (ov & kOvRefs) c->refCap = grow(std::max(h[kCntTotalRefs], c->refCap));
+        // A clip-queue overflow hides the clip-stage demand: grow those too.
+        if (ov & kOvClipQueue) {
+            c->extraRecCap = std::max(c->extraRecCap, grow(2 * c->extraRecCap));
+            c->extraVertCap = std::max(c->extraVertCap, grow(2 * c->extraVertCap));
+        }
+        c->stats.retries = attempt + 1;
+    }
+    return c->fail(EDXR_E_LIMIT, "capacity retries exhausted");
+}
+
+// Uploads the host-memory meshes of a batch into one device buffer and
+// rewrites their pointers (device meshes are used in place).
+int stage_batch_meshes(edxr_ctx* c, const edxr_job* jobs, uint32_t n, std::vector<JobInput>& in) {
+    uint64_t bytes = 0;
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    for (uint32_t k = 0; k < n; k++) {
+        const edxr_mesh& m = jobs[k].mesh;
+        if (m.memory != EDXR_MEM_HOST) continue;
+        const float* planes[9] = {m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb};
+        for (const float* p : planes)
+            if (p) bytes += al(4ull * m.num_vertices);
+        bytes += al(12ull * m.num_triangles);
+    }
+    if (!bytes) return EDXR_OK;
+    hipError_t e;
+    if ((e = c->batchMesh.ensure(bytes)) != hipSuccess) return c->hip_fail(e, "hipMalloc(batch meshes)");
+    uint64_t off = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const edxr_mesh& m = jobs[k].mesh;
+        if (m.memory != EDXR_MEM_HOST) continue;
+        const float* planes[9] = {m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb};
+        const float* dst[9];
+        for (int q = 0; q < 9; q++) {
+            dst[q] = nullptr;
+            if (!planes[q]) continue;
+            uint8_t* d = c->batchMesh.p + off;
+            if (m.num_vertices &&
+                (e = hipMemcpyAsync(d, planes[q], 4ull * m.num_vertices, hipMemcpyHostToDevice, c->stream)) !=
+                    hipSuccess)
+                return c->hip_fail(e, "hipMemcpyAsync(batch mesh)");
+            dst[q] = (const float*)d;
+            off += al(4ull * m.num_vertices);
+        }
+        uint8_t* di = c->batchMesh.p + off;
+        if (m.num_triangles &&
+            (e = hipMemcpyAsync(di, m.indices, 12ull * m.num_triangles, hipMemcpyHostToDevice, c->stream)) !=
+                hipSuccess)
+            return c->hip_fail(e, "hipMemcpyAsync(batch indices)");
+        off += al(12ull * m.num_triangles);
+        in[k].mesh = MeshPtrs{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8],
+                              (const uint32_t*)di};
+    }
+    return EDXR_OK;
 }
 
 }  // namespace
@@ -212,6 +602,7 @@ int edxr_create(edxr_ctx** out, uint32_t width, uint32_t height) {
     edxr_ctx* c = new edxr_ctx();
     edxr_default_material(c->shader, &c->mat);
     hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = init_kernels();
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hostCounters, sizeof(uint32_t) * kCntSlots, hipHostMallocDefault);
     if (e == hipSuccess) e = c->counters.ensure(kCntSlots);
@@ -236,18 +627,24 @@ int edxr_create(edxr_ctx** out, uint32_t width, uint32_t height) {
 void edxr_destroy(edxr_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     for (auto& b : c->mesh) b.release();
     c->meshIdx.release();
+    c->batchMesh.release();
     c->rrec.release();
     c->srec.release();
     for (DevBuf<uint32_t>* b : {&c->desc, &c->clipQueue, &c->binCounts, &c->tileTotals, &c->tileStart, &c->refs,
-                                &c->counters, &c->color, &c->ids, &c->work, &c->tileItem})
+                                &c->counters, &c->color, &c->ids, &c->work, &c->tileItem, &c->groupSums,
+                                &c->jobTables, &c->bcolor, &c->bids})
         b->release();
     c->depth.release();
+    c->bdepth.release();
     c->partials.release();
     c->slotRefs.release();
-    c->groupSums.release();
     c->verts.release();
+    c->jobsDev.release();
+    c->jobsHost.release();
+    c->tablesHost.release();
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
     for (auto& ev : c->events)
         if (ev) (void)hipEventDestroy(ev);
@@ -301,6 +698,19 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
             if (value < 0 || value > 65536) return c->fail(EDXR_E_INVALID, "bin chunks out of range");
             c->binChunksSetting = (uint32_t)value;
             return EDXR_OK;
+        case EDXR_STATE_GRAPH:
+            c->useGraph = value ? 1 : 0;
+            if (!value) drop_graph(c);
+            return EDXR_OK;
+        case EDXR_STATE_TRIS_PER_CHUNK:
+            if (value < 0 || (value > 0 && value < 64) || value > (1 << 20))
+                return c->fail(EDXR_E_INVALID, "triangles per chunk out of range (0 or 64..2^20)");
+            c->trisPerChunk = (uint32_t)value;
+            return EDXR_OK;
+        case EDXR_STATE_RASTER_BLOCKS_PER_CU:
+            if (value < 0 || value > 32) return c->fail(EDXR_E_INVALID, "raster blocks per CU out of range (0..32)");
+            c->rasterPerCU = (uint32_t)value;
+            return EDXR_OK;
         default: return c->fail(EDXR_E_INVALID, "unknown state key");
     }
 }
@@ -308,11 +718,8 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
 int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     if (!c || !m) return EDXR_E_INVALID;
     if (!c->haveTransform) return c->fail(EDXR_E_STATE, "edxr_set_transform was not called");
-    if (!m->px || !m->py || !m->pz || (m->num_triangles && !m->indices))
-        return c->fail(EDXR_E_INVALID, "mesh positions/indices missing");
-    if (!m->nx || !m->ny || !m->nz) return c->fail(EDXR_E_INVALID, "mesh normals missing");
-    if (c->shader == EDXR_SHADER_VC_BLINN_PHONG && (!m->cr || !m->cg || !m->cb))
-        return c->fail(EDXR_E_INVALID, "vertex colours required by EDXR_SHADER_VC_BLINN_PHONG");
+    int r = check_mesh(c, m, c->shader);
+    if (r != EDXR_OK) return r;
     const uint32_t V = m->num_vertices, T = m->num_triangles;
     // Record indices (T + clip records) must fit the 28-bit reference payload.
     if ((uint64_t)T + 65536 > kRecordMask) return c->fail(EDXR_E_LIMIT, "too many triangles");
@@ -340,140 +747,152 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
         M = MeshPtrs{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8], c->meshIdx.p};
     }
 
-    FrameParams P;
-    memset(&P, 0, sizeof(P));
-    memcpy(P.mvp, c->mvp, 64);
-    memcpy(P.raster, c->raster, 64);
-    memcpy(P.eye, c->eye, 12);
-    {
-        // Math::Normalize(lightDir) as the pixel shader does (Core/Shader.h:221).
-        const float* L = c->lightDir;
-        const float s = 1.0f / sqrtf((L[0] * L[0] + L[1] * L[1]) + L[2] * L[2]);
-        P.light[0] = L[0] * s;
-        P.light[1] = L[1] * s;
-        P.light[2] = L[2] * s;
+    JobInput in;
+    in.mesh = M;
+    in.V = V;
+    in.T = T;
+    in.W = c->W;
+    in.H = c->H;
+    memcpy(in.mvp, c->mvp, 64);
+    memcpy(in.raster, c->raster, 64);
+    memcpy(in.eye, c->eye, 12);
+    memcpy(in.light, c->lightDir, 12);
+    in.mat = c->mat;
+    in.shader = c->shader;
+    in.hierarchical = c->hierarchical;
+    Plan pl;
+    if ((r = build_plan(c, std::vector<JobInput>{in}, pl)) != EDXR_OK) return r;
+    c->rendered = false;
+    if ((r = run_plan(c, pl, c->color.p, c->depth.p, c->ids.p)) != EDXR_OK) return r;
+    c->rendered = true;
+    c->batchRendered = false;
+    c->lastT = T;
+    c->lastV = V;
+    c->lastChunks = pl.Q.total_chunks;
+    c->lastMesh = M;
+    return EDXR_OK;
+}
+
+int edxr_render_batch(edxr_ctx* c, const edxr_job* jobs, uint32_t n) {
+    if (!c || !jobs || n == 0) return EDXR_E_INVALID;
+    if (n > 65536) return c->fail(EDXR_E_LIMIT, "more than 65536 jobs");
+    std::vector<JobInput> in(n);
+    int r;
+    for (uint32_t k = 0; k < n; k++) {
+        const edxr_job& j = jobs[k];
+        if (j.shader < EDXR_SHADER_LAMBERT_ALBEDO || j.shader > EDXR_SHADER_VC_BLINN_PHONG)
+            return c->fail(EDXR_E_INVALID, "bad shader kind in job " + std::to_string(k));
+        if ((r = check_mesh(c, &j.mesh, j.shader)) != EDXR_OK) {
+            c->err += " (job " + std::to_string(k) + ")";
+            return r;
+        }
+        JobInput& d = in[k];
+        const edxr_mesh& m = j.mesh;
+        d.mesh = MeshPtrs{m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb, m.indices};
+        d.V = m.num_vertices;
+        d.T = m.num_triangles;
+        d.W = j.width;
+        d.H = j.height;
+        mat_mul(j.proj, j.mv, d.mvp);
+        float inv[16];
+        mat_inverse(j.mv, inv);
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        transform_point3(inv, zero, d.eye);
+        memcpy(d.raster, j.raster, 64);
+        memcpy(d.light, j.light, 12);
+        d.mat = j.material;
+        d.shader = j.shader;
+        d.hierarchical = j.hierarchical ? 1 : 0;
     }
-    P.ambient = c->mat.ambient;
-    P.kd = c->mat.kd;
-    P.ks = c->mat.ks;
-    P.shininess = c->mat.shininess;
-    memcpy(P.albedo, c->mat.albedo, 12);
-    P.shader = c->shader;
-    P.shin_int = is_integer_exponent(c->mat.shininess) ? (int32_t)c->mat.shininess : -1;
-    P.clamp_spec = c->mat.clamp_specular;
-    P.hierarchical = c->hierarchical;
-    P.width = c->W;
-    P.height = c->H;
-    P.tiles_x = c->tilesX;
-    P.tiles_y = c->tilesY;
-    P.num_tiles = c->nTiles;
-    P.num_vertices = V;
-    P.num_triangles = T;
-    P.num_chunks = choose_chunks(c, T);
-    P.chunk_slots = (T + P.num_chunks - 1) / P.num_chunks;
+    if ((r = stage_batch_meshes(c, jobs, n, in)) != EDXR_OK) return r;
+    Plan pl;
+    if ((r = build_plan(c, in, pl)) != EDXR_OK) return r;
+    hipError_t e;
+    if ((e = c->bcolor.ensure(pl.pixels)) != hipSuccess || (e = c->bdepth.ensure(pl.pixels)) != hipSuccess ||
+        (e = c->bids.ensure(pl.pixels)) != hipSuccess)
+        return c->hip_fail(e, "hipMalloc(batch frame buffers)");
+    c->batchRendered = false;
+    c->rendered = false;
+    if ((r = run_plan(c, pl, c->bcolor.p, c->bdepth.p, c->bids.p)) != EDXR_OK) return r;
+    c->batchPix.resize(n + 1);
+    c->batchW.resize(n);
+    c->batchH.resize(n);
+    for (uint32_t k = 0; k < n; k++) {
+        c->batchPix[k] = pl.jobs[k].pix_base;
+        c->batchW[k] = pl.jobs[k].width;
+        c->batchH[k] = pl.jobs[k].height;
+    }
+    c->batchPix[n] = pl.pixels;
+    c->batchRendered = true;
+    return EDXR_OK;
+}
 
-    if (!c->clipQueueCap) c->clipQueueCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16));
-    if (!c->extraRecCap) c->extraRecCap = 4096;
-    if (!c->extraVertCap) c->extraVertCap = 8192;
-    if (!c->refCap) c->refCap = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * c->nTiles + 1024, 0xFFFFFFF0ull);
+int edxr_batch_read(edxr_ctx* c, uint32_t job, uint8_t* rgba, float* depth, uint32_t* ids) {
+    if (!c) return EDXR_E_INVALID;
+    if (!c->batchRendered) return c->fail(EDXR_E_STATE, "no batch rendered");
+    if (job >= c->batchW.size()) return c->fail(EDXR_E_INVALID, "job out of range");
+    const uint64_t o = c->batchPix[job], n = (uint64_t)c->batchW[job] * c->batchH[job];
+    hipError_t e = hipSuccess;
+    if (rgba) e = hipMemcpyAsync(rgba, c->bcolor.p + o, 4 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, c->bdepth.p + o, 4 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && ids) e = hipMemcpyAsync(ids, c->bids.p + o, 4 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? EDXR_OK : c->hip_fail(e, "batch_read");
+}
 
-    c->stats = edxr_stats{};
-    for (uint32_t attempt = 0; attempt < 8; attempt++) {
-        if ((uint64_t)T + c->extraRecCap > kRecordMask) return c->fail(EDXR_E_LIMIT, "record capacity exceeded");
-        if ((e = c->rrec.ensure((size_t)T + c->extraRecCap)) != hipSuccess ||
-            (e = c->srec.ensure((size_t)T + c->extraRecCap)) != hipSuccess || (e = c->desc.ensure(T)) != hipSuccess ||
-            (e = c->slotRefs.ensure(T)) != hipSuccess ||
-            (e = c->groupSums.ensure((size_t)((P.num_chunks + kChunkGroup - 1) / kChunkGroup) * c->nTiles)) !=
-                hipSuccess ||
-            (e = c->clipQueue.ensure(c->clipQueueCap)) != hipSuccess ||
-            (e = c->binCounts.ensure((size_t)P.num_chunks * c->nTiles)) != hipSuccess ||
-            (e = c->refs.ensure(c->refCap)) != hipSuccess ||
-            (e = c->work.ensure((size_t)c->nTiles + c->refCap / kSegRefs + 2)) != hipSuccess ||
-            (e = c->partials.ensure(c->work.cap * kTileSize * kTileSize)) != hipSuccess)
-            return c->hip_fail(e, "hipMalloc(pipeline)");
-        if ((e = c->verts.ensure((size_t)V + c->extraVertCap)) != hipSuccess)
-            return c->hip_fail(e, "hipMalloc(vertices)");
+int edxr_batch_device_buffers(edxr_ctx* c, void** color, void** depth, void** ids, uint64_t* offsets) {
+    if (!c) return EDXR_E_INVALID;
+    if (!c->batchRendered) return c->fail(EDXR_E_STATE, "no batch rendered");
+    if (color) *color = c->bcolor.p;
+    if (depth) *depth = c->bdepth.p;
+    if (ids) *ids = c->bids.p;
+    if (offsets)
+        for (size_t k = 0; k < c->batchPix.size(); k++) offsets[k] = c->batchPix[k];
+    return EDXR_OK;
+}
 
-        DeviceBuffers B;
-        B.rrec = c->rrec.p;
-        B.srec = c->srec.p;
-        B.desc = c->desc.p;
-        B.slot_refs = c->slotRefs.p;
-        B.group_sums = c->groupSums.p;
-        B.clip_queue = c->clipQueue.p;
-        B.clip_queue_cap = c->clipQueueCap;
-        B.extra_rec_cap = c->extraRecCap;
-        B.verts = c->verts.p;
-        B.extra_vert_cap = c->extraVertCap;
-        B.bin_counts = c->binCounts.p;
-        B.tile_totals = c->tileTotals.p;
-        B.tile_start = c->tileStart.p;
-        B.refs = c->refs.p;
-        B.ref_cap = c->refCap;
-        B.work = c->work.p;
-        B.work_cap = (uint32_t)c->work.cap;
-        B.tile_item = c->tileItem.p;
-        B.partials = c->partials.p;
-        // Persistent raster grid: enough resident workgroups per CU to hide latency.
-        B.raster_blocks = c->numCUs * 8;
-        B.counters = c->counters.p;
-        B.color = c->color.p;
-        B.depth = c->depth.p;
-        B.ids = c->ids.p;
-
-        if ((e = launch_frame(P, M, B, c->stream, c->profile ? c->events : nullptr)) != hipSuccess)
-            return c->hip_fail(e, "kernel launch");
-        if ((e = hipMemcpyAsync(c->hostCounters, c->counters.p, sizeof(uint32_t) * kCntSlots, hipMemcpyDeviceToHost,
-                                c->stream)) != hipSuccess)
-            return c->hip_fail(e, "hipMemcpyAsync(counters)");
-        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
-        const uint32_t* h = c->hostCounters;
-        const uint32_t ov = h[kCntOverflow];
-        if (ov & kOvPolygon) return c->fail(EDXR_E_LIMIT, "clip polygon exceeded 9 vertices");
-        if (!ov) {
-            if (h[kCntBadIndex]) return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices");
-            uint64_t rejected = 0, mainRecords = 0;
-            for (uint32_t s = 0; s < kStripes; s++) {
-                rejected += h[kCntStriped + kStRejected * kStripes + s];
-                mainRecords += h[kCntStriped + kStMainRecords * kStripes + s];
+int edxr_autotune(edxr_ctx* c, const edxr_job* jobs, uint32_t n, edxr_tuning* out) {
+    if (!c || !jobs || n == 0) return EDXR_E_INVALID;
+    static const uint32_t kTpc[] = {512, 1024, 2048, 4096};
+    static const uint32_t kRpc[] = {4, 8, 16};
+    const uint32_t saveT = c->trisPerChunk, saveR = c->rasterPerCU;
+    double best = 1e30;
+    uint32_t bestT = 1024, bestR = 8;
+    for (uint32_t tpc : kTpc)
+        for (uint32_t rpc : kRpc) {
+            c->trisPerChunk = tpc;
+            c->rasterPerCU = rpc;
+            int r = edxr_render_batch(c, jobs, n);  // warm-up (allocation, capture)
+            if (r != EDXR_OK) {
+                c->trisPerChunk = saveT;
+                c->rasterPerCU = saveR;
+                return r;
             }
-            c->stats.triangles_in = T;
-            c->stats.trivially_rejected = rejected;
-            c->stats.clipped = h[kCntClipped];
-            c->stats.near_clipped = h[kCntNearClipped];
-            c->stats.setup_records = mainRecords + h[kCntExtraRecords];
-            c->stats.refs = h[kCntTotalRefs];
-            c->stats.clip_vertices = h[kCntExtraVertices];
-            c->stats.retries = attempt;
-            c->stats.bin_chunks = P.num_chunks;
-            c->rendered = true;
-            c->lastT = T;
-            c->lastV = V;
-            c->lastChunks = P.num_chunks;
-            c->lastMesh = M;
-            if (c->profile) {
-                for (int k = 0; k < kNumStages; k++) {
-                    float ms = 0.0f;
-                    if (hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]) == hipSuccess) c->stageMs[k] += ms;
+            double times[3];
+            for (double& t : times) {
+                const auto t0 = std::chrono::steady_clock::now();
+                if ((r = edxr_render_batch(c, jobs, n)) != EDXR_OK) {
+                    c->trisPerChunk = saveT;
+                    c->rasterPerCU = saveR;
+                    return r;
                 }
-                c->profiledFrames++;
+                t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             }
-            return EDXR_OK;
+            std::sort(times, times + 3);
+            if (times[1] < best) {
+                best = times[1];
+                bestT = tpc;
+                bestR = rpc;
+            }
         }
-        auto grow = [](uint32_t need) {
-            return (uint32_t)std::min<uint64_t>((uint64_t)need + need / 4 + 1024, 0xFFFFFFF0ull);
-        };
-        if (ov & kOvClipQueue) c->clipQueueCap = grow(h[kCntClipQueue]);
-        if (ov & kOvExtraRecords) c->extraRecCap = grow(std::max(h[kCntExtraRecords], 2 * c->extraRecCap));
-        if (ov & kOvExtraVertices) c->extraVertCap = grow(std::max(h[kCntExtraVertices], 2 * c->extraVertCap));
-        if (ov & kOvRefs) c->refCap = grow(h[kCntTotalRefs]);
-        // A clip-queue overflow hides the clip-stage demand: grow those too.
-        if (ov & kOvClipQueue) {
-            c->extraRecCap = std::max(c->extraRecCap, grow(2 * c->extraRecCap));
-            c->extraVertCap = std::max(c->extraVertCap, grow(2 * c->extraVertCap));
-        }
+    c->trisPerChunk = bestT;
+    c->rasterPerCU = bestR;
+    if (out) {
+        out->tris_per_chunk = bestT;
+        out->raster_blocks_per_cu = bestR;
+        out->frame_ms = best;
     }
-    return c->fail(EDXR_E_LIMIT, "capacity retries exhausted");
+    return EDXR_OK;
 }
 
 int edxr_read_color(edxr_ctx* c, uint8_t* rgba, size_t bytes) {

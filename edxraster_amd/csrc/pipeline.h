@@ -9,7 +9,9 @@ namespace edxr {
 
 constexpr int kTileLog2 = 5;                 // Tile::SIZE_LOG_2 (EDXRaster/Core/Tile.h:12-13)
 constexpr int kTileSize = 1 << kTileLog2;    // 32 x 32 pixels
-constexpr uint32_t kMaxTiles = 16384;        // LDS-resident bin cursors
+constexpr uint32_t kMaxTiles = 16384;        // LDS-resident bin cursors (per job)
+constexpr uint32_t kTileBits = 18;           // global tile index bits in a work item
+constexpr uint32_t kMaxBatchTiles = 1u << kTileBits;
 constexpr uint32_t kMaxDim = 8192;
 constexpr uint32_t kRecordBits = 28;         // record index bits in a tile reference
 constexpr uint32_t kRecordMask = (1u << kRecordBits) - 1;
@@ -17,6 +19,7 @@ constexpr int kMaxPoly = 16;                 // clip polygon capacity (convex in
 constexpr uint32_t kSegRefs = 2048;          // tile references per raster work item
 constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
+constexpr uint32_t kJobAlign = 256;          // job slot alignment = vertex/setup block size
 constexpr uint32_t kMaxChunks = 4096;
 // Per-triangle descriptor: bit 31 set = compact small triangle (record in slot
 // t; tile bbox minTX | minTY << 14 | dx << 28 | dy << 29, dx, dy in {0, 1});
@@ -25,7 +28,20 @@ constexpr uint32_t kDescCompact = 1u << 31;
 
 enum ShaderKind { kLambertAlbedo = 0, kBlinnPhongRef = 1, kVcBlinnPhong = 2 };
 
-// Per-frame constants, passed by value to every kernel.
+struct MeshPtrs {
+    const float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb;
+    const uint32_t* idx;
+};
+
+// Constants of one render job (one mesh, camera, shader and render target),
+// read by the kernels from a device table: a plain frame is a batch of one.
+// Batch-wide arrays hold every job's data at the job's bases: shading vertices
+// from vert_base, triangle slots (setup records of input triangles) from
+// slot_base, tiles from tile_base, binning chunks from chunk_base, its
+// chunk x tile count matrix at count_base, column-scan group sums at
+// group_base and its pixels at pix_base.  vert_base and slot_base are
+// multiples of the 256-lane block, so every block of the vertex and setup
+// passes belongs to one job.
 struct FrameParams {
     float mvp[16];        // proj * mv, row-major, column-vector convention
     float raster[16];
@@ -40,11 +56,25 @@ struct FrameParams {
     uint32_t width, height, tiles_x, tiles_y, num_tiles;
     uint32_t num_vertices, num_triangles;
     uint32_t chunk_slots, num_chunks;
+    uint32_t vert_base, slot_base, tile_base, chunk_base;
+    uint64_t count_base, group_base, pix_base;
+    MeshPtrs mesh;
 };
 
-struct MeshPtrs {
-    const float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb;
-    const uint32_t* idx;
+// Batch-wide sizes and the block -> job tables (null for a single job).
+struct BatchParams {
+    uint32_t num_jobs;
+    uint32_t total_slots;    // triangle slots; clipped pieces' records follow
+    uint32_t total_verts;    // shading-vertex slots; clip-created vertices follow
+    uint32_t total_tiles;
+    uint32_t total_chunks;
+    uint32_t vert_blocks, slot_blocks;
+    uint32_t max_job_tiles;     // max over jobs of num_tiles
+    uint32_t max_job_chunks;    // max over jobs of num_chunks
+    const uint32_t* vblock_job;
+    const uint32_t* sblock_job;
+    const uint32_t* chunk_job;
+    const uint32_t* tile_job;
 };
 
 // Post-transform vertex for shading (ProjectedVertex, Core/Shader.h:13-25,
@@ -70,16 +100,17 @@ struct __attribute__((aligned(16))) RasterRec {
 };
 static_assert(sizeof(RasterRec) == 48, "RasterRec layout");
 
-// Submission order of a record: input triangle, then the index among the
-// triangle's surviving fan pieces (Core/Clipper.h:155-170).  Unclipped
-// triangles live in record slot `tri`; clipped ones at desc[tri] >> 3 + survivor.
+// Submission order of a record: triangle slot (the job's slot_base + input
+// index), then the index among the triangle's surviving fan pieces
+// (Core/Clipper.h:155-170).  Unclipped triangles live in record slot `tri`;
+// clipped ones at desc[tri] >> 3 + survivor.
 __host__ __device__ inline uint32_t make_order(uint32_t tri, uint32_t survivor, bool clipped) {
     return (tri << 4) | (survivor << 1) | (clipped ? 1u : 0u);
 }
 
-// Shading part of a clipped piece: unified vertex ids (< V input, >= V
-// clip-created) and the input triangle index.  Unclipped triangles take their
-// vertex ids from the index buffer.
+// Shading part of a clipped piece: shading-vertex slots (input vertices at the
+// job's vert_base, clip-created ones from total_verts) and the triangle slot.
+// Unclipped triangles take their vertex ids from the job's index buffer.
 struct __attribute__((aligned(16))) ShadeRec {
     uint32_t v0, v1, v2, tri_id;
 };
@@ -130,13 +161,13 @@ struct DeviceBuffers {
     uint32_t* tile_start;  // num_tiles + 1
     uint32_t* refs;
     uint32_t ref_cap;
-    uint32_t* work;        // raster work items: tile | segment << 14
+    uint32_t* work;        // raster work items: global tile | segment << kTileBits
     uint32_t work_cap;
-    uint32_t* tile_item;   // per tile: index of its first work item
+    uint32_t* tile_item;   // per global tile: index of its first work item
     uint64_t* partials;    // work_cap * 1024 (depth, order) keys, one tile image per work item
     uint32_t raster_blocks;
     uint32_t* counters;
-    uint32_t* color;
+    uint32_t* color;       // every job's pixels at its pix_base
     float* depth;
     uint32_t* ids;
 };
@@ -146,8 +177,11 @@ struct DeviceBuffers {
 constexpr int kNumStages = 5;
 
 // Host-side launchers (pipeline.hip).  events may be null (no profiling).
-hipError_t launch_frame(const FrameParams& P, const MeshPtrs& M, const DeviceBuffers& B, hipStream_t s,
-                        hipEvent_t* events);
-hipError_t launch_reset(uint32_t* counters, hipStream_t s);
+// With job0 non-null the reset kernel also stores *job0 as job 0 of the table
+// (a single job, no separate upload); otherwise the table is already resident.
+hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FrameParams* job0, const DeviceBuffers& B,
+                        hipStream_t s, hipEvent_t* events);
+// Kernel attributes (dynamic LDS limits); once per process before launches.
+hipError_t init_kernels();
 
 }  // namespace edxr

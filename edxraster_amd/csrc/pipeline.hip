@@ -1,6 +1,10 @@
 // pipeline.hip -- gfx950 kernels of the EDXRaster hot path.
 //
-// Stages (EDXRaster/Core/Renderer.cpp:100-118 order):
+// Every kernel reads its job's constants from a device table (FrameParams):
+// a plain frame is a batch of one job, a batch renders many independent jobs
+// with the same launches.  Stages (EDXRaster/Core/Renderer.cpp:100-118 order):
+//   k_vertex     one lane per vertex: SoA in, 48-byte shading vertex out
+//                (VertexProcessing, Core/Renderer.cpp:120-127).
 //   k_setup      vertex transform + outcodes + triangle setup, one lane per
 //                triangle (DefaultVertexShader::Execute Core/Shader.h:40-49,
 //                Clipper::Clip Core/Clipper.h:71-189, RasterTriangle::Setup
@@ -9,16 +13,19 @@
 //   k_clip       Sutherland-Hodgman clipping + fan setup of queued triangles
 //                (Core/Clipper.h:109-170,192-288).
 //   k_bin_count  per-chunk tile histograms (binning, Core/Renderer.cpp:161-230)
-//   k_bin_colscan / k_bin_tilescan   exclusive scans -> per-(tile, chunk) offsets
+//   k_bin_colsum / k_bin_groupscan / k_bin_colscan / k_bin_tilescan
+//                exclusive scans -> per-(tile, chunk) offsets and the raster
+//                work list of (tile, segment) items.
 //   k_bin_scatter order-preserving scatter: refs of a tile are in ascending
 //                (triangle, fan index) order, like the reference's per-core lists
 //                concatenated in core order (Core/Renderer.cpp:250-253).
-//   k_raster     one workgroup per 32x32 tile: hierarchical edge-function
-//                coverage (CoarseRasterize / FineRasterize / TrivialAcceptTriangle,
-//                Core/Rasterizer.h:32-200,310-353), depth test with the (depth,
-//                submission order) winner rule, then deferred perspective-correct
-//                interpolation + pixel shading of the winner and one write-back of
-//                colour, depth and triangle ID.
+//   k_raster     persistent: hierarchical edge-function coverage of one (tile,
+//                segment) item at a time (CoarseRasterize / FineRasterize /
+//                TrivialAcceptTriangle, Core/Rasterizer.h:32-200,310-353) and
+//                the depth test as a (depth, submission order) key minimum.
+//   k_shade      per tile: minimum key over the tile's items, perspective-correct
+//                interpolation + pixel shading of the winner and one write-back
+//                of colour, depth and triangle ID (Core/Renderer.cpp:272-350).
 //
 // Arithmetic follows the reference operation by operation: int32 wrap
 // (-fwrapv), truncating 28.4 snap, round-to-nearest int->float, correctly
@@ -33,6 +40,7 @@ namespace {
 
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kBlock = 256;
+static_assert(kBlock == kJobAlign, "a vertex/setup block must not span two jobs");
 
 __device__ __forceinline__ float4 xform4(const float* m, float x, float y, float z, float w) {
     float4 r;
@@ -116,9 +124,18 @@ __device__ __forceinline__ uint32_t wave_count(bool p) { return (uint32_t)__popc
 // ---------------------------------------------------------------------------
 // Counters
 // ---------------------------------------------------------------------------
-__global__ void k_reset(uint32_t* counters) {
+__global__ void k_reset(uint32_t* counters, FrameParams* jobs, FrameParams job0, int writeJob) {
     for (uint32_t i = threadIdx.x; i < kCntSlots; i += blockDim.x) counters[i] = 0;
+    if (writeJob && threadIdx.x == 0) jobs[0] = job0;
 }
+
+namespace {
+// Job of a block-uniform index i (the block -> job tables are null for a
+// single job).  The result is wave-uniform, so job fields load as scalars.
+__device__ __forceinline__ uint32_t job_of(const BatchParams& Q, const uint32_t* tbl, uint32_t i) {
+    return Q.num_jobs == 1 ? 0u : __builtin_amdgcn_readfirstlane(tbl[i]);
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Vertex processing (Core/Renderer.cpp:120-127): one lane per vertex, SoA
@@ -126,8 +143,12 @@ __global__ void k_reset(uint32_t* counters) {
 // DefaultVertexShader's MVP transform (Core/Shader.h:45) as the z-divide loop
 // computes it (Core/Renderer.cpp:144).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_vertex(FrameParams P, MeshPtrs M, ShadeVert* __restrict__ verts) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_vertex(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                   ShadeVert* __restrict__ verts) {
+    const FrameParams& P = jobs[job_of(Q, Q.vblock_job, blockIdx.x)];
+    const MeshPtrs& M = P.mesh;
+    const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t v = slot - P.vert_base;
     if (v >= P.num_vertices) return;
     ShadeVert o;
     o.px = M.px[v];
@@ -143,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_vertex(FrameParams P, MeshPtrs M, Sh
     o.inv_w = 1.0f / w;
     o.pad0 = 0.0f;
     o.pad1 = 0.0f;
-    verts[v] = o;
+    verts[slot] = o;
 }
 
 namespace {
@@ -153,8 +174,9 @@ __device__ __forceinline__ float4 vertex_pos(const ShadeVert* verts, uint32_t v)
 
 // Descriptor of an unclipped triangle's record in slot t.  When its clamped
 // tile bbox spans at most 2x2 tiles, binning lists it in every bbox tile with
-// no corner test (Core/Renderer.cpp:168-180), so the bbox alone (packed here)
-// enumerates its references; otherwise (t << 3) | 1.
+// no corner test (Core/Renderer.cpp:168-180), so the bbox alone (packed here,
+// in the job's tile coordinates) enumerates its references; otherwise
+// (t << 3) | 1.
 __device__ __forceinline__ uint32_t compact_desc(const FrameParams& P, const RasterRec& r, uint32_t t) {
     const int Shift = kTileLog2 + 4;
     const int minTX = max(0, min(r.x0, min(r.x1, r.x2)) >> Shift);
@@ -170,8 +192,12 @@ __device__ __forceinline__ uint32_t compact_desc(const FrameParams& P, const Ras
 // ---------------------------------------------------------------------------
 // Setup: one lane per input triangle.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_setup(FrameParams P, MeshPtrs M, DeviceBuffers B) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                  DeviceBuffers B) {
+    const FrameParams& P = jobs[job_of(Q, Q.sblock_job, blockIdx.x)];
+    const MeshPtrs& M = P.mesh;
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;  // triangle slot
+    const uint32_t t = s - P.slot_base;                     // input triangle of the job
     const bool valid = t < P.num_triangles;
     bool rejected = false, queued = false, bad = false, main = false;
     uint32_t i0 = 0, i1 = 0, i2 = 0;
@@ -184,7 +210,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(FrameParams P, MeshPtrs M, Dev
             bad = true;
         } else {
             // DefaultVertexShader::Execute: projectedPos = MVP * (p, 1).
-            const float4 p0 = vertex_pos(B.verts, i0), p1 = vertex_pos(B.verts, i1), p2 = vertex_pos(B.verts, i2);
+            const float4 p0 = vertex_pos(B.verts, P.vert_base + i0), p1 = vertex_pos(B.verts, P.vert_base + i1),
+                         p2 = vertex_pos(B.verts, P.vert_base + i2);
             const float4 c0 = xform4(P.mvp, p0.x, p0.y, p0.z, 1.0f);
             const float4 c1 = xform4(P.mvp, p1.x, p1.y, p1.z, 1.0f);
             const float4 c2 = xform4(P.mvp, p2.x, p2.y, p2.z, 1.0f);
@@ -213,15 +240,15 @@ __global__ __launch_bounds__(kBlock) void k_setup(FrameParams P, MeshPtrs M, Dev
     if (valid) {
         uint32_t d = 0;
         if (main) {
-            rec.order = make_order(t, 0, false);
-            B.rrec[t] = rec;
-            d = compact_desc(P, rec, t);
+            rec.order = make_order(s, 0, false);
+            B.rrec[s] = rec;
+            d = compact_desc(P, rec, s);
         } else if (queued) {
             const uint32_t slot = qbase + (uint32_t)__popcll(qmask & lanemask_lt());
-            if (slot < B.clip_queue_cap) B.clip_queue[slot] = t;
+            if (slot < B.clip_queue_cap) B.clip_queue[slot] = s;
             else atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvClipQueue);
         }
-        B.desc[t] = d;
+        B.desc[s] = d;
     }
     // Per-workgroup totals, one striped atomic per counter per workgroup.
     __shared__ uint32_t wgCount[3];
@@ -283,11 +310,13 @@ __device__ const uint32_t kPlaneBits[6] = {1u, 2u, 4u, 8u, 32u, 16u};
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void k_clip(FrameParams P, MeshPtrs M, DeviceBuffers B) {
+__global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* __restrict__ jobs, DeviceBuffers B) {
     const uint32_t n = min(B.counters[kCntClipQueue], B.clip_queue_cap);
     for (uint32_t q = blockIdx.x * 64 + threadIdx.x; q < n; q += gridDim.x * 64) {
-        const uint32_t t = B.clip_queue[q];
-        const uint32_t vi[3] = {M.idx[3 * (size_t)t], M.idx[3 * (size_t)t + 1], M.idx[3 * (size_t)t + 2]};
+        const uint32_t t = B.clip_queue[q];  // triangle slot; its job varies per lane
+        const FrameParams& P = jobs[Q.num_jobs == 1 ? 0u : Q.sblock_job[t / kBlock]];
+        const uint32_t* idx = P.mesh.idx + 3 * (size_t)(t - P.slot_base);
+        const uint32_t vi[3] = {P.vert_base + idx[0], P.vert_base + idx[1], P.vert_base + idx[2]};
         float4 c[3];
         for (int k = 0; k < 3; k++) {
             const float4 p = vertex_pos(B.verts, vi[k]);
@@ -376,7 +405,7 @@ __global__ __launch_bounds__(64) void k_clip(FrameParams P, MeshPtrs M, DeviceBu
                 pos = c[which[j]];
             } else {
                 const uint32_t xi = vbase + (uint32_t)e++;
-                vid[j] = P.num_vertices + xi;
+                vid[j] = Q.total_verts + xi;
                 pos = pp[cur][j];
                 const float3 w = ww[cur][j];
                 const ShadeVert& a = B.verts[vi[0]];
@@ -418,7 +447,7 @@ __global__ __launch_bounds__(64) void k_clip(FrameParams P, MeshPtrs M, DeviceBu
             B.desc[t] = 0;
             continue;
         }
-        const uint32_t first = P.num_triangles + rbase;
+        const uint32_t first = Q.total_slots + rbase;
         uint32_t s = 0;
         for (int k = 2; k < cnt; k++) {
             if (!setup_record(P, hx[0], hy[0], hz[0], hx[k - 1], hy[k - 1], hz[k - 1], hx[k], hy[k], hz[k], r)) continue;
@@ -575,13 +604,15 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* w
 
 }  // namespace
 
-__global__ __launch_bounds__(kBlock) void k_bin_count(FrameParams P, DeviceBuffers B) {
+__global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                      DeviceBuffers B) {
     extern __shared__ uint32_t hist[];
-    const uint32_t chunk = blockIdx.x;
+    const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
+    const uint32_t chunk = blockIdx.x - P.chunk_base;  // chunk of the job
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) hist[i] = 0;
     __syncthreads();
-    const uint32_t s0 = chunk * P.chunk_slots;
-    const uint32_t s1 = min(P.num_triangles, s0 + P.chunk_slots);
+    const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
+    const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
     for (uint32_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
         RefEnum e;
         e.init(B.desc[s], s);
@@ -594,13 +625,14 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(FrameParams P, DeviceBuffe
         B.slot_refs[s] = (uint16_t)min(R, (uint32_t)kSlotRefsSaturated);
     }
     __syncthreads();
-    uint32_t* row = B.bin_counts + (size_t)chunk * P.num_tiles;
+    uint32_t* row = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) row[i] = hist[i];
 }
 
-// The chunk x tile count matrix is scanned down each tile's column in three
-// steps over blocks of 64 tiles x kChunkGroup chunks (lanes = tiles, each wave
-// a quarter of the group's chunks), so every step has C/64 x tiles/64 blocks.
+// Each job's chunk x tile count matrix is scanned down each tile's column in
+// three steps over blocks of 64 tiles x kChunkGroup chunks (lanes = tiles, each
+// wave a quarter of the group's chunks), so every step has C/64 x tiles/64
+// blocks per job (grid z = job; blocks beyond a smaller job's matrix exit).
 namespace {
 // Per-wave partial sums of one (64-tile, chunk-group) block, combined in LDS;
 // returns this wave's exclusive base within the group and the group total.
@@ -610,7 +642,7 @@ __device__ __forceinline__ void colgroup_sums(const FrameParams& P, const Device
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     uint32_t sum = 0;
     if (tile < P.num_tiles)
-        for (uint32_t c = c0; c < c1; c++) sum += B.bin_counts[(size_t)c * P.num_tiles + tile];
+        for (uint32_t c = c0; c < c1; c++) sum += B.bin_counts[P.count_base + (size_t)c * P.num_tiles + tile];
     part[wave][lane] = sum;
     __syncthreads();
     waveBase = 0;
@@ -627,42 +659,56 @@ __device__ __forceinline__ void colgroup_range(const FrameParams& P, uint32_t& c
 }
 }  // namespace
 
+// Whether block (x, y) of the column-scan grid lies inside job P's matrix.
+__device__ __forceinline__ bool colgroup_in_job(const FrameParams& P) {
+    return blockIdx.x * kWave < P.num_tiles && blockIdx.y * kChunkGroup < P.num_chunks;
+}
+
 // Step 1: per (tile, chunk group) column sums.
-__global__ __launch_bounds__(kBlock) void k_bin_colsum(FrameParams P, DeviceBuffers B) {
+__global__ __launch_bounds__(kBlock) void k_bin_colsum(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                       DeviceBuffers B) {
     __shared__ uint32_t part[kBlock / kWave][kWave];
+    const FrameParams& P = jobs[blockIdx.z];
+    if (!colgroup_in_job(P)) return;
     const uint32_t tile = blockIdx.x * kWave + __lane_id();
     uint32_t c0, c1, waveBase, groupTotal;
     colgroup_range(P, c0, c1);
     colgroup_sums(P, B, tile, c0, c1, part, waveBase, groupTotal);
-    if (threadIdx.x < kWave && tile < P.num_tiles) B.group_sums[(size_t)blockIdx.y * P.num_tiles + tile] = groupTotal;
+    if (threadIdx.x < kWave && tile < P.num_tiles)
+        B.group_sums[P.group_base + (size_t)blockIdx.y * P.num_tiles + tile] = groupTotal;
 }
 
 // Step 2: per tile, exclusive scan over the chunk groups and the tile total.
-__global__ __launch_bounds__(kBlock) void k_bin_groupscan(FrameParams P, DeviceBuffers B) {
+__global__ __launch_bounds__(kBlock) void k_bin_groupscan(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                          DeviceBuffers B) {
+    const FrameParams& P = jobs[blockIdx.z];
     const uint32_t tile = blockIdx.x * kBlock + threadIdx.x;
     if (tile >= P.num_tiles) return;
     const uint32_t G = (P.num_chunks + kChunkGroup - 1) / kChunkGroup;
     uint32_t run = 0;
     for (uint32_t g = 0; g < G; g++) {
-        uint32_t* p = &B.group_sums[(size_t)g * P.num_tiles + tile];
+        uint32_t* p = &B.group_sums[P.group_base + (size_t)g * P.num_tiles + tile];
         const uint32_t v = *p;
         *p = run;
         run += v;
     }
-    B.tile_totals[tile] = run;
+    B.tile_totals[P.tile_base + tile] = run;
 }
 
 // Step 3: rewrite the counts in place as offsets within the tile's list.
-__global__ __launch_bounds__(kBlock) void k_bin_colscan(FrameParams P, DeviceBuffers B) {
+__global__ __launch_bounds__(kBlock) void k_bin_colscan(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                        DeviceBuffers B) {
     __shared__ uint32_t part[kBlock / kWave][kWave];
+    const FrameParams& P = jobs[blockIdx.z];
+    if (!colgroup_in_job(P)) return;
     const uint32_t tile = blockIdx.x * kWave + __lane_id();
     uint32_t c0, c1, waveBase, groupTotal;
     colgroup_range(P, c0, c1);
     colgroup_sums(P, B, tile, c0, c1, part, waveBase, groupTotal);
     if (tile >= P.num_tiles) return;
-    uint32_t run = B.group_sums[(size_t)blockIdx.y * P.num_tiles + tile] + waveBase;
+    uint32_t run = B.group_sums[P.group_base + (size_t)blockIdx.y * P.num_tiles + tile] + waveBase;
     for (uint32_t c = c0; c < c1; c++) {
-        const size_t o = (size_t)c * P.num_tiles + tile;
+        const size_t o = P.count_base + (size_t)c * P.num_tiles + tile;
         const uint32_t v = B.bin_counts[o];
         B.bin_counts[o] = run;
         run += v;
@@ -694,22 +740,26 @@ __device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* wsum, uint32_
 __device__ __forceinline__ uint32_t tile_segments(uint32_t n) { return (n + kSegRefs - 1) / kSegRefs; }
 }  // namespace
 
-// Exclusive scan of the tile totals (num_tiles <= 16384, one block of 1024),
-// and the raster work list: one item per (tile, segment of <= kSegRefs refs).
-__global__ __launch_bounds__(1024) void k_bin_tilescan(FrameParams P, DeviceBuffers B) {
+// Exclusive scan of the tile totals over every job's tiles (global tiles <=
+// 2^18, one block of 1024), and the raster work list: one item per (tile,
+// segment of <= kSegRefs refs).
+__global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuffers B) {
     __shared__ uint32_t wsum[16];
-    const uint32_t per = (P.num_tiles + 1023) / 1024;
-    const uint32_t t0 = min(P.num_tiles, threadIdx.x * per), t1 = min(P.num_tiles, t0 + per);
+    const uint32_t nt = Q.total_tiles;
+    const uint32_t per = (nt + 1023) / 1024;
+    const uint32_t t0 = min(nt, threadIdx.x * per), t1 = min(nt, t0 + per);
     uint32_t localRefs = 0, localSegs = 0;
+    bool segsFit = true;
     for (uint32_t t = t0; t < t1; t++) {
         const uint32_t n = B.tile_totals[t];
         localRefs += n;
         localSegs += tile_segments(n);
+        segsFit &= tile_segments(n) <= (1u << (32 - kTileBits));
     }
     uint32_t total, totalSegs;
     uint32_t run = scan1024(localRefs, wsum, total);
     uint32_t seg = scan1024(localSegs, wsum, totalSegs);
-    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap;
+    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && __syncthreads_and(segsFit);
     for (uint32_t t = t0; t < t1; t++) {
         const uint32_t n = B.tile_totals[t];
         B.tile_start[t] = run;
@@ -717,12 +767,12 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(FrameParams P, DeviceBuff
         if (fits) {
             const uint32_t ns = tile_segments(n);
             B.tile_item[t] = seg;
-            for (uint32_t s = 0; s < ns; s++) B.work[seg + s] = t | (s << 14);
+            for (uint32_t s = 0; s < ns; s++) B.work[seg + s] = t | (s << kTileBits);
             seg += ns;
         }
     }
     if (threadIdx.x == 0) {
-        B.tile_start[P.num_tiles] = total;
+        B.tile_start[nt] = total;
         B.counters[kCntTotalRefs] = total;
         B.counters[kCntWorkItems] = totalSegs;
         if (!fits) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvRefs);
@@ -732,18 +782,20 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(FrameParams P, DeviceBuff
 // Order-preserving scatter.  Each block walks its chunk in rounds of 256
 // slots; the refs of a round are enumerated in submission order in windows of
 // 256, ranked per tile with wave ballots and written at the tile's cursor.
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(FrameParams P, DeviceBuffers B) {
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                        DeviceBuffers B) {
     extern __shared__ uint32_t cursor[];
     __shared__ uint32_t stageTile[kBlock], stagePay[kBlock];
     __shared__ uint32_t wsum[kBlock / kWave];
     if (B.counters[kCntOverflow] & kOvRefs) return;
-    const uint32_t chunk = blockIdx.x;
-    const uint32_t* offs = B.bin_counts + (size_t)chunk * P.num_tiles;
-    for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) cursor[i] = B.tile_start[i] + offs[i];
+    const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
+    const uint32_t chunk = blockIdx.x - P.chunk_base;
+    const uint32_t* offs = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
+    for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) cursor[i] = B.tile_start[P.tile_base + i] + offs[i];
     __syncthreads();
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
-    const uint32_t s0 = chunk * P.chunk_slots;
-    const uint32_t s1 = min(P.num_triangles, s0 + P.chunk_slots);
+    const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
+    const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
     for (uint32_t rb = s0; rb < s1; rb += kBlock) {
         const uint32_t s = rb + threadIdx.x;
         const uint32_t desc = s < s1 ? B.desc[s] : 0u;
@@ -1063,16 +1115,18 @@ __device__ __forceinline__ RTri broadcast_rtri(const RTri& t, int l) {
 // tile.  Each wave streams its own 64-reference slices through the tile's LDS
 // key buffer (no workgroup barrier per slice): small triangles take one lane
 // each, large ones the whole wave.  The item's keys go to partials[workIndex].
-__device__ __forceinline__ void raster_item(const FrameParams& P, const DeviceBuffers& B, uint32_t workIndex,
-                                            uint64_t* keys) {
-    const uint32_t workItem = B.work[workIndex];
-    const uint32_t tile = workItem & 0x3FFFu, seg = workItem >> 14;
+__device__ __forceinline__ void raster_item(const BatchParams& Q, const FrameParams* __restrict__ jobs,
+                                            const DeviceBuffers& B, uint32_t workIndex, uint64_t* keys) {
+    const uint32_t workItem = __builtin_amdgcn_readfirstlane(B.work[workIndex]);
+    const uint32_t gtile = workItem & (kMaxBatchTiles - 1), seg = workItem >> kTileBits;
+    const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
+    const uint32_t tile = gtile - P.tile_base;
     const int tx = (int)(tile % P.tiles_x), ty = (int)(tile / P.tiles_x);
     const int tminX = tx * kTileSize, tminY = ty * kTileSize;
     const int tmaxX = min(tminX + kTileSize, (int)P.width), tmaxY = min(tminY + kTileSize, (int)P.height);
     for (uint32_t i = threadIdx.x; i < kTileSize * kTileSize; i += kBlock) keys[i] = bg_key();
-    const uint32_t start = B.tile_start[tile];
-    const uint32_t n = B.tile_start[tile + 1] - start;
+    const uint32_t start = B.tile_start[gtile];
+    const uint32_t n = B.tile_start[gtile + 1] - start;
     const uint32_t segEnd = min(n, (seg + 1) * kSegRefs);
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     __syncthreads();
@@ -1136,7 +1190,8 @@ __device__ __forceinline__ void raster_item(const FrameParams& P, const DeviceBu
 
 // Persistent raster kernel: workgroups pull (tile, segment) items from the
 // list built by k_bin_tilescan until it is exhausted.
-__global__ __launch_bounds__(kBlock) void k_raster(FrameParams P, DeviceBuffers B) {
+__global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                   DeviceBuffers B) {
     __shared__ uint64_t keys[kTileSize * kTileSize];
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
@@ -1144,9 +1199,9 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams P, DeviceBuffers 
     for (;;) {
         if (threadIdx.x == 0) sItem = atomicAdd(&B.counters[kCntWorkNext], 1u);
         __syncthreads();
-        const uint32_t item = sItem;
+        const uint32_t item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= nItems) break;
-        raster_item(P, B, item, keys);
+        raster_item(Q, jobs, B, item, keys);
         __syncthreads();
     }
 }
@@ -1156,19 +1211,22 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams P, DeviceBuffers 
 // sequential d <= D test is an associative minimum), then perspective-correct
 // interpolation, the pixel shader and one write of colour, depth and
 // triangle ID (Core/Renderer.cpp:272-350).
-__global__ __launch_bounds__(1024) void k_shade(FrameParams P, MeshPtrs M, DeviceBuffers B) {
+__global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                 DeviceBuffers B) {
     constexpr uint32_t kPix = kTileSize * kTileSize;
     if (B.counters[kCntOverflow] & kOvRefs) return;
-    const uint32_t tile = blockIdx.x;
+    const uint32_t gtile = blockIdx.x;
+    const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
+    const uint32_t tile = gtile - P.tile_base;
     const int x = (int)(tile % P.tiles_x) * kTileSize + (int)(threadIdx.x & 31u);
     const int y = (int)(tile / P.tiles_x) * kTileSize + (int)(threadIdx.x >> 5);
     if (x >= (int)P.width || y >= (int)P.height) return;
-    const uint32_t start = B.tile_start[tile];
-    const uint32_t n = B.tile_start[tile + 1] - start;
+    const uint32_t start = B.tile_start[gtile];
+    const uint32_t n = B.tile_start[gtile + 1] - start;
     uint64_t key = bg_key();
     if (n) {
         const uint32_t nseg = tile_segments(n);
-        const uint64_t* g = B.partials + (size_t)B.tile_item[tile] * kPix + threadIdx.x;
+        const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * kPix + threadIdx.x;
         key = g[0];
         uint32_t s = 1;
         for (; s + 4 <= nseg; s += 4) {  // four independent loads in flight
@@ -1183,7 +1241,7 @@ __global__ __launch_bounds__(1024) void k_shade(FrameParams P, MeshPtrs M, Devic
             key = v < key ? v : key;
         }
     }
-    const size_t o = (size_t)(P.height - 1 - y) * P.width + x;
+    const size_t o = P.pix_base + (size_t)(P.height - 1 - y) * P.width + x;
     if (key == bg_key()) {
         B.color[o] = 0u;
         B.depth[o] = 1.0f;
@@ -1192,7 +1250,7 @@ __global__ __launch_bounds__(1024) void k_shade(FrameParams P, MeshPtrs M, Devic
     }
     // The key's low word is the winner's submission order (make_order).
     const uint32_t order = 0xFFFFFFFEu - (uint32_t)key;
-    const uint32_t tri = order >> 4;
+    const uint32_t tri = order >> 4;  // triangle slot
     uint32_t v0i, v1i, v2i, ri;
     if (order & 1u) {
         ri = (B.desc[tri] >> 3) + ((order >> 1) & 7u);
@@ -1202,9 +1260,10 @@ __global__ __launch_bounds__(1024) void k_shade(FrameParams P, MeshPtrs M, Devic
         v2i = s.v2;
     } else {
         ri = tri;
-        v0i = M.idx[3 * (size_t)tri];
-        v1i = M.idx[3 * (size_t)tri + 1];
-        v2i = M.idx[3 * (size_t)tri + 2];
+        const uint32_t* idx = P.mesh.idx + 3 * (size_t)(tri - P.slot_base);
+        v0i = P.vert_base + idx[0];
+        v1i = P.vert_base + idx[1];
+        v2i = P.vert_base + idx[2];
     }
     RTri t;
     load_rtri(B.rrec[ri], t);
@@ -1213,53 +1272,52 @@ __global__ __launch_bounds__(1024) void k_shade(FrameParams P, MeshPtrs M, Devic
     const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
     B.color[o] = shade_pixel(P, l0, l1, v0, v1, v2);
     B.depth[o] = d;
-    B.ids[o] = tri;
+    B.ids[o] = tri - P.slot_base;
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_reset(uint32_t* counters, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, counters);
-    return hipGetLastError();
+hipError_t init_kernels() {
+    // Bin cursors for up to kMaxTiles tiles live in LDS (160 KB per CU on gfx950).
+    const int maxDyn = (int)(sizeof(uint32_t) * kMaxTiles);
+    hipError_t e = hipFuncSetAttribute((const void*)k_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
+    return e;
 }
 
-hipError_t launch_frame(const FrameParams& P, const MeshPtrs& M, const DeviceBuffers& B, hipStream_t s,
-                        hipEvent_t* events) {
-    const size_t lds = sizeof(uint32_t) * P.num_tiles;
-    static bool ldsAttr = false;
-    if (!ldsAttr) {
-        // Bin cursors for up to kMaxTiles tiles live in LDS (160 KB per CU on gfx950).
-        const int maxDyn = (int)(sizeof(uint32_t) * kMaxTiles);
-        (void)hipFuncSetAttribute((const void*)k_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
-        (void)hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
-        ldsAttr = true;
-    }
+hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FrameParams* job0, const DeviceBuffers& B,
+                        hipStream_t s, hipEvent_t* events) {
+    const size_t lds = sizeof(uint32_t) * Q.max_job_tiles;
     auto mark = [&](int i) {
         if (events) (void)hipEventRecord(events[i], s);
     };
+    const FrameParams none{};
     mark(0);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, B.counters);
-    if (P.num_vertices)
-        hipLaunchKernelGGL(k_vertex, dim3((P.num_vertices + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, M, B.verts);
-    if (P.num_triangles) {
-        hipLaunchKernelGGL(k_setup, dim3((P.num_triangles + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, M, B);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, B.counters, jobs, job0 ? *job0 : none, job0 ? 1 : 0);
+    if (Q.vert_blocks) hipLaunchKernelGGL(k_vertex, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts);
+    if (Q.slot_blocks) {
+        hipLaunchKernelGGL(k_setup, dim3(Q.slot_blocks), dim3(kBlock), 0, s, Q, jobs, B);
         const uint32_t clipBlocks = max(1u, min(1024u, (B.clip_queue_cap + 63) / 64));
-        hipLaunchKernelGGL(k_clip, dim3(clipBlocks), dim3(64), 0, s, P, M, B);
+        hipLaunchKernelGGL(k_clip, dim3(clipBlocks), dim3(64), 0, s, Q, jobs, B);
     }
     mark(1);
-    hipLaunchKernelGGL(k_bin_count, dim3(P.num_chunks), dim3(kBlock), lds, s, P, B);
+    if (Q.total_chunks) hipLaunchKernelGGL(k_bin_count, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
     mark(2);
-    const dim3 colGrid((P.num_tiles + kWave - 1) / kWave, (P.num_chunks + kChunkGroup - 1) / kChunkGroup);
-    hipLaunchKernelGGL(k_bin_colsum, colGrid, dim3(kBlock), 0, s, P, B);
-    hipLaunchKernelGGL(k_bin_groupscan, dim3((P.num_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, B);
-    hipLaunchKernelGGL(k_bin_colscan, colGrid, dim3(kBlock), 0, s, P, B);
-    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, P, B);
+    const dim3 colGrid((Q.max_job_tiles + kWave - 1) / kWave, (Q.max_job_chunks + kChunkGroup - 1) / kChunkGroup,
+                       Q.num_jobs);
+    // The group scan also writes the tile totals of jobs without triangles.
+    if (Q.total_chunks) hipLaunchKernelGGL(k_bin_colsum, colGrid, dim3(kBlock), 0, s, Q, jobs, B);
+    hipLaunchKernelGGL(k_bin_groupscan, dim3((Q.max_job_tiles + kBlock - 1) / kBlock, 1, Q.num_jobs), dim3(kBlock), 0,
+                       s, Q, jobs, B);
+    if (Q.total_chunks) hipLaunchKernelGGL(k_bin_colscan, colGrid, dim3(kBlock), 0, s, Q, jobs, B);
+    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, Q, B);
     mark(3);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(P.num_chunks), dim3(kBlock), lds, s, P, B);
+    if (Q.total_chunks) hipLaunchKernelGGL(k_bin_scatter, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
     mark(4);
-    hipLaunchKernelGGL(k_raster, dim3(B.raster_blocks), dim3(kBlock), 0, s, P, B);
-    hipLaunchKernelGGL(k_shade, dim3(P.num_tiles), dim3(1024), 0, s, P, M, B);
+    hipLaunchKernelGGL(k_raster, dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
+    hipLaunchKernelGGL(k_shade, dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
     mark(5);
     return hipGetLastError();
 }

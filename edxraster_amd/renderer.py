@@ -228,6 +228,62 @@ class Renderer:
         _check(self._ctx, L.lib().edxr_get_stats(self._ctx, C.byref(s)))
         return {name: getattr(s, name) for name, _ in L.Stats._fields_}
 
+    # ---- batched rendering (edxr_render_batch) ----
+    @staticmethod
+    def make_job(mesh, width, height, mv, proj, raster, light=(1.0, 1.0, -1.0), shader=L.SHADER_VC_BLINN_PHONG,
+                 material=None, hierarchical=True):
+        """edxr_job for a batch; mesh is a Mesh (host arrays, kept alive by the
+        caller) or an edxr_mesh structure (e.g. device pointers)."""
+        j = L.Job()
+        j.mesh = mesh.c_struct() if isinstance(mesh, Mesh) else mesh
+        j.width, j.height = int(width), int(height)
+        j.mv[:] = [float(x) for x in _mat(mv)]
+        j.proj[:] = [float(x) for x in _mat(proj)]
+        j.raster[:] = [float(x) for x in _mat(raster)]
+        j.light[:] = [float(x) for x in light]
+        j.shader = shader
+        j.material = material if material is not None else default_material(shader)
+        j.hierarchical = 1 if hierarchical else 0
+        return j
+
+    def render_batch(self, jobs):
+        """Renders a list (or ctypes array) of edxr_job in one launch sequence."""
+        arr = jobs if isinstance(jobs, C.Array) else (L.Job * len(jobs))(*jobs)
+        self._batch_sizes = [(j.width, j.height) for j in arr]
+        _check(self._ctx, L.lib().edxr_render_batch(self._ctx, arr, len(arr)))
+
+    def batch_read(self, k):
+        """(rgba (H,W,4) uint8, depth (H,W) float32, ids (H,W) uint32) of job k."""
+        w, h = self._batch_sizes[k]
+        rgba = np.empty((h, w, 4), np.uint8)
+        depth = np.empty((h, w), np.float32)
+        ids = np.empty((h, w), np.uint32)
+        _check(self._ctx, L.lib().edxr_batch_read(self._ctx, k, rgba.ctypes.data, depth.ctypes.data,
+                                                  ids.ctypes.data))
+        return rgba, depth, ids
+
+    def batch_device_buffers(self):
+        n = len(self._batch_sizes)
+        c, d, i = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        offs = (C.c_uint64 * (n + 1))()
+        _check(self._ctx, L.lib().edxr_batch_device_buffers(self._ctx, C.byref(c), C.byref(d), C.byref(i), offs))
+        return c.value, d.value, i.value, list(offs)
+
+    def set_graph(self, on):
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_GRAPH, 1 if on else 0))
+
+    def set_tuning(self, tris_per_chunk=0, raster_blocks_per_cu=0):
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_TRIS_PER_CHUNK, int(tris_per_chunk)))
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_RASTER_BLOCKS_PER_CU, int(raster_blocks_per_cu)))
+
+    def autotune(self, jobs):
+        arr = jobs if isinstance(jobs, C.Array) else (L.Job * len(jobs))(*jobs)
+        t = L.Tuning()
+        _check(self._ctx, L.lib().edxr_autotune(self._ctx, arr, len(arr), C.byref(t)))
+        self._batch_sizes = [(j.width, j.height) for j in arr]
+        return {"tris_per_chunk": t.tris_per_chunk, "raster_blocks_per_cu": t.raster_blocks_per_cu,
+                "frame_ms": t.frame_ms}
+
     def debug_records(self):
         """Setup records in submission order, (N, 16) uint32 (see edxr_debug_read_records)."""
         n = L.lib().edxr_debug_read_records(self._ctx, None, 0)

@@ -69,7 +69,14 @@ enum edxr_state_key {
     /* Bin chunks used by the order-preserving binning (0 = automatic). */
     EDXR_STATE_BIN_CHUNKS = 2,
     /* 1 = record HIP events around each pipeline stage of every frame. */
-    EDXR_STATE_PROFILE = 3
+    EDXR_STATE_PROFILE = 3,
+    /* 1 = capture each frame's launches in a HIP graph and replay it while the
+     * launch layout (sizes, capacities, tuning) repeats; default 0. */
+    EDXR_STATE_GRAPH = 4,
+    /* Triangles per binning chunk (0 = automatic, 1024). */
+    EDXR_STATE_TRIS_PER_CHUNK = 5,
+    /* Resident raster workgroups per compute unit (0 = automatic, 8). */
+    EDXR_STATE_RASTER_BLOCKS_PER_CU = 6
 };
 
 /* Pipeline stages timed when EDXR_STATE_PROFILE is on. */
@@ -85,8 +92,8 @@ enum edxr_stage {
 enum edxr_memory { EDXR_MEM_HOST = 0, EDXR_MEM_DEVICE = 1 };
 
 /* One mesh (Utils/Mesh.h:17-69, Utils/InputBuffer.h:43-194), SoA fp32.
- * Normals and colours may be NULL for EDXR_SHADER_LAMBERT_ALBEDO /
- * EDXR_SHADER_BLINN_PHONG_REF (they are then read as (0,0,0)). */
+ * Positions, normals and indices are required; colours may be NULL except for
+ * EDXR_SHADER_VC_BLINN_PHONG (they are then read as (0,0,0)). */
 typedef struct edxr_mesh {
     const float *px, *py, *pz;
     const float *nx, *ny, *nz;
@@ -109,6 +116,26 @@ typedef struct edxr_stats {
     uint32_t retries;            /* re-runs after a capacity overflow */
     uint32_t bin_chunks;
 } edxr_stats;
+
+/* One independent render job of a batch: a mesh, a camera, a light, a pixel
+ * shader and a W x H render target (the state one Renderer::RenderMesh call
+ * uses).  Jobs may share one mesh. */
+typedef struct edxr_job {
+    edxr_mesh mesh;
+    uint32_t width, height;          /* <= 8192, at most 16384 tiles */
+    float mv[16], proj[16], raster[16];
+    float light[3];
+    int32_t shader;                  /* EDXR_SHADER_* */
+    edxr_material material;
+    int32_t hierarchical;            /* as EDXR_STATE_HIERARCHICAL */
+} edxr_job;
+
+/* Launch parameters picked by edxr_autotune. */
+typedef struct edxr_tuning {
+    uint32_t tris_per_chunk;
+    uint32_t raster_blocks_per_cu;
+    double frame_ms;                 /* best measured time */
+} edxr_tuning;
 
 typedef struct edxr_ctx edxr_ctx;
 
@@ -151,6 +178,23 @@ int edxr_profile_reset(edxr_ctx* ctx);
 /* The HIP stream (hipStream_t) the context launches on. */
 void* edxr_stream(edxr_ctx* ctx);
 const char* edxr_last_error(const edxr_ctx* ctx);
+
+/* ---- batched rendering (no counterpart in the reference) ---- */
+/* Renders num_jobs independent jobs with one sequence of launches: every
+ * kernel covers all jobs at once through a device job table, so a batch costs
+ * about as many launches as one frame.  Synchronous.  Each job's output has
+ * the layout of edxr_read_color / _depth / _triangle_ids at its own size. */
+int edxr_render_batch(edxr_ctx* ctx, const edxr_job* jobs, uint32_t num_jobs);
+/* Copies job `job` of the last batch to caller memory; any pointer may be
+ * NULL.  Sizes are width * height (* 4 bytes for rgba). */
+int edxr_batch_read(edxr_ctx* ctx, uint32_t job, uint8_t* rgba, float* depth, uint32_t* ids);
+/* Device addresses of the last batch's colour, depth and ID arrays and, when
+ * pixel_offsets is non-NULL, the first pixel of every job (num_jobs + 1). */
+int edxr_batch_device_buffers(edxr_ctx* ctx, void** color, void** depth, void** ids, uint64_t* pixel_offsets);
+/* Times candidate (triangles per chunk, raster workgroups per CU) pairs on the
+ * given batch (or single job, num_jobs = 1), keeps the fastest in the
+ * context's state and returns it in *out (may be NULL). */
+int edxr_autotune(edxr_ctx* ctx, const edxr_job* jobs, uint32_t num_jobs, edxr_tuning* out);
 
 /* ---- introspection for parity tests (no counterpart in the reference) ---- */
 /* Setup records in submission order: per record 16 uint32 words
