@@ -27,6 +27,7 @@ STATE_PROFILE = 3
 STATE_GRAPH = 4
 STATE_TRIS_PER_CHUNK = 5
 STATE_RASTER_BLOCKS_PER_CU = 6
+STATE_FLOAT_COLOR = 7
 STAGES = ("setup", "bin_count", "bin_scan", "bin_scatter", "raster")
 
 MEM_HOST = 0
@@ -58,6 +59,11 @@ class Job(C.Structure):
                 ("mv", C.c_float * 16), ("proj", C.c_float * 16), ("raster", C.c_float * 16),
                 ("light", C.c_float * 3), ("shader", C.c_int32), ("material", Material),
                 ("hierarchical", C.c_int32)]
+
+
+class Grads(C.Structure):
+    _fields_ = [(n, F) for n in ("d_px", "d_py", "d_pz", "d_nx", "d_ny", "d_nz", "d_cr", "d_cg", "d_cb",
+                                 "d_light", "d_material")]
 
 
 class Tuning(C.Structure):
@@ -96,6 +102,9 @@ SIGNATURES = [
     ("edxr_batch_device_buffers", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                             C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
     ("edxr_autotune", C.c_int, [C.c_void_p, C.POINTER(Job), C.c_uint32, C.POINTER(Tuning)]),
+    ("edxr_float_color", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("edxr_copy_float_color", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("edxr_backward", C.c_int, [C.c_void_p, F, C.POINTER(Grads), C.c_uint32]),
     ("edxr_debug_read_records", C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64]),
     ("edxr_debug_read_bins", C.c_int64, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("edxr_scene_sphere", C.c_int, [C.POINTER(HostMesh), C.c_uint32, C.c_uint32, C.c_float, F, C.c_uint64]),

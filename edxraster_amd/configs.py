@@ -123,6 +123,36 @@ def batch_jobs(n_jobs=256, seed_base=4000, tmin=1e3, tmax=5e5, resolutions=(256,
     return jobs
 
 
+def diff_scene(n_views=64, res=512, hf_n=71, sphere_segments=(80, 64), shininess=32.0, light=(1.0, 1.0, -1.0)):
+    """Sphere above a heightfield, seen from a ring of views.  A 71x71
+    heightfield (9,800 triangles, amplitude 0.2, seeds 5/55) and an 80x64 UV
+    sphere (10,240 triangles, r = 0.35, centre (0, 0.55, 0), colour seed 56)
+    merged into one mesh; view k looks at (0, 0.15, 0) from distance 2.4 at
+    elevation 30 degrees and azimuth 5.625 k degrees, res x res pixels."""
+    hf = Mesh.load_heightfield(hf_n, 0.2, height_seed=5, color_seed=55)
+    sph = Mesh.load_sphere(sphere_segments[0], sphere_segments[1], 0.35, (0.0, 0.55, 0.0), color_seed=56)
+    mesh = Mesh.merge(hf, sph)
+    views = []
+    el = np.radians(30.0)
+    target = np.array([0.0, 0.15, 0.0])
+    for k in range(n_views):
+        az = np.radians(5.625 * k)
+        eye = target + 2.4 * np.array([np.cos(el) * np.sin(az), np.sin(el), -np.cos(el) * np.cos(az)])
+        mv, p, r = camera(res, res, eye, target)
+        views.append(Workload("diff_view%d_%dx%d" % (k, res, res), res, res, mesh, mv, p, r, light=light,
+                              shininess=shininess))
+    return mesh, views
+
+
+def rotate(v, axis, deg):
+    """Rodrigues rotation of v about axis by deg degrees."""
+    v = np.asarray(v, np.float64)
+    k = np.asarray(axis, np.float64)
+    k = k / np.linalg.norm(k)
+    t = np.radians(deg)
+    return v * np.cos(t) + np.cross(k, v) * np.sin(t) + k * np.dot(k, v) * (1.0 - np.cos(t))
+
+
 WORKLOADS = {
     "small": small_sphere,
     "grid1m": sphere_grid_1080p,

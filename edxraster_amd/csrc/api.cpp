@@ -159,6 +159,7 @@ struct edxr_ctx {
     uint32_t trisPerChunk = 0, rasterPerCU = 0;
     int32_t profile = 0;
     int32_t useGraph = 0;
+    int32_t floatColor = 0;
     hipEvent_t events[kNumStages + 1] = {};
     double stageMs[kNumStages] = {};
     uint64_t profiledFrames = 0;
@@ -183,6 +184,13 @@ struct edxr_ctx {
     // batch outputs
     DevBuf<uint32_t> bcolor, bids;
     DevBuf<float> bdepth;
+    DevBuf<float> colorF, bcolorF;  // float RGB images (EDXR_STATE_FLOAT_COLOR)
+    // adjoint
+    Plan lastPlan;                  // layout of the last render (tables stay resident)
+    bool lastPlanValid = false, lastFloat = false;
+    DevBuf<GradPtrs> gradsDev;
+    HostBuf<GradPtrs> gradsHost;
+    DevBuf<double> adjAcc;
     std::vector<uint64_t> batchPix;
     std::vector<uint32_t> batchW, batchH;
     bool batchRendered = false;
@@ -273,6 +281,7 @@ FrameParams job_params(const JobInput& in, uint32_t tilesX, uint32_t tilesY) {
     P.light[0] = L[0] * s;
     P.light[1] = L[1] * s;
     P.light[2] = L[2] * s;
+    P.light_len = 1.0f / s;
     P.ambient = in.mat.ambient;
     P.kd = in.mat.kd;
     P.ks = in.mat.ks;
@@ -375,7 +384,7 @@ void drop_graph(edxr_ctx* c) {
 // Runs a planned batch (sized, launched, checked; re-run with larger buffers
 // after a capacity overflow).  For a single job the job table is written by
 // the reset kernel, otherwise it is uploaded from pinned memory.
-int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids) {
+int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids, float* colorF) {
     hipError_t e;
     BatchParams& Q = pl.Q;
     const bool single = Q.num_jobs == 1;
@@ -454,6 +463,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.color = color;
         B.depth = depth;
         B.ids = ids;
+        B.color_f32 = colorF;
 
         const bool prof = c->profile && !graph;
         if (graph) {
@@ -639,6 +649,11 @@ void edxr_destroy(edxr_ctx* c) {
         b->release();
     c->depth.release();
     c->bdepth.release();
+    c->colorF.release();
+    c->bcolorF.release();
+    c->gradsDev.release();
+    c->gradsHost.release();
+    c->adjAcc.release();
     c->partials.release();
     c->slotRefs.release();
     c->verts.release();
@@ -707,6 +722,7 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
                 return c->fail(EDXR_E_INVALID, "triangles per chunk out of range (0 or 64..2^20)");
             c->trisPerChunk = (uint32_t)value;
             return EDXR_OK;
+        case EDXR_STATE_FLOAT_COLOR: c->floatColor = value ? 1 : 0; return EDXR_OK;
         case EDXR_STATE_RASTER_BLOCKS_PER_CU:
             if (value < 0 || value > 32) return c->fail(EDXR_E_INVALID, "raster blocks per CU out of range (0..32)");
             c->rasterPerCU = (uint32_t)value;
@@ -763,7 +779,14 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     Plan pl;
     if ((r = build_plan(c, std::vector<JobInput>{in}, pl)) != EDXR_OK) return r;
     c->rendered = false;
-    if ((r = run_plan(c, pl, c->color.p, c->depth.p, c->ids.p)) != EDXR_OK) return r;
+    c->lastPlanValid = false;
+    if (c->floatColor && (e = c->colorF.ensure(3ull * c->W * c->H)) != hipSuccess)
+        return c->hip_fail(e, "hipMalloc(float colour)");
+    if ((r = run_plan(c, pl, c->color.p, c->depth.p, c->ids.p, c->floatColor ? c->colorF.p : nullptr)) != EDXR_OK)
+        return r;
+    c->lastPlan = pl;
+    c->lastPlanValid = true;
+    c->lastFloat = c->floatColor != 0;
     c->rendered = true;
     c->batchRendered = false;
     c->lastT = T;
@@ -811,9 +834,17 @@ int edxr_render_batch(edxr_ctx* c, const edxr_job* jobs, uint32_t n) {
     if ((e = c->bcolor.ensure(pl.pixels)) != hipSuccess || (e = c->bdepth.ensure(pl.pixels)) != hipSuccess ||
         (e = c->bids.ensure(pl.pixels)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(batch frame buffers)");
+    if (c->floatColor && (e = c->bcolorF.ensure(3 * pl.pixels)) != hipSuccess)
+        return c->hip_fail(e, "hipMalloc(batch float colour)");
     c->batchRendered = false;
     c->rendered = false;
-    if ((r = run_plan(c, pl, c->bcolor.p, c->bdepth.p, c->bids.p)) != EDXR_OK) return r;
+    c->lastPlanValid = false;
+    if ((r = run_plan(c, pl, c->bcolor.p, c->bdepth.p, c->bids.p, c->floatColor ? c->bcolorF.p : nullptr)) !=
+        EDXR_OK)
+        return r;
+    c->lastPlan = pl;
+    c->lastPlanValid = true;
+    c->lastFloat = c->floatColor != 0;
     c->batchPix.resize(n + 1);
     c->batchW.resize(n);
     c->batchH.resize(n);
@@ -848,6 +879,55 @@ int edxr_batch_device_buffers(edxr_ctx* c, void** color, void** depth, void** id
     if (ids) *ids = c->bids.p;
     if (offsets)
         for (size_t k = 0; k < c->batchPix.size(); k++) offsets[k] = c->batchPix[k];
+    return EDXR_OK;
+}
+
+int edxr_float_color(edxr_ctx* c, void** rgb) {
+    if (!c || !rgb) return EDXR_E_INVALID;
+    if (!c->lastPlanValid || !c->lastFloat)
+        return c->fail(EDXR_E_STATE, "no render with EDXR_STATE_FLOAT_COLOR on");
+    *rgb = c->rendered ? (void*)c->colorF.p : (void*)c->bcolorF.p;
+    return EDXR_OK;
+}
+
+int edxr_copy_float_color(edxr_ctx* c, void* dst, size_t bytes) {
+    if (!c || !dst) return EDXR_E_INVALID;
+    void* src = nullptr;
+    const int r = edxr_float_color(c, &src);
+    if (r != EDXR_OK) return r;
+    const uint64_t need = 12ull * c->lastPlan.pixels;
+    if (bytes < need) return c->fail(EDXR_E_INVALID, "buffer too small");
+    hipError_t e = hipMemcpyAsync(dst, src, need, hipMemcpyDefault, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? EDXR_OK : c->hip_fail(e, "copy_float_color");
+}
+
+int edxr_backward(edxr_ctx* c, const float* d_rgb, const edxr_grads* g, uint32_t n) {
+    if (!c || !d_rgb || !g) return EDXR_E_INVALID;
+    if (!c->lastPlanValid) return c->fail(EDXR_E_STATE, "backward before a render");
+    const Plan& pl = c->lastPlan;
+    if (n != pl.Q.num_jobs) return c->fail(EDXR_E_INVALID, "one edxr_grads per job of the last render");
+    for (const FrameParams& P : pl.jobs) {
+        const float* R = P.raster;
+        if (R[2] != 0.0f || R[6] != 0.0f || R[12] != 0.0f || R[13] != 0.0f || R[14] != 0.0f || R[15] != 1.0f ||
+            R[0] * R[5] - R[1] * R[4] == 0.0f)
+            return c->fail(EDXR_E_INVALID, "backward needs a raster matrix affine in x and y");
+    }
+    hipError_t e;
+    if ((e = c->gradsDev.ensure(n)) != hipSuccess || (e = c->gradsHost.ensure(n)) != hipSuccess ||
+        (e = c->adjAcc.ensure(8ull * n)) != hipSuccess)
+        return c->hip_fail(e, "hipMalloc(adjoint)");
+    for (uint32_t k = 0; k < n; k++)
+        c->gradsHost.p[k] = GradPtrs{g[k].d_px, g[k].d_py, g[k].d_pz, g[k].d_nx, g[k].d_ny, g[k].d_nz,
+                                     g[k].d_cr, g[k].d_cg, g[k].d_cb, g[k].d_light, g[k].d_material};
+    if ((e = hipMemcpyAsync(c->gradsDev.p, c->gradsHost.p, sizeof(GradPtrs) * n, hipMemcpyHostToDevice,
+                            c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(grads)");
+    const uint32_t* ids = c->rendered ? c->ids.p : c->bids.p;
+    if ((e = launch_adjoint(pl.Q, c->jobsDev.p, c->gradsDev.p, c->verts.p, ids, d_rgb, c->adjAcc.p, c->stream)) !=
+        hipSuccess)
+        return c->hip_fail(e, "adjoint launch");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
     return EDXR_OK;
 }
 

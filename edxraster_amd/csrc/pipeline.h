@@ -47,6 +47,7 @@ struct FrameParams {
     float raster[16];
     float eye[3];         // TransformPoint(0, MV^-1)
     float light[3];       // Normalize(light direction)
+    float light_len;      // length of the light direction as given
     float ambient, kd, ks, shininess;
     float albedo[3];
     int32_t shader;
@@ -170,6 +171,17 @@ struct DeviceBuffers {
     uint32_t* color;       // every job's pixels at its pix_base
     float* depth;
     uint32_t* ids;
+    float* color_f32;      // optional clamped float RGB (3 per pixel) for the adjoint
+};
+
+// Gradient outputs of one job (device pointers, accumulated with +=; any may
+// be null).  Positions, normals and colours are per input vertex of the job's
+// mesh; light is w.r.t. the light direction as given, material is (ambient,
+// kd, ks, shininess).
+struct GradPtrs {
+    float *d_px, *d_py, *d_pz, *d_nx, *d_ny, *d_nz, *d_cr, *d_cg, *d_cb;
+    float* d_light;
+    float* d_material;
 };
 
 // Stage boundaries recorded when profiling: events[0] before the counter reset,
@@ -181,6 +193,12 @@ constexpr int kNumStages = 5;
 // (a single job, no separate upload); otherwise the table is already resident.
 hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FrameParams* job0, const DeviceBuffers& B,
                         hipStream_t s, hipEvent_t* events);
+// Adjoint of the shading of a rendered batch with visibility held fixed (the
+// triangle-ID buffer): d_rgb holds dLoss/d(clamped float RGB) per pixel in the
+// layout of color_f32; acc is 8 doubles of scratch per job.
+hipError_t launch_adjoint(const BatchParams& Q, const FrameParams* jobs, const GradPtrs* grads,
+                          const ShadeVert* verts, const uint32_t* ids, const float* d_rgb, double* acc,
+                          hipStream_t s);
 // Kernel attributes (dynamic LDS limits); once per process before launches.
 hipError_t init_kernels();
 

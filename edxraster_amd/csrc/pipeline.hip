@@ -940,15 +940,14 @@ __device__ __forceinline__ float pow_decl(float x, int n, float s) {
     return result;
 }
 
-__device__ __forceinline__ uint32_t to_byte(float c) {
-    c = fminf(fmaxf(c, 0.0f), 1.0f);
-    return (uint32_t)(int)(c * 255.0f + 0.5f);
-}
+// Color4b::FromFloats clamp (NaN -> 0) and its byte conversion.
+__device__ __forceinline__ float clamp01(float c) { return fminf(fmaxf(c, 0.0f), 1.0f); }
+__device__ __forceinline__ uint32_t to_byte(float c) { return (uint32_t)(int)(clamp01(c) * 255.0f + 0.5f); }
 
 // Fragment::Interpolate (Core/Shader.h:142-170) + pixel shader
-// (Core/Shader.h:209-282) + Color4b::FromFloats.
-__device__ __forceinline__ uint32_t shade_pixel(const FrameParams& P, float b0, float b1, const ShadeVert& v0,
-                                                const ShadeVert& v1, const ShadeVert& v2) {
+// (Core/Shader.h:209-282); unclamped RGB.
+__device__ __forceinline__ float3 shade_pixel(const FrameParams& P, float b0, float b1, const ShadeVert& v0,
+                                              const ShadeVert& v1, const ShadeVert& v2) {
     float b2 = 1.0f - b0 - b1;
     b0 *= v0.inv_w;
     b1 *= v1.inv_w;
@@ -996,7 +995,7 @@ __device__ __forceinline__ uint32_t shade_pixel(const FrameParams& P, float b0, 
             b = diffuse * Cb + spec;
         }
     }
-    return to_byte(r) | (to_byte(g) << 8) | (to_byte(b) << 16) | (255u << 24);
+    return make_float3(r, g, b);
 }
 
 }  // namespace
@@ -1246,6 +1245,7 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
         B.color[o] = 0u;
         B.depth[o] = 1.0f;
         B.ids[o] = 0xFFFFFFFFu;
+        if (B.color_f32) B.color_f32[3 * o] = B.color_f32[3 * o + 1] = B.color_f32[3 * o + 2] = 0.0f;
         return;
     }
     // The key's low word is the winner's submission order (make_order).
@@ -1270,7 +1270,13 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
     float l0, l1, d;
     bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
     const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
-    B.color[o] = shade_pixel(P, l0, l1, v0, v1, v2);
+    const float3 c = shade_pixel(P, l0, l1, v0, v1, v2);
+    B.color[o] = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
+    if (B.color_f32) {
+        B.color_f32[3 * o] = clamp01(c.x);
+        B.color_f32[3 * o + 1] = clamp01(c.y);
+        B.color_f32[3 * o + 2] = clamp01(c.z);
+    }
     B.depth[o] = d;
     B.ids[o] = tri - P.slot_base;
 }

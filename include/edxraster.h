@@ -76,7 +76,10 @@ enum edxr_state_key {
     /* Triangles per binning chunk (0 = automatic, 1024). */
     EDXR_STATE_TRIS_PER_CHUNK = 5,
     /* Resident raster workgroups per compute unit (0 = automatic, 8). */
-    EDXR_STATE_RASTER_BLOCKS_PER_CU = 6
+    EDXR_STATE_RASTER_BLOCKS_PER_CU = 6,
+    /* 1 = also write the clamped float RGB image (3 floats per pixel, the
+     * function edxr_backward differentiates); default 0. */
+    EDXR_STATE_FLOAT_COLOR = 7
 };
 
 /* Pipeline stages timed when EDXR_STATE_PROFILE is on. */
@@ -129,6 +132,19 @@ typedef struct edxr_job {
     edxr_material material;
     int32_t hierarchical;            /* as EDXR_STATE_HIERARCHICAL */
 } edxr_job;
+
+/* Gradient outputs of one job for edxr_backward: device arrays that are
+ * accumulated into (+=), any of which may be NULL.  Positions, normals and
+ * colours hold one value per input vertex of the job's mesh; d_light (3) is
+ * w.r.t. the light direction as given, d_material (4) w.r.t. (ambient, kd,
+ * ks, shininess).  Jobs that share a mesh may share the arrays. */
+typedef struct edxr_grads {
+    float *d_px, *d_py, *d_pz;
+    float *d_nx, *d_ny, *d_nz;
+    float *d_cr, *d_cg, *d_cb;
+    float *d_light;
+    float *d_material;
+} edxr_grads;
 
 /* Launch parameters picked by edxr_autotune. */
 typedef struct edxr_tuning {
@@ -195,6 +211,22 @@ int edxr_batch_device_buffers(edxr_ctx* ctx, void** color, void** depth, void** 
  * given batch (or single job, num_jobs = 1), keeps the fastest in the
  * context's state and returns it in *out (may be NULL). */
 int edxr_autotune(edxr_ctx* ctx, const edxr_job* jobs, uint32_t num_jobs, edxr_tuning* out);
+
+/* ---- differentiation (no counterpart in the reference) ---- */
+/* Device address of the last render's float RGB image (EDXR_STATE_FLOAT_COLOR
+ * must have been on): 3 floats per pixel, the layout of edxr_read_color for a
+ * single frame, every job at 3 * its pixel offset for a batch. */
+int edxr_float_color(edxr_ctx* ctx, void** rgb);
+/* Copies the last render's float RGB image (all jobs) to dst, host or device
+ * memory, bytes = 12 * total pixels. */
+int edxr_copy_float_color(edxr_ctx* ctx, void* dst, size_t bytes);
+/* Adjoint of the last render (edxr_render_mesh: num_jobs = 1; or
+ * edxr_render_batch) with visibility held fixed: d_rgb (device, the layout
+ * of edxr_float_color) is dLoss/dRGB; gradients are added to grads[job].
+ * Barycentrics are those of the continuous (unsnapped) projection of the
+ * winning input triangle; clamped channels pass no gradient.  Requires a
+ * raster matrix that is affine in x and y (as edxr_raster_matrix's). */
+int edxr_backward(edxr_ctx* ctx, const float* d_rgb, const edxr_grads* grads, uint32_t num_jobs);
 
 /* ---- introspection for parity tests (no counterpart in the reference) ---- */
 /* Setup records in submission order: per record 16 uint32 words
