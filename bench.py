@@ -56,6 +56,109 @@ def cpu_baseline(w, budget_s):
                       "CPU pipeline, %d threads" % (len(times), w.name, med * 1e3, cores)}
 
 
+def batched_leg(steps, warmup):
+    """256 independent jobs (19.2M triangles, 256^2-1024^2): one batched,
+    graph-captured launch sequence vs 256 plain per-job renders."""
+    import torch
+    from edxraster_amd import configs, Renderer
+    from edxraster_amd import _lib as L
+    jobs = configs.batch_jobs()
+    dm, keep = [], []
+    for w in jobs:
+        m, k = device_mesh(w.mesh)
+        dm.append(m)
+        keep.append(k)
+    arr = (L.Job * len(jobs))(*[w.job(mesh=m) for w, m in zip(jobs, dm)])
+    r = Renderer(64, 64)
+    tuning = r.autotune(arr)
+    r.set_graph(True)
+    for _ in range(warmup):
+        r.render_batch(arr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render_batch(arr)
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / steps
+    st = r.stats()
+    r.close()
+    p = Renderer(64, 64)
+
+    def plain():
+        for w, m in zip(jobs, dm):
+            p.resize(w.width, w.height)
+            w.apply(p)
+            p.render_mesh(m)
+    plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = max(1, steps // 2)
+    for _ in range(reps):
+        plain()
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / reps
+    p.close()
+    T = sum(w.mesh.num_triangles for w in jobs)
+    return {"metric": "batched frames/s", "jobs": len(jobs), "triangles": T,
+            "pixels": int(sum(w.width * w.height for w in jobs)),
+            "frames_per_s": len(jobs) / batch_s, "batch_ms": batch_s * 1e3,
+            "plain_frames_per_s": len(jobs) / plain_s, "plain_ms": plain_s * 1e3,
+            "speedup_vs_plain": plain_s / batch_s, "mtris_per_s": T / batch_s / 1e6,
+            "tuning": tuning, "refs": int(st["refs"]), "graph": True}
+
+
+def diff_leg(steps, warmup):
+    """64 views at 512x512 of the 20k-triangle sphere-on-heightfield scene:
+    forward (float RGB) + hand-written adjoint, L2 loss against targets with
+    perturbed colours, light and shininess."""
+    import torch
+    from edxraster_amd import configs, Renderer, diff
+    mesh, views = configs.diff_scene()
+    idx, planes = diff.mesh_tensors(mesh)
+    rng = np.random.default_rng(5)
+    light0 = np.array(views[0].light, np.float64)
+    axis = rng.normal(size=3)
+    r = Renderer(64, 64)
+    with torch.no_grad():
+        tgt_cols = [(planes[k] + torch.from_numpy(rng.normal(0, 0.1, mesh.num_vertices).astype(np.float32)).cuda())
+                    for k in (6, 7, 8)]
+        tl = torch.tensor(configs.rotate(light0, axis, 5.0), dtype=torch.float32)
+        tm = torch.tensor([0.2, 3.0, 3.0, 48.0])
+        target = diff.render_views(r, views, idx, *planes[:6], *tgt_cols, tl, tm).clone()
+    for t in planes:
+        t.requires_grad_(True)
+    light = torch.tensor(light0, dtype=torch.float32, requires_grad=True)
+    material = torch.tensor([0.2, 3.0, 3.0, 32.0], requires_grad=True)
+
+    def step():
+        img = diff.render_views(r, views, idx, *planes, light, material)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        return img
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    fwd = bwd = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        img = diff.render_views(r, views, idx, *planes, light, material)
+        loss = ((img - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd += t1 - t0
+        bwd += t2 - t1
+    r.close()
+    fwd, bwd = fwd / steps * 1e3, bwd / steps * 1e3
+    return {"metric": "fwd+bwd ms/view", "views": len(views), "res": views[0].width,
+            "triangles": mesh.num_triangles, "fwd_ms": fwd, "bwd_ms": bwd,
+            "ms_per_view": (fwd + bwd) / len(views), "bwd_over_fwd": bwd / fwd,
+            "note": "fwd includes the L2 loss; bwd the loss gradient + edxr_backward (adjoint kernels)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,9 +167,10 @@ def main():
     ap.add_argument("--workload", default="grid1m")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and differentiable legs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: the multi-GPU path is not built (see DESIGN.md)",
+        print("bench.py runs on exactly one GPU: the multi-GPU path is not built",
               file=sys.stderr)
         sys.exit(2)
 
@@ -134,11 +238,14 @@ def main():
         "stats": {k: int(v) for k, v in st.items()},
         "v_ref": v_ref,
     }
+    r.close()
+    if not args.no_extras:
+        out["batched"] = batched_leg(max(3, args.steps // 4), 2)
+        out["differentiable"] = diff_leg(max(3, args.steps // 4), 2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     else:
         out["cpu_baseline"] = None
-    r.close()
     print(json.dumps(out), flush=True)
 
 

@@ -177,6 +177,7 @@ struct edxr_ctx {
     DevBuf<uint16_t> slotRefs;
     DevBuf<uint32_t> groupSums;
     DevBuf<ShadeVert> verts;  // shading vertices + extraVertCap clip-created ones
+    DevBuf<ScreenVert> sverts;
     DevBuf<FrameParams> jobsDev;
     DevBuf<uint32_t> jobTables;
     HostBuf<FrameParams> jobsHost;
@@ -434,7 +435,8 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
             (e = c->refs.ensure(c->refCap)) != hipSuccess ||
             (e = c->work.ensure((size_t)Q.total_tiles + c->refCap / kSegRefs + 2)) != hipSuccess ||
             (e = c->partials.ensure(c->work.cap * kTileSize * kTileSize)) != hipSuccess ||
-            (e = c->verts.ensure((size_t)V + c->extraVertCap)) != hipSuccess)
+            (e = c->verts.ensure((size_t)V + c->extraVertCap)) != hipSuccess ||
+            (e = c->sverts.ensure(std::max<uint32_t>(V, 1))) != hipSuccess)
             return c->hip_fail(e, "hipMalloc(pipeline)");
 
         DeviceBuffers B;
@@ -447,6 +449,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.clip_queue_cap = c->clipQueueCap;
         B.extra_rec_cap = c->extraRecCap;
         B.verts = c->verts.p;
+        B.sverts = c->sverts.p;
         B.extra_vert_cap = c->extraVertCap;
         B.bin_counts = c->binCounts.p;
         B.tile_totals = c->tileTotals.p;
@@ -657,6 +660,7 @@ void edxr_destroy(edxr_ctx* c) {
     c->partials.release();
     c->slotRefs.release();
     c->verts.release();
+    c->sverts.release();
     c->jobsDev.release();
     c->jobsHost.release();
     c->tablesHost.release();

@@ -90,6 +90,16 @@ struct __attribute__((aligned(16))) ShadeVert {
 };
 static_assert(sizeof(ShadeVert) == 48, "ShadeVert layout");
 
+// Per input vertex, the values every unclipped triangle's setup uses: the 28.4
+// snapped raster position of the homogeneous-projected vertex, z/w as
+// Renderer::RenderMesh's z-divide computes it (Core/Renderer.cpp:139-147) and
+// the outcode (Core/Clipper.h:48-68).
+struct __attribute__((aligned(16))) ScreenVert {
+    int32_t x, y;
+    float z;
+    uint32_t code;
+};
+
 // Rasterisation part of a setup record (RasterTriangle, Core/RasterTriangle.h:13-23).
 struct __attribute__((aligned(16))) RasterRec {
     int32_t x0, y0, x1, y1;
@@ -154,6 +164,7 @@ struct DeviceBuffers {
     uint32_t clip_queue_cap;
     uint32_t extra_rec_cap;
     ShadeVert* verts;      // V + extra_vert_cap shading vertices
+    ScreenVert* sverts;    // V screen vertices (input vertices only)
     uint32_t extra_vert_cap;
     uint16_t* slot_refs;   // per input triangle: tile references (saturating)
     uint32_t* bin_counts;  // num_chunks * num_tiles, scanned in place

@@ -87,24 +87,18 @@ __device__ __forceinline__ uint32_t corner_pair(int B, int C) {
     return rej | (acc << 8);
 }
 
-// RasterTriangle::Setup on homogeneous-projected points (Core/RasterTriangle.h:27-153).
-__device__ __forceinline__ bool setup_record(const FrameParams& P, float ax, float ay, float az, float bx, float by,
-                                             float bz, float cx, float cy, float cz, RasterRec& r) {
+// The raster matrix applied to a homogeneous-projected point and the 28.4 snap
+// (Core/RasterTriangle.h:29-40).  An affine raster matrix gives w == 1, where
+// v / 1 == v exactly and the divide is skipped.
+__device__ __forceinline__ int2 snap_point(const FrameParams& P, float ax, float ay, float az) {
     const float4 A = xform4(P.raster, ax, ay, az, 1.0f);
-    const float4 Bv = xform4(P.raster, bx, by, bz, 1.0f);
-    const float4 Cv = xform4(P.raster, cx, cy, cz, 1.0f);
-    int x0, y0, x1, y1, x2, y2;
-    // An affine raster matrix gives w == 1 and v / 1 == v exactly; the wave
-    // skips the six divides only when every lane has w == 1.
-    if (__all(A.w == 1.0f && Bv.w == 1.0f && Cv.w == 1.0f)) {
-        x0 = snap(A.x); y0 = snap(A.y);
-        x1 = snap(Bv.x); y1 = snap(Bv.y);
-        x2 = snap(Cv.x); y2 = snap(Cv.y);
-    } else {
-        x0 = snap(A.x / A.w); y0 = snap(A.y / A.w);
-        x1 = snap(Bv.x / Bv.w); y1 = snap(Bv.y / Bv.w);
-        x2 = snap(Cv.x / Cv.w); y2 = snap(Cv.y / Cv.w);
-    }
+    if (A.w == 1.0f) return make_int2(snap(A.x), snap(A.y));
+    return make_int2(snap(A.x / A.w), snap(A.y / A.w));
+}
+
+// RasterTriangle::Setup from snapped vertices (Core/RasterTriangle.h:42-153);
+// false when culled (det <= 0).
+__device__ __forceinline__ bool setup_snapped(int x0, int y0, int x1, int y1, int x2, int y2, RasterRec& r) {
     const int B0 = y0 - y1, C0 = x1 - x0, B1 = y1 - y2, C1 = x2 - x1, B2 = y2 - y0, C2 = x0 - x2;
     const int det = C2 * B1 - C1 * B2;
     if (det <= 0) return false;
@@ -113,6 +107,13 @@ __device__ __forceinline__ bool setup_record(const FrameParams& P, float ax, flo
     const uint32_t c0 = corner_pair(B0, C0), c1 = corner_pair(B1, C1), c2 = corner_pair(B2, C2);
     r.info = (c0 & 3u) | ((c1 & 3u) << 2) | ((c2 & 3u) << 4) | ((c0 >> 8) << 6) | ((c1 >> 8) << 8) | ((c2 >> 8) << 10);
     return true;
+}
+
+// RasterTriangle::Setup on homogeneous-projected points (Core/RasterTriangle.h:27-153).
+__device__ __forceinline__ bool setup_record(const FrameParams& P, float ax, float ay, float az, float bx, float by,
+                                             float bz, float cx, float cy, float cz, RasterRec& r) {
+    const int2 a = snap_point(P, ax, ay, az), b = snap_point(P, bx, by, bz), c = snap_point(P, cx, cy, cz);
+    return setup_snapped(a.x, a.y, b.x, b.y, c.x, c.y, r);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
@@ -144,7 +145,7 @@ __device__ __forceinline__ uint32_t job_of(const BatchParams& Q, const uint32_t*
 // computes it (Core/Renderer.cpp:144).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_vertex(BatchParams Q, const FrameParams* __restrict__ jobs,
-                                                   ShadeVert* __restrict__ verts) {
+                                                   ShadeVert* __restrict__ verts, ScreenVert* __restrict__ sverts) {
     const FrameParams& P = jobs[job_of(Q, Q.vblock_job, blockIdx.x)];
     const MeshPtrs& M = P.mesh;
     const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
@@ -160,11 +161,21 @@ __global__ __launch_bounds__(kBlock) void k_vertex(BatchParams Q, const FramePar
     o.cr = M.cr ? M.cr[v] : 0.0f;
     o.cg = M.cg ? M.cg[v] : 0.0f;
     o.cb = M.cb ? M.cb[v] : 0.0f;
-    const float w = ((P.mvp[12] * o.px + P.mvp[13] * o.py) + P.mvp[14] * o.pz) + P.mvp[15] * 1.0f;
-    o.inv_w = 1.0f / w;
+    // DefaultVertexShader::Execute: projectedPos = MVP * (p, 1); the outcode,
+    // the homogeneous projection and the snapped raster position are per
+    // vertex, so every unclipped triangle's setup reuses them.
+    const float4 c = xform4(P.mvp, o.px, o.py, o.pz, 1.0f);
+    o.inv_w = 1.0f / c.w;
     o.pad0 = 0.0f;
     o.pad1 = 0.0f;
     verts[slot] = o;
+    ScreenVert sv;
+    const int2 xy = snap_point(P, c.x / c.w, c.y / c.w, c.z / c.w);
+    sv.x = xy.x;
+    sv.y = xy.y;
+    sv.z = c.z * (1.0f / c.w);
+    sv.code = clip_code(c);
+    sverts[slot] = sv;
 }
 
 namespace {
@@ -209,22 +220,16 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
         if (i0 >= P.num_vertices || i1 >= P.num_vertices || i2 >= P.num_vertices) {
             bad = true;
         } else {
-            // DefaultVertexShader::Execute: projectedPos = MVP * (p, 1).
-            const float4 p0 = vertex_pos(B.verts, P.vert_base + i0), p1 = vertex_pos(B.verts, P.vert_base + i1),
-                         p2 = vertex_pos(B.verts, P.vert_base + i2);
-            const float4 c0 = xform4(P.mvp, p0.x, p0.y, p0.z, 1.0f);
-            const float4 c1 = xform4(P.mvp, p1.x, p1.y, p1.z, 1.0f);
-            const float4 c2 = xform4(P.mvp, p2.x, p2.y, p2.z, 1.0f);
-            const uint32_t k0 = clip_code(c0), k1 = clip_code(c1), k2 = clip_code(c2);
-            if (k0 | k1 | k2) {
-                if (k0 & k1 & k2) rejected = true;
+            const ScreenVert s0 = B.sverts[P.vert_base + i0], s1 = B.sverts[P.vert_base + i1],
+                             s2 = B.sverts[P.vert_base + i2];
+            if (s0.code | s1.code | s2.code) {
+                if (s0.code & s1.code & s2.code) rejected = true;
                 else queued = true;
-            } else if (setup_record(P, c0.x / c0.w, c0.y / c0.w, c0.z / c0.w, c1.x / c1.w, c1.y / c1.w,
-                                    c1.z / c1.w, c2.x / c2.w, c2.y / c2.w, c2.z / c2.w, rec)) {
+            } else if (setup_snapped(s0.x, s0.y, s1.x, s1.y, s2.x, s2.y, rec)) {
                 // invW and the z divide (Core/Renderer.cpp:139-147).
-                rec.z0 = c0.z * (1.0f / c0.w);
-                rec.z1 = c1.z * (1.0f / c1.w);
-                rec.z2 = c2.z * (1.0f / c2.w);
+                rec.z0 = s0.z;
+                rec.z1 = s1.z;
+                rec.z2 = s2.z;
                 main = true;
             }
         }
@@ -1266,7 +1271,17 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
         v2i = P.vert_base + idx[2];
     }
     RTri t;
-    load_rtri(B.rrec[ri], t);
+    if (order & 1u) {
+        load_rtri(B.rrec[ri], t);
+    } else {
+        // An unclipped record is its vertices' screen values and 1/det.
+        const ScreenVert s0 = B.sverts[v0i], s1 = B.sverts[v1i], s2 = B.sverts[v2i];
+        t.x0 = s0.x; t.y0 = s0.y; t.x1 = s1.x; t.y1 = s1.y; t.x2 = s2.x; t.y2 = s2.y;
+        t.B1 = t.y1 - t.y2; t.C1 = t.x2 - t.x1;
+        t.B2 = t.y2 - t.y0; t.C2 = t.x0 - t.x2;
+        t.inv_det = 1.0f / (float)(t.C2 * t.B1 - t.C1 * t.B2);
+        t.z0 = s0.z; t.z1 = s1.z; t.z2 = s2.z;
+    }
     float l0, l1, d;
     bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
     const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
@@ -1302,7 +1317,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     const FrameParams none{};
     mark(0);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, B.counters, jobs, job0 ? *job0 : none, job0 ? 1 : 0);
-    if (Q.vert_blocks) hipLaunchKernelGGL(k_vertex, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts);
+    if (Q.vert_blocks)
+        hipLaunchKernelGGL(k_vertex, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts, B.sverts);
     if (Q.slot_blocks) {
         hipLaunchKernelGGL(k_setup, dim3(Q.slot_blocks), dim3(kBlock), 0, s, Q, jobs, B);
         const uint32_t clipBlocks = max(1u, min(1024u, (B.clip_queue_cap + 63) / 64));

@@ -131,7 +131,7 @@ def test_gpu_float_image_is_the_byte_image(built):
     from edxraster_amd import _lib as L
     mesh, views = configs.diff_scene(2, 128)
     r = Renderer(64, 64)
-    r._check_state = L.lib().edxr_set_state(r.ctx, L.STATE_FLOAT_COLOR, 1)
+    assert L.lib().edxr_set_state(r.ctx, L.STATE_FLOAT_COLOR, 1) == 0
     r.render_batch([v.job() for v in views])
     f = np.empty((2, 128, 128, 3), np.float32)
     assert L.lib().edxr_copy_float_color(r.ctx, f.ctypes.data, f.nbytes) == 0
