@@ -1007,10 +1007,11 @@ __device__ __forceinline__ float3 shade_pixel(const FrameParams& P, float b0, fl
 
 namespace {
 
-// Large references for one wave: the lanes are the 8x8 quads of a 16x16-pixel
-// quadrant and the wave walks the tile's four quadrants.  Covers
-// TrivialAcceptTriangle over the tile, CoarseRasterize's stepped quadrant
-// tests and FineRasterize with the bbox clamp (Core/Rasterizer.h:32-200,310-353).
+// Tile-level references for one wave: the lanes are the 8x8 quads of a
+// 16x16-pixel quadrant and the wave walks the tile's four quadrants.  Covers
+// TrivialAcceptTriangle over the tile and CoarseRasterize's stepped quadrant
+// tests with FineRasterize inside partly covered quadrants
+// (Core/Rasterizer.h:32-200,310-353).
 __device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RTri& t, uint32_t pay,
                                                   int tminX, int tminY, int tmaxX, int tmaxY, uint64_t* keys) {
     const uint32_t lane = __lane_id();
@@ -1023,7 +1024,7 @@ __device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RT
             const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
             if (qx <= tmaxX - 1 && qy <= tmaxY - 1) quad_depth(t, qx, qy, false, tminX, tminY, keys);
         }
-    } else if (big && P.hierarchical) {
+    } else {
         // CoarseRasterize (Core/Rasterizer.h:32-111): stepped corner tests per 16x16 quadrant.
         const int baseX = tminX << 4, baseY = tminY << 4;
         const int off = (kTileSize << 4) - 1, stepSize = kTileSize << 3;
@@ -1075,20 +1076,17 @@ __device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RT
                     quad_depth(t, qx, qy, true, tminX, tminY, keys);
             }
         }
-    } else {
-        // FineRasterize over the tile with the bbox clamp: the lanes cover the
-        // triangle's quad bbox, 64 quads per pass.
-        int minX = max(tminX, min(t.x0, min(t.x1, t.x2)) >> 4);
-        const int maxX = min(tmaxX - 1, max(t.x0, max(t.x1, t.x2)) >> 4);
-        int minY = max(tminY, min(t.y0, min(t.y1, t.y2)) >> 4);
-        const int maxY = min(tmaxY - 1, max(t.y0, max(t.y1, t.y2)) >> 4);
-        minX -= minX % 2;
-        minY -= minY % 2;
-        if (maxX < minX || maxY < minY) return;
-        const int nqx = (maxX - minX) / 2 + 1, nq = nqx * ((maxY - minY) / 2 + 1);
-        for (int k = (int)lane; k - (int)lane < nq; k += kWave)
-            if (k < nq) quad_depth(t, minX + 2 * (k % nqx), minY + 2 * (k / nqx), true, tminX, tminY, keys);
     }
+}
+
+// Inclusive prefix sum over the wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = __lane_id();
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(v, d);
+        if (lane >= (uint32_t)d) v += y;
+    }
+    return v;
 }
 
 // The record of lane l, made wave-uniform.
@@ -1118,9 +1116,11 @@ __device__ __forceinline__ RTri broadcast_rtri(const RTri& t, int l) {
 // One raster work item: references [seg * kSegRefs, (seg + 1) * kSegRefs) of a
 // tile.  Each wave streams its own 64-reference slices through the tile's LDS
 // key buffer (no workgroup barrier per slice): small triangles take one lane
-// each, large ones the whole wave.  The item's keys go to partials[workIndex].
+// each, medium ones one quad per lane, tile-level ones the whole wave.  The
+// item's keys go to partials[workIndex].
 __device__ __forceinline__ void raster_item(const BatchParams& Q, const FrameParams* __restrict__ jobs,
-                                            const DeviceBuffers& B, uint32_t workIndex, uint64_t* keys) {
+                                            const DeviceBuffers& B, uint32_t workIndex, uint64_t* keys,
+                                            uint32_t (*sIncl)[kWave]) {
     const uint32_t workItem = __builtin_amdgcn_readfirstlane(B.work[workIndex]);
     const uint32_t gtile = workItem & (kMaxBatchTiles - 1), seg = workItem >> kTileBits;
     const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
@@ -1141,10 +1141,14 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
         const uint32_t pay = valid ? B.refs[start + p] : 0u;
         const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
         const bool big = (pay >> 31) != 0;
-        bool large = valid && (trivial || (big && P.hierarchical));
+        // Tile-level paths (RasterizeTile, Core/Renderer.cpp:255-268):
+        // trivially accepted, or big with hierarchical rasterisation on.
+        const bool large = valid && (trivial || (big && P.hierarchical));
         // Every lane loads its own record; large ones are broadcast below.
         RTri t;
         if (valid) load_rtri(B.rrec[pay & kRecordMask], t);
+        uint32_t myQ = 0;  // quads of a medium FineRasterize reference
+        int fx = 0, fy = 0, fnx = 1;
         if (valid && !large) {
             const int bx0 = min(t.x0, min(t.x1, t.x2)) >> 4, bx1 = max(t.x0, max(t.x1, t.x2)) >> 4;
             const int by0 = min(t.y0, min(t.y1, t.y2)) >> 4, by1 = max(t.y0, max(t.y1, t.y2)) >> 4;
@@ -1170,8 +1174,50 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
                             for (int qx = 0; qx < nqx; qx++)
                                 quad_depth(t, minX + 2 * qx, minY + 2 * qy, true, tminX, tminY, keys);
                     } else {
-                        large = true;
+                        myQ = (uint32_t)(nqx * nqy);
+                        fx = minX;
+                        fy = minY;
+                        fnx = nqx;
                     }
+                }
+            }
+        }
+        // Medium references: the wave's FineRasterize quads (bbox clamped to
+        // the tile, Core/Rasterizer.h:126-200) are expanded into (reference,
+        // quad) pairs and tested 64 per pass, one quad per lane; the owner of
+        // pair k is the first lane whose inclusive quad count exceeds k.
+        const uint32_t incl = wave_incl_scan(myQ);
+        const uint32_t totalQ = __builtin_amdgcn_readlane(incl, kWave - 1);
+        if (totalQ) {
+            sIncl[wave][lane] = incl;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t base = 0; base < totalQ; base += kWave) {
+                const uint32_t k = base + lane;
+                int lo = 0, hi = kWave - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sIncl[wave][mid] > k) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int j = k < totalQ ? lo : 0;
+                RTri u;
+                u.x0 = __shfl(t.x0, j); u.y0 = __shfl(t.y0, j);
+                u.x1 = __shfl(t.x1, j); u.y1 = __shfl(t.y1, j);
+                u.x2 = __shfl(t.x2, j); u.y2 = __shfl(t.y2, j);
+                u.B0 = u.y0 - u.y1; u.C0 = u.x1 - u.x0;
+                u.B1 = u.y1 - u.y2; u.C1 = u.x2 - u.x1;
+                u.B2 = u.y2 - u.y0; u.C2 = u.x0 - u.x2;
+                u.pb0 = __shfl(t.pb0, j); u.pb1 = __shfl(t.pb1, j); u.pb2 = __shfl(t.pb2, j);
+                u.inv_det = __shfl(t.inv_det, j);
+                u.z0 = __shfl(t.z0, j); u.z1 = __shfl(t.z1, j); u.z2 = __shfl(t.z2, j);
+                u.order = __shfl(t.order, j);
+                const int ux = __shfl(fx, j), uy = __shfl(fy, j), unx = __shfl(fnx, j);
+                const uint32_t first = __shfl(incl - myQ, j);
+                if (k < totalQ) {
+                    const int kk = (int)(k - first);
+                    quad_depth(u, ux + 2 * (kk % unx), uy + 2 * (kk / unx), true, tminX, tminY, keys);
                 }
             }
         }
@@ -1197,6 +1243,7 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
 __global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                    DeviceBuffers B) {
     __shared__ uint64_t keys[kTileSize * kTileSize];
+    __shared__ uint32_t sIncl[kBlock / kWave][kWave];
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const uint32_t nItems = B.counters[kCntWorkItems];
@@ -1205,7 +1252,7 @@ __global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FramePar
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= nItems) break;
-        raster_item(Q, jobs, B, item, keys);
+        raster_item(Q, jobs, B, item, keys, sIncl);
         __syncthreads();
     }
 }
