@@ -56,6 +56,69 @@ def cpu_baseline(w, budget_s):
                       "CPU pipeline, %d threads" % (len(times), w.name, med * 1e3, cores)}
 
 
+# HBM bytes per launch of the raster/shade pass (k_raster + k_shade), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this bench, scaled by a
+# float4 copy kernel of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
+# profiles/pmc_raster_shade_*.md.  None where not measured.
+RASTER_SHADE_TRAFFIC = {}
+
+
+def frame_leg(w, steps, warmup, brief=False):
+    """Frames of one workload: the timed loop replays the captured frame
+    (hipGraph); a second loop of eager frames records HIP events around each
+    stage on the library's stream, which time the raster/shade pass for the
+    roofline (algorithmic bytes: 4 B bin entry + 40 B minimal setup record per
+    tile reference, 44 B per referenced vertex, 12 B per pixel written)."""
+    import torch
+    from edxraster_amd import Renderer
+    T, P = w.mesh.num_triangles, w.width * w.height
+    r = Renderer(w.width, w.height)
+    w.apply(r)
+    dm, keep = device_mesh(w.mesh)
+    for _ in range(warmup):
+        r.render_mesh(dm)
+    st = r.stats()
+    recs = r.debug_records()
+    v_ref = int(np.unique(recs[:, 13:16]).size) if recs.size else 0
+
+    r.set_graph(True)
+    for _ in range(max(1, warmup)):
+        r.render_mesh(dm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render_mesh(dm)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    r.set_graph(False)
+
+    r.set_profiling(True)
+    r.profile(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render_mesh(dm)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+    stage_ms, frames = r.profile()
+    r.set_profiling(False)
+    r.close()
+    stage_avg = {k: v / max(frames, 1) for k, v in stage_ms.items()}
+    n_ref = int(st["refs"])
+    b_rs = n_ref * 44 + 44 * v_ref + 12 * P
+    raster_ms = stage_avg["raster"]
+    achieved = b_rs / (raster_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": RASTER_SHADE_TRAFFIC.get(w.name),
+            "kernel": "k_raster+k_shade (raster/shade pass)", "algorithmic_bytes": b_rs, "kernel_ms": raster_ms}
+    out = {"workload": w.name, "ms": ms, "eager_ms": eager_ms, "stage_ms": stage_avg, "roofline": roof,
+           "stats": st, "v_ref": v_ref}
+    if brief:
+        out = {"workload": w.name, "triangles": T, "frame_ms": ms, "mtris_per_s": T / (ms * 1e-3) / 1e6,
+               "stage_ms": stage_avg, "roofline": roof}
+    return out
+
+
 def batched_leg(steps, warmup):
     """256 independent jobs (19.2M triangles, 256^2-1024^2): one batched,
     graph-captured launch sequence vs 256 plain per-job renders."""
@@ -179,37 +242,8 @@ def main():
 
     w = configs.WORKLOADS[args.workload]()
     T, V, P = w.mesh.num_triangles, w.mesh.num_vertices, w.width * w.height
-    r = Renderer(w.width, w.height)
-    w.apply(r)
-    dm, keep = device_mesh(w.mesh)
-
-    for _ in range(args.warmup):
-        r.render_mesh(dm)
-    st = r.stats()
-    recs = r.debug_records()
-    v_ref = int(np.unique(recs[:, 13:16]).size) if recs.size else 0
-
-    r.set_profiling(True)
-    r.profile(reset=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.render_mesh(dm)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stage_ms, frames = r.profile()
-    r.set_profiling(False)
-    ms = elapsed * 1e3 / args.steps
-    stage_avg = {k: v / max(frames, 1) for k, v in stage_ms.items()}
-
-    # Algorithmic bytes of the raster/shade kernel per launch: 4 B bin entry + 40 B
-    # minimal setup record per tile reference, 44 B per referenced post-clip vertex,
-    # 12 B per pixel (RGBA8 + fp32 depth + uint32 triangle ID written once).
-    n_ref = int(st["refs"])
-    b_rs = n_ref * 44 + 44 * v_ref + 12 * P
-    dom = max(stage_avg, key=stage_avg.get)
-    raster_ms = stage_avg["raster"]
-    achieved = b_rs / (raster_ms * 1e-3) / 1e9
+    fr = frame_leg(w, args.steps, args.warmup)
+    ms, stage_avg, st = fr["ms"], fr["stage_ms"], fr["stats"]
     b_frame = 36 * V + 12 * T + 12 * P
 
     out = {
@@ -226,20 +260,22 @@ def main():
         "dtype": "fp32+int32",
         "data": "synthetic (seeded SplitMix64 scene generated on the host, resident in HBM)",
         "config": {"workload": w.name, "triangles": T, "vertices": V, "width": w.width, "height": w.height,
-                   "shader": "vertex-colour Blinn-Phong s=32", "parallelism": "single GPU"},
+                   "shader": "vertex-colour Blinn-Phong s=32", "parallelism": "single GPU",
+                   "launch": "hipGraph replay per frame"},
         "frame_ms": ms,
         "gpix_s": P / (ms * 1e-3) / 1e9,
+        "eager_frame_ms": fr["eager_ms"],
         "stage_ms": stage_avg,
-        "dominant_stage": dom,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_raster",
-                     "algorithmic_bytes": b_rs, "kernel_ms": raster_ms},
+        "dominant_stage": max(stage_avg, key=stage_avg.get),
+        "roofline": fr["roofline"],
         "frame_roofline_frac": b_frame / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "stats": {k: int(v) for k, v in st.items()},
-        "v_ref": v_ref,
+        "v_ref": fr["v_ref"],
     }
-    r.close()
     if not args.no_extras:
+        if args.workload != "hf10m":
+            h = configs.WORKLOADS["hf10m"]()
+            out["frame_10m"] = frame_leg(h, max(5, args.steps // 2), 2, brief=True)
         out["batched"] = batched_leg(max(3, args.steps // 4), 2)
         out["differentiable"] = diff_leg(max(3, args.steps // 4), 2)
     if not args.no_cpu:

@@ -313,11 +313,24 @@ __device__ __forceinline__ void plane_snap(float4& v, int p) {
 // Plane order of ClipPolygon (Core/Clipper.h:238-278): L R B T F N.
 __device__ const uint32_t kPlaneBits[6] = {1u, 2u, 4u, 8u, 32u, 16u};
 
+// Setup of fan triangle (0, k-1, k) of a projected clipped polygon.
+__device__ __forceinline__ bool clip_fan(const FrameParams& P, const float4 (*h)[64], uint32_t lane, int k,
+                                         RasterRec& r) {
+    const float4 h0 = h[0][lane], h1 = h[k - 1][lane], h2 = h[k][lane];
+    return setup_record(P, h0.x, h0.y, h0.z, h1.x, h1.y, h1.z, h2.x, h2.y, h2.z, r);
+}
+
 }  // namespace
 
+// One lane per queued triangle.  The polygons live in LDS ([buffer][vertex][lane],
+// conflict-free): Sutherland-Hodgman output positions are data-dependent, and
+// private arrays indexed that way would go to scratch memory.
 __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* __restrict__ jobs, DeviceBuffers B) {
+    __shared__ float4 sPos[2][kMaxPoly][64];  // clip-space positions
+    __shared__ float4 sWgt[2][kMaxPoly][64];  // clip weights (Core/Clipper.h:111-119)
+    const uint32_t lane = threadIdx.x;
     const uint32_t n = min(B.counters[kCntClipQueue], B.clip_queue_cap);
-    for (uint32_t q = blockIdx.x * 64 + threadIdx.x; q < n; q += gridDim.x * 64) {
+    for (uint32_t q = blockIdx.x * 64 + lane; q < n; q += gridDim.x * 64) {
         const uint32_t t = B.clip_queue[q];  // triangle slot; its job varies per lane
         const FrameParams& P = jobs[Q.num_jobs == 1 ? 0u : Q.sblock_job[t / kBlock]];
         const uint32_t* idx = P.mesh.idx + 3 * (size_t)(t - P.slot_base);
@@ -332,13 +345,11 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
         atomicAdd(&B.counters[kCntClipped], 1u);
         if ((k0 | k1 | k2) & 16u) atomicAdd(&B.counters[kCntNearClipped], 1u);
 
-        float4 pp[2][kMaxPoly];
-        float3 ww[2][kMaxPoly];
         int cnt = 3, cur = 0;
         bool ok = true;
         for (int k = 0; k < 3; k++) {
-            pp[0][k] = c[k];
-            ww[0][k] = make_float3(k == 0 ? 1.0f : 0.0f, k == 1 ? 1.0f : 0.0f, k == 2 ? 1.0f : 0.0f);
+            sPos[0][k][lane] = c[k];
+            sWgt[0][k][lane] = make_float4(k == 0 ? 1.0f : 0.0f, k == 1 ? 1.0f : 0.0f, k == 2 ? 1.0f : 0.0f, 0.0f);
         }
         // Clipper::ClipByPlane (Core/Clipper.h:192-232).
         for (int p = 0; p < 6 && ok; p++) {
@@ -348,24 +359,25 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
             for (int i = 0; i < cnt; i++) {
                 int i1 = i + 1;
                 if (i1 == cnt) i1 = 0;
-                const float4 v0 = pp[cur][i], v1 = pp[cur][i1];
-                const float3 w0 = ww[cur][i], w1 = ww[cur][i1];
+                const float4 v0 = sPos[cur][i][lane], v1 = sPos[cur][i1][lane];
+                const float4 w0 = sWgt[cur][i][lane], w1 = sWgt[cur][i1][lane];
                 const bool in0 = plane_inside(v0, p), in1 = plane_inside(v1, p);
                 if (in0 && in1) {
                     if (m >= kMaxPoly) { ok = false; break; }
-                    pp[nx][m] = v1; ww[nx][m] = w1; m++;
+                    sPos[nx][m][lane] = v1; sWgt[nx][m][lane] = w1; m++;
                 } else if (in0 != in1) {
                     const float tt = plane_t(v0, v1, p);
                     const float s = 1 - tt;
                     float4 pos = make_float4(v0.x * s + v1.x * tt, v0.y * s + v1.y * tt, v0.z * s + v1.z * tt,
                                              v0.w * s + v1.w * tt);
                     plane_snap(pos, p);
-                    const float3 w = make_float3(w0.x * s + w1.x * tt, w0.y * s + w1.y * tt, w0.z * s + w1.z * tt);
+                    const float4 w = make_float4(w0.x * s + w1.x * tt, w0.y * s + w1.y * tt, w0.z * s + w1.z * tt,
+                                                 0.0f);
                     if (m >= kMaxPoly) { ok = false; break; }
-                    pp[nx][m] = pos; ww[nx][m] = w; m++;
+                    sPos[nx][m][lane] = pos; sWgt[nx][m][lane] = w; m++;
                     if (!in0) {
                         if (m >= kMaxPoly) { ok = false; break; }
-                        pp[nx][m] = v1; ww[nx][m] = w1; m++;
+                        sPos[nx][m][lane] = v1; sWgt[nx][m][lane] = w1; m++;
                     }
                 }
             }
@@ -378,19 +390,16 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
             continue;
         }
         for (int i = 0; i < cnt; i++)
-            if (pp[cur][i].w <= 0.0f) { cnt = 0; break; }
+            if (sPos[cur][i][lane].w <= 0.0f) { cnt = 0; break; }
         if (cnt < 3) {
             B.desc[t] = 0;
             continue;
         }
         // Vertex ids: weight == 1 reuses a parent vertex (Core/Clipper.h:121-153).
-        int which[kMaxPoly];
+        auto parent = [](const float4& w) { return (w.x == 1.0f) ? 0 : (w.y == 1.0f) ? 1 : (w.z == 1.0f) ? 2 : -1; };
         int nNew = 0;
-        for (int j = 0; j < cnt; j++) {
-            const float3 w = ww[cur][j];
-            which[j] = (w.x == 1.0f) ? 0 : (w.y == 1.0f) ? 1 : (w.z == 1.0f) ? 2 : -1;
-            if (which[j] < 0) nNew++;
-        }
+        for (int j = 0; j < cnt; j++)
+            if (parent(sWgt[cur][j][lane]) < 0) nNew++;
         uint32_t vbase = 0;
         if (nNew) {
             vbase = atomicAdd(&B.counters[kCntExtraVertices], (uint32_t)nNew);
@@ -400,19 +409,22 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
                 continue;
             }
         }
-        uint32_t vid[kMaxPoly];
-        float hx[kMaxPoly], hy[kMaxPoly], hz[kMaxPoly], zz[kMaxPoly];
+        // Projected vertices (x/w, y/w, z/w, z * (1/w)) and vertex ids go to the
+        // other buffer.
+        const int out = cur ^ 1;
         int e = 0;
         for (int j = 0; j < cnt; j++) {
+            const float4 w = sWgt[cur][j][lane];
+            const int which = parent(w);
             float4 pos;
-            if (which[j] >= 0) {
-                vid[j] = vi[which[j]];
-                pos = c[which[j]];
+            uint32_t vid;
+            if (which >= 0) {
+                vid = which == 0 ? vi[0] : which == 1 ? vi[1] : vi[2];
+                pos = which == 0 ? c[0] : which == 1 ? c[1] : c[2];
             } else {
                 const uint32_t xi = vbase + (uint32_t)e++;
-                vid[j] = Q.total_verts + xi;
-                pos = pp[cur][j];
-                const float3 w = ww[cur][j];
+                vid = Q.total_verts + xi;
+                pos = sPos[cur][j][lane];
                 const ShadeVert& a = B.verts[vi[0]];
                 const ShadeVert& b = B.verts[vi[1]];
                 const ShadeVert& cc = B.verts[vi[2]];
@@ -430,18 +442,16 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
                 o.inv_w = 1.0f / pos.w;
                 o.pad0 = 0.0f;
                 o.pad1 = 0.0f;
-                B.verts[vid[j]] = o;
+                B.verts[vid] = o;
             }
-            hx[j] = pos.x / pos.w;
-            hy[j] = pos.y / pos.w;
-            hz[j] = pos.z / pos.w;
-            zz[j] = pos.z * (1.0f / pos.w);
+            sPos[out][j][lane] = make_float4(pos.x / pos.w, pos.y / pos.w, pos.z / pos.w, pos.z * (1.0f / pos.w));
+            sWgt[out][j][lane] = make_float4(__uint_as_float(vid), 0.0f, 0.0f, 0.0f);
         }
         // Fan triangulation (Core/Clipper.h:155-170): count survivors, reserve, write.
         uint32_t nrec = 0;
         RasterRec r;
         for (int k = 2; k < cnt; k++)
-            if (setup_record(P, hx[0], hy[0], hz[0], hx[k - 1], hy[k - 1], hz[k - 1], hx[k], hy[k], hz[k], r)) nrec++;
+            if (clip_fan(P, sPos[out], lane, k, r)) nrec++;
         if (nrec == 0) {
             B.desc[t] = 0;
             continue;
@@ -455,14 +465,15 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
         const uint32_t first = Q.total_slots + rbase;
         uint32_t s = 0;
         for (int k = 2; k < cnt; k++) {
-            if (!setup_record(P, hx[0], hy[0], hz[0], hx[k - 1], hy[k - 1], hz[k - 1], hx[k], hy[k], hz[k], r)) continue;
-            r.z0 = zz[0];
-            r.z1 = zz[k - 1];
-            r.z2 = zz[k];
+            if (!clip_fan(P, sPos[out], lane, k, r)) continue;
+            r.z0 = sPos[out][0][lane].w;
+            r.z1 = sPos[out][k - 1][lane].w;
+            r.z2 = sPos[out][k][lane].w;
             r.info |= (uint32_t)(k - 2) << 12;
             r.order = make_order(t, s, true);
             B.rrec[first + s] = r;
-            B.srec[first + s] = ShadeRec{vid[0], vid[k - 1], vid[k], t};
+            B.srec[first + s] = ShadeRec{__float_as_uint(sWgt[out][0][lane].x), __float_as_uint(sWgt[out][k - 1][lane].x),
+                                         __float_as_uint(sWgt[out][k][lane].x), t};
             s++;
         }
         B.desc[t] = (first << 3) | nrec;
@@ -586,6 +597,15 @@ struct RefEnum {
     }
 };
 
+// Reference k (row-major over the bbox) of a compact descriptor: its tile.
+__device__ __forceinline__ uint32_t compact_tile(uint32_t desc, uint32_t k, uint32_t tilesX) {
+    const uint32_t w = 1u + ((desc >> 28) & 1u);
+    return ((desc >> 14) & 0x3FFFu) * tilesX + (desc & 0x3FFFu) + (k % w) + (k / w) * tilesX;
+}
+__device__ __forceinline__ uint32_t compact_refs(uint32_t desc) {
+    return (1u + ((desc >> 28) & 1u)) * (1u + ((desc >> 29) & 1u));
+}
+
 // Block-wide exclusive scan of one value per thread (256 threads).
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
@@ -618,16 +638,35 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
     __syncthreads();
     const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
     const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
-    for (uint32_t s = s0 + threadIdx.x; s < s1; s += kBlock) {
-        RefEnum e;
-        e.init(B.desc[s], s);
-        uint32_t tile, payload, R = 0;
-        while (e.next(P, B.rrec, tile, payload)) {
-            atomicAdd(&hist[tile], 1u);
-            R++;
+    // Four slots per thread and step, their descriptors loaded together so
+    // the loads' latencies overlap.
+    constexpr uint32_t kUnroll = 4;
+    const uint32_t tilesX = P.tiles_x;
+    for (uint32_t sb = s0 + threadIdx.x; sb < s1; sb += kUnroll * kBlock) {
+        uint32_t d[kUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; u++) d[u] = sb + u * kBlock < s1 ? B.desc[sb + u * kBlock] : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; u++) {
+            const uint32_t s = sb + u * kBlock;
+            if (s >= s1) break;
+            uint32_t R = 0;
+            if (d[u] & kDescCompact) {
+                // at most 2x2 tiles, no corner tests
+                R = compact_refs(d[u]);
+                for (uint32_t k = 0; k < R; k++) atomicAdd(&hist[compact_tile(d[u], k, tilesX)], 1u);
+            } else {
+                RefEnum e;
+                e.init(d[u], s);
+                uint32_t tile, payload;
+                while (e.next(P, B.rrec, tile, payload)) {
+                    atomicAdd(&hist[tile], 1u);
+                    R++;
+                }
+            }
+            // Saturated counts are recomputed by the scatter.
+            B.slot_refs[s] = (uint16_t)min(R, (uint32_t)kSlotRefsSaturated);
         }
-        // Saturated counts are recomputed by the scatter.
-        B.slot_refs[s] = (uint16_t)min(R, (uint32_t)kSlotRefsSaturated);
     }
     __syncthreads();
     uint32_t* row = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
@@ -801,12 +840,20 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
     const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
+    const uint32_t tilesX = P.tiles_x;
+    // The next round's descriptor and count are loaded before this round's
+    // work, so their latency overlaps it.
+    uint32_t descNext = s0 + threadIdx.x < s1 ? B.desc[s0 + threadIdx.x] : 0u;
+    uint32_t refsNext = s0 + threadIdx.x < s1 ? B.slot_refs[s0 + threadIdx.x] : 0u;
     for (uint32_t rb = s0; rb < s1; rb += kBlock) {
         const uint32_t s = rb + threadIdx.x;
-        const uint32_t desc = s < s1 ? B.desc[s] : 0u;
+        const uint32_t desc = descNext;
+        uint32_t R = refsNext, tile, payload;
+        const uint32_t sn = s + kBlock;
+        descNext = sn < s1 ? B.desc[sn] : 0u;
+        refsNext = sn < s1 ? B.slot_refs[sn] : 0u;
         RefEnum e;
         e.init(desc, s);
-        uint32_t R = s < s1 ? B.slot_refs[s] : 0u, tile, payload;
         if (R == kSlotRefsSaturated) {
             R = 0;
             while (e.next(P, B.rrec, tile, payload)) R++;
@@ -817,10 +864,17 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
         for (uint32_t win = 0; win < total; win += kBlock) {
             // Emit this thread's refs that fall in [win, win + 256).
             const uint32_t lo = max(base, win), hi = min(base + R, win + kBlock);
-            for (uint32_t pos = lo; pos < hi; pos++) {
-                e.next(P, B.rrec, tile, payload);
-                stageTile[pos - win] = tile;
-                stagePay[pos - win] = payload;
+            if (desc & kDescCompact) {
+                for (uint32_t pos = lo; pos < hi; pos++) {
+                    stageTile[pos - win] = compact_tile(desc, pos - base, tilesX);
+                    stagePay[pos - win] = s;
+                }
+            } else {
+                for (uint32_t pos = lo; pos < hi; pos++) {
+                    e.next(P, B.rrec, tile, payload);
+                    stageTile[pos - win] = tile;
+                    stagePay[pos - win] = payload;
+                }
             }
             __syncthreads();
             const bool valid = win + threadIdx.x < total;
@@ -1368,7 +1422,7 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
         hipLaunchKernelGGL(k_vertex, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts, B.sverts);
     if (Q.slot_blocks) {
         hipLaunchKernelGGL(k_setup, dim3(Q.slot_blocks), dim3(kBlock), 0, s, Q, jobs, B);
-        const uint32_t clipBlocks = max(1u, min(1024u, (B.clip_queue_cap + 63) / 64));
+        const uint32_t clipBlocks = max(1u, min(512u, (B.clip_queue_cap + 63) / 64));
         hipLaunchKernelGGL(k_clip, dim3(clipBlocks), dim3(64), 0, s, Q, jobs, B);
     }
     mark(1);
