@@ -304,7 +304,8 @@ FrameParams job_params(const JobInput& in, uint32_t tilesX, uint32_t tilesY) {
 }
 
 // Places the jobs in the batch arrays.  Vertex and triangle slots start on
-// kBlock boundaries so each block of k_vertex / k_setup belongs to one job.
+// kJobAlign boundaries so each block of k_vertex / k_setup belongs to one job;
+// the block -> job tables have one entry per kJobTableBlock slots.
 int build_plan(edxr_ctx* c, const std::vector<JobInput>& in, Plan& pl) {
     const bool single = in.size() == 1;
     pl.jobs.clear();
@@ -346,19 +347,22 @@ int build_plan(edxr_ctx* c, const std::vector<JobInput>& in, Plan& pl) {
     Q.total_slots = (uint32_t)slot;
     Q.total_tiles = (uint32_t)tile;
     Q.total_chunks = (uint32_t)chunk;
-    Q.vert_blocks = (uint32_t)((vert + kJobAlign - 1) / kJobAlign);
-    Q.slot_blocks = (uint32_t)((slot + kJobAlign - 1) / kJobAlign);
+    Q.vert_blocks = (uint32_t)((vert + kJobTableBlock - 1) / kJobTableBlock);
+    Q.slot_blocks = (uint32_t)((slot + kJobTableBlock - 1) / kJobTableBlock);
     if (!single) {
         pl.tables.resize((size_t)Q.vert_blocks + Q.slot_blocks + Q.total_chunks + Q.total_tiles);
         uint32_t* vb = pl.tables.data();
         uint32_t* sb = vb + Q.vert_blocks;
         uint32_t* cj = sb + Q.slot_blocks;
         uint32_t* tj = cj + Q.total_chunks;
+        // Padding blocks up to the next job's base belong to the job before it.
         for (uint32_t k = 0; k < Q.num_jobs; k++) {
             const FrameParams& P = pl.jobs[k];
-            for (uint32_t b = P.vert_base / kJobAlign; b < (P.vert_base + P.num_vertices + kJobAlign - 1) / kJobAlign; b++)
+            const uint32_t vEnd = k + 1 < Q.num_jobs ? pl.jobs[k + 1].vert_base : Q.total_verts;
+            const uint32_t sEnd = k + 1 < Q.num_jobs ? pl.jobs[k + 1].slot_base : Q.total_slots;
+            for (uint32_t b = P.vert_base / kJobTableBlock; b < (vEnd + kJobTableBlock - 1) / kJobTableBlock; b++)
                 vb[b] = k;
-            for (uint32_t b = P.slot_base / kJobAlign; b < (P.slot_base + P.num_triangles + kJobAlign - 1) / kJobAlign; b++)
+            for (uint32_t b = P.slot_base / kJobTableBlock; b < (sEnd + kJobTableBlock - 1) / kJobTableBlock; b++)
                 sb[b] = k;
             for (uint32_t i = 0; i < P.num_chunks; i++) cj[P.chunk_base + i] = k;
             for (uint32_t i = 0; i < P.num_tiles; i++) tj[P.tile_base + i] = k;
@@ -1051,9 +1055,12 @@ int64_t edxr_debug_read_records(edxr_ctx* c, uint32_t* out, uint64_t cap) {
     std::vector<RasterRec> rr(nrec);
     std::vector<ShadeRec> sr(nrec);
     std::vector<uint32_t> idx(3ull * T);
+    std::vector<ScreenVert> sv(c->lastV);
     hipError_t e = hipSuccess;
     if (T) e = hipMemcpy(desc.data(), c->desc.p, 4ull * T, hipMemcpyDeviceToHost);
     if (T && e == hipSuccess) e = hipMemcpy(idx.data(), c->lastMesh.idx, 12ull * T, hipMemcpyDeviceToHost);
+    if (c->lastV && e == hipSuccess)
+        e = hipMemcpy(sv.data(), c->sverts.p, sizeof(ScreenVert) * c->lastV, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(rr.data(), c->rrec.p, sizeof(RasterRec) * nrec, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(sr.data(), c->srec.p, sizeof(ShadeRec) * nrec, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return c->hip_fail(e, "debug_read_records");
@@ -1063,7 +1070,21 @@ int64_t edxr_debug_read_records(edxr_ctx* c, uint32_t* out, uint64_t cap) {
         const uint32_t first = compact ? t : desc[t] >> 3, cnt = compact ? 1u : desc[t] & 7u;
         for (uint32_t k = 0; k < cnt; k++, n++) {
             if (n >= cap || !out) continue;
-            const RasterRec& r = rr[first + k];
+            RasterRec r = rr[first + k];
+            if (compact) {
+                // Not stored on the device: rebuilt from the vertices' screen
+                // values as the raster kernel rebuilds it.
+                const ScreenVert& a = sv[idx[3 * t]];
+                const ScreenVert& b = sv[idx[3 * t + 1]];
+                const ScreenVert& d = sv[idx[3 * t + 2]];
+                r.x0 = a.x; r.y0 = a.y; r.x1 = b.x; r.y1 = b.y; r.x2 = d.x; r.y2 = d.y;
+                const int B0 = r.y0 - r.y1, C0 = r.x1 - r.x0, B1 = r.y1 - r.y2, C1 = r.x2 - r.x1;
+                const int B2 = r.y2 - r.y0, C2 = r.x0 - r.x2;
+                r.inv_det = 1.0f / (float)(C2 * B1 - C1 * B2);
+                r.z0 = a.z; r.z1 = b.z; r.z2 = d.z;
+                r.info = corner_info(B0, C0, B1, C1, B2, C2);
+                r.order = make_order(t, 0, false);
+            }
             // Unclipped triangles keep their input vertex ids (no ShadeRec).
             const ShadeRec s = (r.order & 1u) ? sr[first + k] : ShadeRec{idx[3 * t], idx[3 * t + 1], idx[3 * t + 2], t};
             uint32_t* o = out + 16 * n;
@@ -1104,8 +1125,9 @@ int64_t edxr_debug_read_bins(edxr_ctx* c, uint64_t* offsets, uint32_t* refs3, ui
             const uint32_t p = refs[i], ri = p & kRecordMask;
             const uint32_t acc = (p >> kRecordBits) & 7u;
             const uint32_t big = p >> 31;
-            refs3[3 * i] = rr[ri].order >> 4;
-            refs3[3 * i + 1] = (rr[ri].info >> 12) & 7u;
+            const bool compact = !big && ri < c->lastT;  // record slot = triangle, not stored
+            refs3[3 * i] = compact ? ri : rr[ri].order >> 4;
+            refs3[3 * i + 1] = compact ? 0u : (rr[ri].info >> 12) & 7u;
             refs3[3 * i + 2] = acc | ((acc == 7u) ? 8u : 0u) | (big ? 16u : 0u);
         }
     return (int64_t)total;

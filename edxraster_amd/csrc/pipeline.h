@@ -19,7 +19,8 @@ constexpr int kMaxPoly = 16;                 // clip polygon capacity (convex in
 constexpr uint32_t kSegRefs = 2048;          // tile references per raster work item
 constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
-constexpr uint32_t kJobAlign = 256;          // job slot alignment = vertex/setup block size
+constexpr uint32_t kJobAlign = 1024;         // job slot alignment: vertex/setup blocks never span jobs
+constexpr uint32_t kJobTableBlock = 256;     // slots (vertices) per entry of the block -> job tables
 constexpr uint32_t kMaxChunks = 4096;
 // Per-triangle descriptor: bit 31 set = compact small triangle (record in slot
 // t; tile bbox minTX | minTY << 14 | dx << 28 | dy << 29, dx, dy in {0, 1});
@@ -111,10 +112,33 @@ struct __attribute__((aligned(16))) RasterRec {
 };
 static_assert(sizeof(RasterRec) == 48, "RasterRec layout");
 
+// Trivial reject/accept corner choice (Core/RasterTriangle.h:66-150).  The
+// reference branches on slope = float(B) / float(C) >= 0; int->float keeps the
+// sign and the quotient of nonzero ints cannot underflow, so the test is a sign
+// test: C == 0 gives +-inf (or NaN for B == 0), B == 0 with C != 0 gives +-0.
+__host__ __device__ inline uint32_t corner_pair(int B, int C) {
+    const bool slopeNonNeg = (C == 0) ? (B > 0) : (B == 0 || ((B > 0) == (C > 0)));
+    uint32_t rej, acc;
+    if (slopeNonNeg) {
+        if (C >= 0) { rej = 3; acc = 0; } else { rej = 0; acc = 3; }
+    } else {
+        if (C >= 0) { rej = 2; acc = 1; } else { rej = 1; acc = 2; }
+    }
+    return rej | (acc << 8);
+}
+
+// RasterRec::info corner codes: reject corners in bits 0-5, accept corners in
+// bits 6-11, two bits per edge.
+__host__ __device__ inline uint32_t corner_info(int B0, int C0, int B1, int C1, int B2, int C2) {
+    const uint32_t c0 = corner_pair(B0, C0), c1 = corner_pair(B1, C1), c2 = corner_pair(B2, C2);
+    return (c0 & 3u) | ((c1 & 3u) << 2) | ((c2 & 3u) << 4) | ((c0 >> 8) << 6) | ((c1 >> 8) << 8) | ((c2 >> 8) << 10);
+}
+
 // Submission order of a record: triangle slot (the job's slot_base + input
 // index), then the index among the triangle's surviving fan pieces
-// (Core/Clipper.h:155-170).  Unclipped triangles live in record slot `tri`;
-// clipped ones at desc[tri] >> 3 + survivor.
+// (Core/Clipper.h:155-170).  Unclipped triangles live in record slot `tri`
+// (stored only when their descriptor is not compact); clipped ones at
+// desc[tri] >> 3 + survivor.
 __host__ __device__ inline uint32_t make_order(uint32_t tri, uint32_t survivor, bool clipped) {
     return (tri << 4) | (survivor << 1) | (clipped ? 1u : 0u);
 }

@@ -40,7 +40,9 @@ namespace {
 
 constexpr uint32_t kWave = 64;
 constexpr uint32_t kBlock = 256;
-static_assert(kBlock == kJobAlign, "a vertex/setup block must not span two jobs");
+constexpr uint32_t kSetupSlots = 4;  // triangle slots per k_setup thread
+static_assert(kJobAlign % (kBlock * kSetupSlots) == 0, "a vertex/setup block must not span two jobs");
+static_assert(kBlock == kJobTableBlock, "block -> job tables are indexed by 256-slot block");
 
 __device__ __forceinline__ float4 xform4(const float* m, float x, float y, float z, float w) {
     float4 r;
@@ -72,21 +74,6 @@ __device__ __forceinline__ int snap(float a) {
     return (int)s;
 }
 
-// Trivial reject/accept corner choice (Core/RasterTriangle.h:66-150).  The
-// reference branches on slope = float(B) / float(C) >= 0; int->float keeps the
-// sign and the quotient of nonzero ints cannot underflow, so the test is a sign
-// test: C == 0 gives +-inf (or NaN for B == 0), B == 0 with C != 0 gives +-0.
-__device__ __forceinline__ uint32_t corner_pair(int B, int C) {
-    const bool slopeNonNeg = (C == 0) ? (B > 0) : (B == 0 || ((B > 0) == (C > 0)));
-    uint32_t rej, acc;
-    if (slopeNonNeg) {
-        if (C >= 0) { rej = 3; acc = 0; } else { rej = 0; acc = 3; }
-    } else {
-        if (C >= 0) { rej = 2; acc = 1; } else { rej = 1; acc = 2; }
-    }
-    return rej | (acc << 8);
-}
-
 // The raster matrix applied to a homogeneous-projected point and the 28.4 snap
 // (Core/RasterTriangle.h:29-40).  An affine raster matrix gives w == 1, where
 // v / 1 == v exactly and the divide is skipped.
@@ -104,8 +91,7 @@ __device__ __forceinline__ bool setup_snapped(int x0, int y0, int x1, int y1, in
     if (det <= 0) return false;
     r.x0 = x0; r.y0 = y0; r.x1 = x1; r.y1 = y1; r.x2 = x2; r.y2 = y2;
     r.inv_det = 1.0f / (float)det;
-    const uint32_t c0 = corner_pair(B0, C0), c1 = corner_pair(B1, C1), c2 = corner_pair(B2, C2);
-    r.info = (c0 & 3u) | ((c1 & 3u) << 2) | ((c2 & 3u) << 4) | ((c0 >> 8) << 6) | ((c1 >> 8) << 8) | ((c2 >> 8) << 10);
+    r.info = corner_info(B0, C0, B1, C1, B2, C2);
     return true;
 }
 
@@ -203,25 +189,45 @@ __device__ __forceinline__ uint32_t compact_desc(const FrameParams& P, const Ras
 // ---------------------------------------------------------------------------
 // Setup: one lane per input triangle.
 // ---------------------------------------------------------------------------
+// Each thread takes kSetupSlots slots kBlock apart; all their index and
+// screen-vertex loads are issued before any is used, so one wave keeps
+// several dependent-load chains in flight.
 __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                   DeviceBuffers B) {
-    const FrameParams& P = jobs[job_of(Q, Q.sblock_job, blockIdx.x)];
+    const FrameParams& P = jobs[job_of(Q, Q.sblock_job, blockIdx.x * kSetupSlots)];
     const MeshPtrs& M = P.mesh;
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;  // triangle slot
-    const uint32_t t = s - P.slot_base;                     // input triangle of the job
-    const bool valid = t < P.num_triangles;
-    bool rejected = false, queued = false, bad = false, main = false;
-    uint32_t i0 = 0, i1 = 0, i2 = 0;
-    RasterRec rec;
-    if (valid) {
-        i0 = M.idx[3 * (size_t)t];
-        i1 = M.idx[3 * (size_t)t + 1];
-        i2 = M.idx[3 * (size_t)t + 2];
-        if (i0 >= P.num_vertices || i1 >= P.num_vertices || i2 >= P.num_vertices) {
-            bad = true;
-        } else {
-            const ScreenVert s0 = B.sverts[P.vert_base + i0], s1 = B.sverts[P.vert_base + i1],
-                             s2 = B.sverts[P.vert_base + i2];
+    uint32_t s[kSetupSlots], i0[kSetupSlots], i1[kSetupSlots], i2[kSetupSlots];
+    bool valid[kSetupSlots];
+#pragma unroll
+    for (uint32_t u = 0; u < kSetupSlots; u++) {
+        s[u] = (blockIdx.x * kSetupSlots + u) * kBlock + threadIdx.x;  // triangle slot
+        const uint32_t t = s[u] - P.slot_base;                        // input triangle of the job
+        valid[u] = t < P.num_triangles;
+        i0[u] = i1[u] = i2[u] = 0;
+        if (valid[u]) {
+            i0[u] = M.idx[3 * (size_t)t];
+            i1[u] = M.idx[3 * (size_t)t + 1];
+            i2[u] = M.idx[3 * (size_t)t + 2];
+        }
+    }
+    ScreenVert sv[kSetupSlots][3];
+    bool bad[kSetupSlots];
+#pragma unroll
+    for (uint32_t u = 0; u < kSetupSlots; u++) {
+        bad[u] = valid[u] && (i0[u] >= P.num_vertices || i1[u] >= P.num_vertices || i2[u] >= P.num_vertices);
+        if (valid[u] && !bad[u]) {
+            sv[u][0] = B.sverts[P.vert_base + i0[u]];
+            sv[u][1] = B.sverts[P.vert_base + i1[u]];
+            sv[u][2] = B.sverts[P.vert_base + i2[u]];
+        }
+    }
+    uint32_t nrej = 0, nbad = 0, nmain = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kSetupSlots; u++) {
+        bool rejected = false, queued = false, main = false;
+        RasterRec rec;
+        if (valid[u] && !bad[u]) {
+            const ScreenVert &s0 = sv[u][0], &s1 = sv[u][1], &s2 = sv[u][2];
             if (s0.code | s1.code | s2.code) {
                 if (s0.code & s1.code & s2.code) rejected = true;
                 else queued = true;
@@ -233,33 +239,39 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
                 main = true;
             }
         }
-    }
-    // Wave-aggregated clip-queue append.
-    const uint64_t qmask = __ballot(queued);
-    uint32_t qbase = 0;
-    if (qmask) {
-        const int leader = __ffsll((unsigned long long)qmask) - 1;
-        if ((int)__lane_id() == leader) qbase = atomicAdd(&B.counters[kCntClipQueue], (uint32_t)__popcll(qmask));
-        qbase = __shfl(qbase, leader);
-    }
-    if (valid) {
-        uint32_t d = 0;
-        if (main) {
-            rec.order = make_order(s, 0, false);
-            B.rrec[s] = rec;
-            d = compact_desc(P, rec, s);
-        } else if (queued) {
-            const uint32_t slot = qbase + (uint32_t)__popcll(qmask & lanemask_lt());
-            if (slot < B.clip_queue_cap) B.clip_queue[slot] = s;
-            else atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvClipQueue);
+        // Wave-aggregated clip-queue append.
+        const uint64_t qmask = __ballot(queued);
+        uint32_t qbase = 0;
+        if (qmask) {
+            const int leader = __ffsll((unsigned long long)qmask) - 1;
+            if ((int)__lane_id() == leader) qbase = atomicAdd(&B.counters[kCntClipQueue], (uint32_t)__popcll(qmask));
+            qbase = __shfl(qbase, leader);
         }
-        B.desc[s] = d;
+        if (valid[u]) {
+            uint32_t d = 0;
+            if (main) {
+                // A compact triangle's record is not stored: it is the three
+                // vertices' screen values, which the raster re-reads.
+                d = compact_desc(P, rec, s[u]);
+                if (!(d & kDescCompact)) {
+                    rec.order = make_order(s[u], 0, false);
+                    B.rrec[s[u]] = rec;
+                }
+            } else if (queued) {
+                const uint32_t slot = qbase + (uint32_t)__popcll(qmask & lanemask_lt());
+                if (slot < B.clip_queue_cap) B.clip_queue[slot] = s[u];
+                else atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvClipQueue);
+            }
+            B.desc[s[u]] = d;
+        }
+        nrej += wave_count(rejected);
+        nbad += wave_count(bad[u]);
+        nmain += wave_count(main);
     }
     // Per-workgroup totals, one striped atomic per counter per workgroup.
     __shared__ uint32_t wgCount[3];
     if (threadIdx.x < 3) wgCount[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t nrej = wave_count(rejected), nbad = wave_count(bad), nmain = wave_count(main);
     if (__lane_id() == 0) {
         if (nrej) atomicAdd(&wgCount[0], nrej);
         if (nbad) atomicAdd(&wgCount[1], nbad);
@@ -944,6 +956,25 @@ __device__ __forceinline__ void load_rtri(const RasterRec& r, RTri& t) {
     t.order = r.order;
 }
 
+// Record of an unclipped triangle in slot r from its vertices' screen values
+// (the values k_setup used); corner codes are not needed below tile level.
+__device__ __forceinline__ void screen_rtri(const FrameParams& P, const DeviceBuffers& B, uint32_t r, RTri& t) {
+    const uint32_t* idx = P.mesh.idx + 3 * (size_t)(r - P.slot_base);
+    const ScreenVert s0 = B.sverts[P.vert_base + idx[0]], s1 = B.sverts[P.vert_base + idx[1]],
+                     s2 = B.sverts[P.vert_base + idx[2]];
+    t.x0 = s0.x; t.y0 = s0.y; t.x1 = s1.x; t.y1 = s1.y; t.x2 = s2.x; t.y2 = s2.y;
+    t.B0 = t.y0 - t.y1; t.C0 = t.x1 - t.x0;
+    t.B1 = t.y1 - t.y2; t.C1 = t.x2 - t.x1;
+    t.B2 = t.y2 - t.y0; t.C2 = t.x0 - t.x2;
+    t.pb0 = pixel_bias(t.x0, t.y0, t.x1, t.y1);
+    t.pb1 = pixel_bias(t.x1, t.y1, t.x2, t.y2);
+    t.pb2 = pixel_bias(t.x2, t.y2, t.x0, t.y0);
+    t.inv_det = 1.0f / (float)(t.C2 * t.B1 - t.C1 * t.B2);
+    t.z0 = s0.z; t.z1 = s1.z; t.z2 = s2.z;
+    t.info = 0;
+    t.order = make_order(r, 0, false);
+}
+
 // Ordered depth key: (depth, submission order) min = the winner of the
 // sequential d <= D test (Core/FrameBuffer.cpp:54-68).  -0 and +0 compare equal.
 __device__ __forceinline__ uint32_t depth_bits(float d) {
@@ -1199,8 +1230,14 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
         // trivially accepted, or big with hierarchical rasterisation on.
         const bool large = valid && (trivial || (big && P.hierarchical));
         // Every lane loads its own record; large ones are broadcast below.
+        // References without the big bit below total_slots are compact
+        // unclipped triangles, whose record is rebuilt from their vertices.
         RTri t;
-        if (valid) load_rtri(B.rrec[pay & kRecordMask], t);
+        if (valid) {
+            const uint32_t rid = pay & kRecordMask;
+            if (big || rid >= Q.total_slots) load_rtri(B.rrec[rid], t);
+            else screen_rtri(P, B, rid, t);
+        }
         uint32_t myQ = 0;  // quads of a medium FineRasterize reference
         int fx = 0, fy = 0, fnx = 1;
         if (valid && !large) {
@@ -1421,7 +1458,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     if (Q.vert_blocks)
         hipLaunchKernelGGL(k_vertex, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts, B.sverts);
     if (Q.slot_blocks) {
-        hipLaunchKernelGGL(k_setup, dim3(Q.slot_blocks), dim3(kBlock), 0, s, Q, jobs, B);
+        hipLaunchKernelGGL(k_setup, dim3((Q.slot_blocks + kSetupSlots - 1) / kSetupSlots), dim3(kBlock), 0, s, Q,
+                           jobs, B);
         const uint32_t clipBlocks = max(1u, min(512u, (B.clip_queue_cap + 63) / 64));
         hipLaunchKernelGGL(k_clip, dim3(clipBlocks), dim3(64), 0, s, Q, jobs, B);
     }
