@@ -75,6 +75,9 @@ def frame_leg(w, steps, warmup, brief=False):
     r = Renderer(w.width, w.height)
     w.apply(r)
     dm, keep = device_mesh(w.mesh)
+    # Launch tuning (triangles per binning chunk, raster workgroups per CU)
+    # picked by the library's autotuner on this frame before timing.
+    tuning = r.autotune([w.job(mesh=dm)])
     for _ in range(warmup):
         r.render_mesh(dm)
     st = r.stats()
@@ -112,10 +115,10 @@ def frame_leg(w, steps, warmup, brief=False):
             "frac": achieved / HBM_PEAK_GBS, "traffic": RASTER_SHADE_TRAFFIC.get(w.name),
             "kernel": "k_raster+k_shade (raster/shade pass)", "algorithmic_bytes": b_rs, "kernel_ms": raster_ms}
     out = {"workload": w.name, "ms": ms, "eager_ms": eager_ms, "stage_ms": stage_avg, "roofline": roof,
-           "stats": st, "v_ref": v_ref}
+           "stats": st, "v_ref": v_ref, "tuning": tuning}
     if brief:
         out = {"workload": w.name, "triangles": T, "frame_ms": ms, "mtris_per_s": T / (ms * 1e-3) / 1e6,
-               "stage_ms": stage_avg, "roofline": roof}
+               "stage_ms": stage_avg, "roofline": roof, "tuning": tuning}
     return out
 
 
@@ -265,6 +268,7 @@ def main():
         "frame_ms": ms,
         "gpix_s": P / (ms * 1e-3) / 1e9,
         "eager_frame_ms": fr["eager_ms"],
+        "tuning": fr["tuning"],
         "stage_ms": stage_avg,
         "dominant_stage": max(stage_avg, key=stage_avg.get),
         "roofline": fr["roofline"],

@@ -1220,10 +1220,14 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     __syncthreads();
 
+    // The next slice's references are loaded before this slice's work.
+    uint32_t payNext = seg * kSegRefs + wave * kWave + lane < segEnd ? B.refs[start + seg * kSegRefs + wave * kWave + lane]
+                                                                      : 0u;
     for (uint32_t b = seg * kSegRefs + wave * kWave; b < segEnd; b += kBlock) {
         const uint32_t p = b + lane;
         const bool valid = p < segEnd;
-        const uint32_t pay = valid ? B.refs[start + p] : 0u;
+        const uint32_t pay = payNext;
+        payNext = p + kBlock < segEnd ? B.refs[start + p + kBlock] : 0u;
         const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
         const bool big = (pay >> 31) != 0;
         // Tile-level paths (RasterizeTile, Core/Renderer.cpp:255-268):
