@@ -992,30 +992,49 @@ __device__ __forceinline__ void bary_depth(const RTri& t, int px, int py, float&
     d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
 }
 
-// Depth-tests pixel (x, y): edge test at the centre (16x+8, 16y+8) when
-// useEdges (Core/Rasterizer.h:145-162), then the (depth, order) key update.
-__device__ __forceinline__ void pixel_depth(const RTri& t, int x, int y, bool useEdges, int tminX, int tminY,
-                                            uint64_t* keys) {
-    const int px = x * 16 + 8, py = y * 16 + 8;
-    if (useEdges) {
-        const int e0 = t.B0 * (px - t.x0) + t.C0 * (py - t.y0) + t.pb0;
-        const int e1 = t.B1 * (px - t.x1) + t.C1 * (py - t.y1) + t.pb1;
-        const int e2 = t.B2 * (px - t.x2) + t.C2 * (py - t.y2) + t.pb2;
-        if ((e0 | e1 | e2) < 0) return;
-    }
-    float l0, l1, d;
-    bary_depth(t, px, py, l0, l1, d);
+// Depth test of one pixel from its edge values e0..e2 (coverage when
+// useEdges, Core/Rasterizer.h:162) and barycentric numerators n0, n1
+// (Core/RasterTriangle.h:324-337), then the (depth, order) key update.
+__device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2, int n0, int n1, bool useEdges,
+                                          uint64_t* slot) {
+    if (useEdges && (e0 | e1 | e2) < 0) return;
+    const float l0 = (float)n0 * t.inv_det;
+    const float l1 = (float)n1 * t.inv_det;
+    const float l2 = 1.0f - l0 - l1;
+    const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
     if (!(d <= 1.0f)) return;   // never passes d <= D with D <= 1 (NaN included)
     const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - t.order);
-    uint64_t* slot = &keys[(y - tminY) * kTileSize + (x - tminX)];
     if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
 }
 
-// The 4 pixels of the 2x2 quad at (qx, qy) (one SSE lane each in the reference).
+// Pixels (qx + dx, qy + dy), dx, dy in {0, 1}, that lie within (maxX, maxY):
+// the 2x2 quad at (qx, qy) (one SSE lane per pixel in the reference).  Edge
+// functions and barycentric numerators are evaluated at (qx, qy)'s centre
+// (16x+8, 16y+8) and stepped by 16 B and 16 C to the other pixels, which is
+// exact in wrapping int32 arithmetic (the reference steps the same way,
+// Core/Rasterizer.h:190-198).
+__device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
+                                           uint64_t* keys, int maxX, int maxY) {
+    const int px = qx * 16 + 8, py = qy * 16 + 8;
+    const int e0 = t.B0 * (px - t.x0) + t.C0 * (py - t.y0) + t.pb0;
+    const int e1 = t.B1 * (px - t.x1) + t.C1 * (py - t.y1) + t.pb1;
+    const int e2 = t.B2 * (px - t.x2) + t.C2 * (py - t.y2) + t.pb2;
+    const int n0 = t.B1 * (px - t.x2) + t.C1 * (py - t.y2);
+    const int n1 = t.B2 * (px - t.x2) + t.C2 * (py - t.y2);
+    const int sB0 = t.B0 * 16, sB1 = t.B1 * 16, sB2 = t.B2 * 16;
+    const int sC0 = t.C0 * 16, sC1 = t.C1 * 16, sC2 = t.C2 * 16;
+    uint64_t* row = &keys[(qy - tminY) * kTileSize + (qx - tminX)];
+    const bool right = qx + 1 <= maxX, down = qy + 1 <= maxY;
+    pixel_key(t, e0, e1, e2, n0, n1, useEdges, row);
+    if (right) pixel_key(t, e0 + sB0, e1 + sB1, e2 + sB2, n0 + sB1, n1 + sB2, useEdges, row + 1);
+    if (down) pixel_key(t, e0 + sC0, e1 + sC1, e2 + sC2, n0 + sC1, n1 + sC2, useEdges, row + kTileSize);
+    if (right && down)
+        pixel_key(t, e0 + sB0 + sC0, e1 + sB1 + sC1, e2 + sB2 + sC2, n0 + sB1 + sC1, n1 + sB2 + sC2, useEdges,
+                  row + kTileSize + 1);
+}
 __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
                                            uint64_t* keys) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) pixel_depth(t, qx + (k & 1), qy + (k >> 1), useEdges, tminX, tminY, keys);
+    quad_depth(t, qx, qy, useEdges, tminX, tminY, keys, qx + 1, qy + 1);
 }
 
 __device__ __forceinline__ float pow_decl(float x, int n, float s) {
@@ -1257,8 +1276,7 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
                 // reference's even-aligned quads add pixels whose centres lie
                 // outside the bbox, hence outside the triangle, and with edges
                 // this short no edge function can wrap.
-                for (int y = minY; y <= maxY; y++)
-                    for (int x = minX; x <= maxX; x++) pixel_depth(t, x, y, true, tminX, tminY, keys);
+                if (maxX >= minX && maxY >= minY) quad_depth(t, minX, minY, true, tminX, tminY, keys, maxX, maxY);
             } else {
                 minX -= minX % 2;
                 minY -= minY % 2;
