@@ -60,10 +60,13 @@ def cpu_baseline(w, budget_s):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this bench, scaled by a
 # float4 copy kernel of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
 # profiles/pmc_raster_shade_*.md.  None where not measured.
-RASTER_SHADE_TRAFFIC = {}
+RASTER_SHADE_TRAFFIC = {
+    "sphere_grid_16x16x64_1920x1080": 53195 * 1024,   # (5618 + 5111) KiB x 2 + (7437 + 24300) KiB
+    "heightfield_2236_3840x2160": 794197 * 1024,     # (135301 + 190306) KiB x 2 + (45778 + 97205) KiB
+}
 
 
-def frame_leg(w, steps, warmup, brief=False):
+def frame_leg(w, steps, warmup, brief=False, tuning=None):
     """Frames of one workload: the timed loop replays the captured frame
     (hipGraph); a second loop of eager frames records HIP events around each
     stage on the library's stream, which time the raster/shade pass for the
@@ -75,9 +78,13 @@ def frame_leg(w, steps, warmup, brief=False):
     r = Renderer(w.width, w.height)
     w.apply(r)
     dm, keep = device_mesh(w.mesh)
-    # Launch tuning (triangles per binning chunk, raster workgroups per CU)
-    # picked by the library's autotuner on this frame before timing.
-    tuning = r.autotune([w.job(mesh=dm)])
+    # Launch tuning (triangles per binning chunk, raster workgroups per CU):
+    # given, or picked by the library's autotuner on this frame before timing.
+    if tuning:
+        r.set_tuning(*tuning)
+        tuning = {"tris_per_chunk": tuning[0], "raster_blocks_per_cu": tuning[1], "frame_ms": None}
+    else:
+        tuning = r.autotune([w.job(mesh=dm)])
     for _ in range(warmup):
         r.render_mesh(dm)
     st = r.stats()
@@ -234,6 +241,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and differentiable legs")
+    ap.add_argument("--tuning", default=None,
+                    help="TRIS_PER_CHUNK,RASTER_BLOCKS_PER_CU instead of autotuning (e.g. for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the multi-GPU path is not built",
@@ -245,7 +254,8 @@ def main():
 
     w = configs.WORKLOADS[args.workload]()
     T, V, P = w.mesh.num_triangles, w.mesh.num_vertices, w.width * w.height
-    fr = frame_leg(w, args.steps, args.warmup)
+    tuning = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None
+    fr = frame_leg(w, args.steps, args.warmup, tuning=tuning)
     ms, stage_avg, st = fr["ms"], fr["stage_ms"], fr["stats"]
     b_frame = 36 * V + 12 * T + 12 * P
 

@@ -15,9 +15,9 @@
 // (max(N.L, 0) + ambient) kd / pi, spec = pow(N.H, s) ks) clamped to [0, 1];
 // the adjoint back-propagates dL/dcolour to per-vertex positions (through
 // shading and the barycentrics), normals, colours, the light direction and
-// (ambient, kd, ks, shininess).  Per-vertex sums are reduced per wave over the
-// lanes that share a triangle before the atomics; scalar sums per workgroup
-// into doubles.
+// (ambient, kd, ks, shininess).  Per-vertex sums of a tile go to an LDS table
+// keyed by vertex (LDS float atomics), which is added to the outputs once per
+// tile; scalar sums per workgroup into doubles.
 #include <hip/hip_runtime.h>
 
 #include "pipeline.h"
@@ -56,6 +56,24 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+constexpr uint32_t kVertSlots = 1024;  // per-tile vertex table (open addressing)
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+// Slot of vertex v in the tile's table (inserted if absent), or kVertSlots
+// when the table is full.
+__device__ __forceinline__ uint32_t vertex_slot(uint32_t* keys, uint32_t v) {
+    uint32_t h = (v * 2654435761u) >> 22;  // 10-bit multiplicative hash
+    for (uint32_t probe = 0; probe < kVertSlots; probe++, h = (h + 1) & (kVertSlots - 1)) {
+        const uint32_t k = keys[h];
+        if (k == v) return h;
+        if (k == kEmpty) {
+            const uint32_t old = atomicCAS(&keys[h], kEmpty, v);
+            if (old == kEmpty || old == v) return h;
+        }
+    }
+    return kVertSlots;
+}
+
 __device__ __forceinline__ uint32_t job_of(const BatchParams& Q, const uint32_t* tbl, uint32_t i) {
     return Q.num_jobs == 1 ? 0u : __builtin_amdgcn_readfirstlane(tbl[i]);
 }
@@ -74,6 +92,13 @@ __global__ __launch_bounds__(1024) void k_adjoint(BatchParams Q, const FramePara
                                                   const uint32_t* __restrict__ ids, const float* __restrict__ d_rgb,
                                                   double* acc) {
     __shared__ float wsum[16][8];
+    __shared__ uint32_t vkeys[kVertSlots];
+    __shared__ float vsum[9][kVertSlots];
+    for (uint32_t i = threadIdx.x; i < kVertSlots; i += blockDim.x) {
+        vkeys[i] = kEmpty;
+        for (int j = 0; j < 9; j++) vsum[j][i] = 0.0f;
+    }
+    __syncthreads();
     const uint32_t gtile = blockIdx.x;
     const uint32_t job = job_of(Q, Q.tile_job, gtile);
     const FrameParams& P = jobs[job];
@@ -225,27 +250,17 @@ __global__ __launch_bounds__(1024) void k_adjoint(BatchParams Q, const FramePara
         }
     }
 
-    // Per-vertex sums: one pass per distinct triangle among the wave's lanes.
+    // Per-vertex sums: LDS atomics into the tile's vertex table; a vertex the
+    // full table cannot take goes straight to the outputs.
     const GradPtrs& G = grads[job];
     float* const outs[9] = {G.d_px, G.d_py, G.d_pz, G.d_nx, G.d_ny, G.d_nz, G.d_cr, G.d_cg, G.d_cb};
-    uint64_t pending = __ballot(has);
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t tid = __builtin_amdgcn_readlane(id, leader);
-        const uint64_t m = __ballot(has && id == tid);
-        pending &= ~m;
-        const bool mine = (m >> lane) & 1ull;
-        if (__popcll(m) == 1) {
-            if (mine)
-                for (int k = 0; k < 3; k++)
-                    for (int j = 0; j < 9; j++)
-                        if (outs[j]) atomicAdd(outs[j] + vi[k], gv[k][j]);
-        } else {
-            for (int k = 0; k < 3; k++)
-                for (int j = 0; j < 9; j++) {
-                    const float s = wave_sum(mine ? gv[k][j] : 0.0f);
-                    if ((int)lane == leader && outs[j]) atomicAdd(outs[j] + vi[k], s);
-                }
+    if (has) {
+        for (int k = 0; k < 3; k++) {
+            const uint32_t slot = vertex_slot(vkeys, vi[k]);
+            for (int j = 0; j < 9; j++) {
+                if (slot < kVertSlots) atomicAdd(&vsum[j][slot], gv[k][j]);
+                else if (outs[j]) atomicAdd(outs[j] + vi[k], gv[k][j]);
+            }
         }
     }
     // Scalars: wave sums, then one double atomic per workgroup and value.
@@ -258,6 +273,12 @@ __global__ __launch_bounds__(1024) void k_adjoint(BatchParams Q, const FramePara
         double t = 0.0;
         for (int w = 0; w < 16; w++) t += (double)wsum[w][threadIdx.x];
         if (t != 0.0) atomicAdd(&acc[8 * job + threadIdx.x], t);
+    }
+    for (uint32_t i = threadIdx.x; i < kVertSlots; i += blockDim.x) {
+        const uint32_t v = vkeys[i];
+        if (v == kEmpty) continue;
+        for (int j = 0; j < 9; j++)
+            if (outs[j] && vsum[j][i] != 0.0f) atomicAdd(outs[j] + v, vsum[j][i]);
     }
 }
 
