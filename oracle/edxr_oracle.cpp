@@ -25,6 +25,9 @@
  *                          clamp is fminf(fmaxf(c, 0), 1), so NaN gives 0
  *   IntSSE(EDX_INFINITY)   INT_MAX
  *   BoolSSE true           all ones (-1 when added to an IntSSE)
+ *   Color(Color4b b)       (b.r / 255.0f, b.g / 255.0f, b.b / 255.0f, b.a / 255.0f)
+ *   Color4b(Color c)       as FromFloats, per channel including alpha
+ *   Color4b(r, g, b)       alpha 255
  * Parity unpinned: these declared EDXUtil semantics (and the LoadSphere /
  * LoadPlane stand-in topology) have no check against EDXUtil itself, which is
  * absent; the tests pin them only against closed forms and brute force.
@@ -438,10 +441,21 @@ struct TriangleRef {
         : triId(id), acceptEdge0(a0), acceptEdge1(a1), acceptEdge2(a2), trivialAccept(a0 && a1 && a2), big(b) {}
 };
 
-/* Core/Shader.h:105-140 (single sample: 4-bit coverage) plus the triangle ID. */
+/* CoverageMask, Core/Shader.h:52-103: bit (sample << 2) + lane, up to 32 samples. */
+struct CoverageMask {
+    uint32_t bits[4] = {0, 0, 0, 0};
+    void SetBit(int i) { bits[i >> 5] |= 1u << (i & 31); }
+    void SetBit(BoolSSE m, int sampleId) {
+        for (int k = 0; k < 4; k++)
+            if (m[k] != 0) SetBit((sampleId << 2) + k);
+    }
+    bool GetBit(int i) const { return (bits[i >> 5] >> (i & 31)) & 1u; }
+};
+
+/* Core/Shader.h:105-140 plus the triangle ID. */
 struct Fragment {
     FloatSSE lambda0, lambda1;
-    int coverage;
+    CoverageMask coverage;
     int x, y;
     uint32_t vId0, vId1, vId2, coreId, triId;
     uint32_t tileId, intraTileIdx;
@@ -564,6 +578,10 @@ class Renderer {
     std::vector<uint8_t> color_;      /* bottom-up RGBA8 */
     std::vector<uint32_t> ids_;       /* bottom-up */
     std::vector<float> depthLinear_;  /* bottom-up */
+    std::vector<float> depthMS_;      /* [sample][row][x] */
+    std::vector<uint32_t> idsMS_;     /* [sample][row][x] */
+    std::vector<uint8_t> colorMS_;    /* [sample][row][x] RGBA8 */
+    int sampleCount_ = 1, msLevel_ = 0;
     std::vector<oracle_record> records_;
     std::vector<uint64_t> binOffsets_;
     std::vector<oracle_ref> binRefs_;
@@ -581,10 +599,16 @@ class Renderer {
     void CoarseRasterize(Tile& tile, const TriangleRef& ref, int blockSize, Vec2i bmin, Vec2i bmax,
                          const RasterTriangle& tri);
     void FineRasterize(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri);
+    void FineRasterizeSingle(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri);
+    void FineRasterizeMulti(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri);
     void TrivialAcceptTriangle(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri);
-    void EmitFragment(Tile& tile, const TriangleSSE& t, const RasterTriangle& tri, int x, int y, BoolSSE mask);
+    void TrivialAcceptSingle(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri);
+    void TrivialAcceptMulti(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri);
+    void EmitFragment(Tile& tile, const TriangleSSE& t, const RasterTriangle& tri, int x, int y,
+                      const CoverageMask& mask);
     /* FrameBuffer (Core/FrameBuffer.cpp:54-68) */
-    BoolSSE ZTestQuad(const FloatSSE& d, int x, int y, BoolSSE mask);
+    BoolSSE ZTestQuad(const FloatSSE& d, int x, int y, int sampleId, BoolSSE mask);
+    void Resolve();
 
     oracle_params p_{};
     Mat4 mvp_{}, mvInv_{}, raster_{};
@@ -751,11 +775,13 @@ void Renderer::Clipping(const oracle_mesh& m) {
     });
 }
 
-/* Core/FrameBuffer.cpp:54-68: tiled depth with a y-flipped quad row. */
-BoolSSE Renderer::ZTestQuad(const FloatSSE& d, int x, int y, BoolSSE mask) {
+/* Core/FrameBuffer.cpp:54-68: tiled depth with a y-flipped quad row, one
+ * 16x16-quad plane per sample. */
+BoolSSE Renderer::ZTestQuad(const FloatSSE& d, int x, int y, int sampleId, BoolSSE mask) {
     const int tileX = x >> Tile::SIZE_LOG_2, tileY = y >> Tile::SIZE_LOG_2;
     const int ix = x & (Tile::SIZE - 1), iy = y & (Tile::SIZE - 1);
-    FloatSSE& cur = depth_[(size_t)(tileY * tileDim_.x + tileX) * 256 + ((Tile::SIZE - 1 - iy) >> 1) * 16 + (ix >> 1)];
+    FloatSSE& cur = depth_[((size_t)(tileY * tileDim_.x + tileX) * sampleCount_ + sampleId) * 256 +
+                           ((Tile::SIZE - 1 - iy) >> 1) * 16 + (ix >> 1)];
     BoolSSE ret = LessEq(d, cur);
     if (p_.mutation == 2) ret = IntSSE(_mm_castps_si128(_mm_cmplt_ps(d.v, cur.v)));  /* planted: LESS */
     cur = Select(ret & mask, d, cur);
@@ -763,11 +789,11 @@ BoolSSE Renderer::ZTestQuad(const FloatSSE& d, int x, int y, BoolSSE mask) {
 }
 
 void Renderer::EmitFragment(Tile& tile, const TriangleSSE& t, const RasterTriangle& tri, int x, int y,
-                            BoolSSE mask) {
+                            const CoverageMask& mask) {
     Fragment f;
     f.lambda0 = t.lambda0;
     f.lambda1 = t.lambda1;
-    f.coverage = _mm_movemask_ps(_mm_castsi128_ps(mask.v));
+    f.coverage = mask;
     f.x = x;
     f.y = y;
     f.vId0 = tri.vId0; f.vId1 = tri.vId1; f.vId2 = tri.vId2;
@@ -781,8 +807,85 @@ void Renderer::EmitFragment(Tile& tile, const TriangleSSE& t, const RasterTriang
 /* Pixel-centre offsets, Core/Rasterizer.h:23. */
 static const IntSSE kCenterX(8, 24, 8, 24), kCenterY(8, 8, 24, 24);
 
-/* Core/Rasterizer.h:126-200 (FineRasterize_SingleSample). */
+/* Sample offsets in 28.4 units, (x, y) pairs per MSAA level: the table of
+ * FrameBuffer::MultiSampleOffsets (Core/FrameBuffer.cpp:107-191). */
+static const int kSampleOffsets[6][64] = {
+    {0, 0},
+    {4, 4, -4, -4},
+    {-2, -6, 6, -2, -6, 2, 2, 6},
+    {1, -3, -1, 3, 5, 1, -3, -5, -5, 5, -7, -1, 3, 7, 7, -7},
+    {1, 1, -1, -3, -3, 2, 4, -1, -5, -2, 2, 5, 5, 3, 3, -5, -2, 6, 0, -7, -4, -6, -6, 4, -8, 0, 7, -4, 6, 7, -7, -8},
+    {1, 1, -1, -3, -3, 2, 4, -1, -5, -2, 2, 5, 5, 3, 3, -5, -2, 6, 0, -7, -4, -6, -6, 4, -8, 0, 7, -4, 6, 7, -7, -8,
+     1, 3, -3, -3, -3, 0, 6, -2, -7, -1, 3, 4, 7, 3, 3, -6, -2, 7, 0, -4, -2, -5, -7, 6, -8, 3, 4, -1, 2, 7, 4, -8}};
+
+/* Core/Rasterizer.h:113-124 (FineRasterize dispatch on the sample count). */
 void Renderer::FineRasterize(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri) {
+    if (sampleCount_ == 1) FineRasterizeSingle(tile, bmin, bmax, tri);
+    else FineRasterizeMulti(tile, bmin, bmax, tri);
+}
+
+/* Core/Rasterizer.h:202-292 (FineRasterize_MultiSample): coverage and depth per
+ * sample at centre + offset; the fragment's barycentrics at the pixel centre. */
+void Renderer::FineRasterizeMulti(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri) {
+    int minX = std::max(bmin.x, std::min(tri.v0.x, std::min(tri.v1.x, tri.v2.x)) >> 4);
+    int maxX = std::min(bmax.x - 1, std::max(tri.v0.x, std::max(tri.v1.x, tri.v2.x)) >> 4);
+    int minY = std::max(bmin.y, std::min(tri.v0.y, std::min(tri.v1.y, tri.v2.y)) >> 4);
+    int maxY = std::min(bmax.y - 1, std::max(tri.v0.y, std::max(tri.v1.y, tri.v2.y)) >> 4);
+    minX -= minX % 2;
+    minY -= minY % 2;
+    if (maxX < minX || maxY < minY) return;
+
+    TriangleSSE t(tri, p_.mutation);
+    const ProjectedVertex& pv0 = coreVB_[tri.coreId][tri.vId0];
+    const ProjectedVertex& pv1 = coreVB_[tri.coreId][tri.vId1];
+    const ProjectedVertex& pv2 = coreVB_[tri.coreId][tri.vId2];
+    const IntSSE B0(tri.B0), C0(tri.C0), B1(tri.B1), C1(tri.C1), B2(tri.B2), C2(tri.C2);
+
+    IntSSE baseX(minX << 4), baseY(minY << 4);
+    IntSSE e0 = t.EdgeFunc0(baseX + kCenterX, baseY + kCenterY);
+    IntSSE e1 = t.EdgeFunc1(baseX + kCenterX, baseY + kCenterY);
+    IntSSE e2 = t.EdgeFunc2(baseX + kCenterX, baseY + kCenterY);
+    const IntSSE step(32);
+    for (int y = minY; y <= maxY; y += 2, baseY += step) {
+        IntSSE ey0 = e0, ey1 = e1, ey2 = e2;
+        baseX = IntSSE(minX << 4);
+        for (int x = minX; x <= maxX; x += 2, baseX += step) {
+            const IntSSE cx = baseX + kCenterX, cy = baseY + kCenterY;
+            bool genFragment = false;
+            CoverageMask mask;
+            for (int sId = 0; sId < sampleCount_; sId++) {
+                const int ox = kSampleOffsets[msLevel_][2 * sId], oy = kSampleOffsets[msLevel_][2 * sId + 1];
+                const IntSSE se0 = e0 + IntSSE(ox) * B0 + IntSSE(oy) * C0;
+                const IntSSE se1 = e1 + IntSSE(ox) * B1 + IntSSE(oy) * C1;
+                const IntSSE se2 = e2 + IntSSE(ox) * B2 + IntSSE(oy) * C2;
+                const BoolSSE covered = GreaterEqZero(se0 | se1 | se2);
+                if (Any(covered)) {
+                    t.CalcBarycentricCoord(cx + IntSSE(ox), cy + IntSSE(oy));
+                    BoolSSE z = ZTestQuad(t.GetDepth(pv0.projectedPos.z, pv1.projectedPos.z, pv2.projectedPos.z), x,
+                                          y, sId, covered);
+                    BoolSSE visible = z & covered;
+                    if (Any(visible)) {
+                        mask.SetBit(visible, sId);
+                        genFragment = true;
+                    }
+                }
+            }
+            if (genFragment) {
+                t.CalcBarycentricCoord(cx, cy);
+                EmitFragment(tile, t, tri, x, y, mask);
+            }
+            e0 += t.stepB0;
+            e1 += t.stepB1;
+            e2 += t.stepB2;
+        }
+        e0 = ey0 + t.stepC0;
+        e1 = ey1 + t.stepC1;
+        e2 = ey2 + t.stepC2;
+    }
+}
+
+/* Core/Rasterizer.h:126-200 (FineRasterize_SingleSample). */
+void Renderer::FineRasterizeSingle(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri) {
     int minX = std::max(bmin.x, std::min(tri.v0.x, std::min(tri.v1.x, tri.v2.x)) >> 4);
     int maxX = std::min(bmax.x - 1, std::max(tri.v0.x, std::max(tri.v1.x, tri.v2.x)) >> 4);
     int minY = std::max(bmin.y, std::min(tri.v0.y, std::min(tri.v1.y, tri.v2.y)) >> 4);
@@ -808,10 +911,14 @@ void Renderer::FineRasterize(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTri
             BoolSSE covered = GreaterEqZero(e0 | e1 | e2);
             if (Any(covered)) {
                 t.CalcBarycentricCoord(baseX + kCenterX, baseY + kCenterY);
-                BoolSSE z = ZTestQuad(t.GetDepth(pv0.projectedPos.z, pv1.projectedPos.z, pv2.projectedPos.z), x, y,
+                BoolSSE z = ZTestQuad(t.GetDepth(pv0.projectedPos.z, pv1.projectedPos.z, pv2.projectedPos.z), x, y, 0,
                                       covered);
                 BoolSSE visible = z & covered;
-                if (Any(visible)) EmitFragment(tile, t, tri, x, y, visible);
+                if (Any(visible)) {
+                    CoverageMask m;
+                    m.SetBit(visible, 0);
+                    EmitFragment(tile, t, tri, x, y, m);
+                }
             }
             e0 += t.stepB0;
             e1 += t.stepB1;
@@ -823,8 +930,47 @@ void Renderer::FineRasterize(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTri
     }
 }
 
-/* Core/Rasterizer.h:310-353 (TrivialAcceptTriangle_SingleSample). */
+/* Core/Rasterizer.h:302-308 (TrivialAcceptTriangle dispatch). */
 void Renderer::TrivialAcceptTriangle(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri) {
+    if (sampleCount_ == 1) TrivialAcceptSingle(tile, bmin, bmax, tri);
+    else TrivialAcceptMulti(tile, bmin, bmax, tri);
+}
+
+/* Core/Rasterizer.h:355-414 (TrivialAcceptTriangle_MultiSample). */
+void Renderer::TrivialAcceptMulti(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri) {
+    int minX = bmin.x, maxX = bmax.x - 1, minY = bmin.y, maxY = bmax.y - 1;
+    minX -= minX % 2;
+    minY -= minY % 2;
+    TriangleSSE t(tri, p_.mutation);
+    const ProjectedVertex& pv0 = coreVB_[tri.coreId][tri.vId0];
+    const ProjectedVertex& pv1 = coreVB_[tri.coreId][tri.vId1];
+    const ProjectedVertex& pv2 = coreVB_[tri.coreId][tri.vId2];
+    const IntSSE allTrue(-1);
+    for (int y = minY; y <= maxY; y += 2) {
+        for (int x = minX; x <= maxX; x += 2) {
+            const IntSSE cx = IntSSE(x << 4) + kCenterX, cy = IntSSE(y << 4) + kCenterY;
+            CoverageMask mask;
+            bool genFragment = false;
+            for (int sId = 0; sId < sampleCount_; sId++) {
+                const int ox = kSampleOffsets[msLevel_][2 * sId], oy = kSampleOffsets[msLevel_][2 * sId + 1];
+                t.CalcBarycentricCoord(cx + IntSSE(ox), cy + IntSSE(oy));
+                BoolSSE z = ZTestQuad(t.GetDepth(pv0.projectedPos.z, pv1.projectedPos.z, pv2.projectedPos.z), x, y,
+                                      sId, allTrue);
+                if (Any(z)) {
+                    mask.SetBit(z, sId);
+                    genFragment = true;
+                }
+            }
+            if (genFragment) {
+                t.CalcBarycentricCoord(cx, cy);
+                EmitFragment(tile, t, tri, x, y, mask);
+            }
+        }
+    }
+}
+
+/* Core/Rasterizer.h:310-353 (TrivialAcceptTriangle_SingleSample). */
+void Renderer::TrivialAcceptSingle(Tile& tile, Vec2i bmin, Vec2i bmax, const RasterTriangle& tri) {
     int minX = bmin.x, maxX = bmax.x - 1, minY = bmin.y, maxY = bmax.y - 1;
     minX -= minX % 2;
     minY -= minY % 2;
@@ -836,9 +982,13 @@ void Renderer::TrivialAcceptTriangle(Tile& tile, Vec2i bmin, Vec2i bmax, const R
     for (int y = minY; y <= maxY; y += 2) {
         for (int x = minX; x <= maxX; x += 2) {
             t.CalcBarycentricCoord(IntSSE(x << 4) + kCenterX, IntSSE(y << 4) + kCenterY);
-            BoolSSE z = ZTestQuad(t.GetDepth(pv0.projectedPos.z, pv1.projectedPos.z, pv2.projectedPos.z), x, y,
+            BoolSSE z = ZTestQuad(t.GetDepth(pv0.projectedPos.z, pv1.projectedPos.z, pv2.projectedPos.z), x, y, 0,
                                   allTrue);
-            if (Any(z)) EmitFragment(tile, t, tri, x, y, z);
+            if (Any(z)) {
+                CoverageMask m;
+                m.SetBit(z, 0);
+                EmitFragment(tile, t, tri, x, y, m);
+            }
         }
     }
 }
@@ -1006,18 +1156,43 @@ void Renderer::FragmentProcessing() {
 
 /* Core/Renderer.cpp:305-350 and FrameBuffer::SetPixel (Core/FrameBuffer.cpp:39-42). */
 void Renderer::UpdateFrameBuffer() {
+    const size_t P = (size_t)W_ * H_;
     pool_->parallel_for(0, (int)tiles_.size(), [&](int i) {
         const Tile& tile = tiles_[i];
         for (size_t j = 0; j < tile.fragmentBuf.size(); j++) {
             const Fragment& f = tile.fragmentBuf[j];
-            for (int k = 0; k < 4; k++) {
-                if (!(f.coverage & (1 << k))) continue;
-                const int x = f.x + (k & 1), y = f.y + (k >> 1);
-                if (x >= (int)W_ || y >= (int)H_) continue;
-                const size_t o = (size_t)(H_ - 1 - y) * W_ + x;
-                memcpy(&color_[o * 4], &shaded_[i][j * 4 + k], 4);
-                ids_[o] = f.triId;
+            for (int sId = 0; sId < sampleCount_; sId++) {
+                for (int k = 0; k < 4; k++) {
+                    if (!f.coverage.GetBit((sId << 2) + k)) continue;
+                    const int x = f.x + (k & 1), y = f.y + (k >> 1);
+                    if (x >= (int)W_ || y >= (int)H_) continue;
+                    const size_t o = sId * P + (size_t)(H_ - 1 - y) * W_ + x;
+                    memcpy(&colorMS_[o * 4], &shaded_[i][j * 4 + k], 4);
+                    idsMS_[o] = f.triId;
+                }
             }
+        }
+    });
+    Resolve();
+}
+
+/* FrameBuffer::Resolve (Core/FrameBuffer.cpp:70-87): mean of the samples'
+ * colours, with the declared Color <-> Color4b conversions.  One sample: the
+ * sample buffer is the back buffer (Core/FrameBuffer.h:55-58). */
+void Renderer::Resolve() {
+    const size_t P = (size_t)W_ * H_;
+    if (sampleCount_ == 1) {
+        memcpy(color_.data(), colorMS_.data(), P * 4);
+        return;
+    }
+    const float invSampleCount = 1.0f / float(sampleCount_);
+    pool_->parallel_for(0, (int)H_, [&](int row) {
+        for (uint32_t x = 0; x < W_; x++) {
+            const size_t o = (size_t)row * W_ + x;
+            float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int s = 0; s < sampleCount_; s++)
+                for (int ch = 0; ch < 4; ch++) c[ch] += (float)colorMS_[(s * P + o) * 4 + ch] / 255.0f;
+            for (int ch = 0; ch < 4; ch++) color_[o * 4 + ch] = ToByte(c[ch] * invSampleCount);
         }
     });
 }
@@ -1028,8 +1203,11 @@ int Renderer::Render(const oracle_params& p, const oracle_mesh& m, oracle_stats*
     if (p.width == 0 || p.height == 0) return -1;
     for (uint64_t i = 0; i < 3ull * m.num_triangles; i++)
         if (m.indices[i] >= m.num_vertices) return -2;
+    if (p.msaa_log2 < 0 || p.msaa_log2 > 5) return -1;
     p_ = p;
     Initialize(p.width, p.height, p.num_threads);
+    msLevel_ = p.msaa_log2;
+    sampleCount_ = 1 << msLevel_;
     /* Core/Renderer.cpp:85-92: SetTransform. */
     const Mat4 mv = load_mat(p.mv), proj = load_mat(p.proj);
     raster_ = load_mat(p.raster);
@@ -1039,9 +1217,11 @@ int Renderer::Render(const oracle_params& p, const oracle_mesh& m, oracle_stats*
     auto t0 = clk::now();
     /* FrameBuffer::Clear, Core/FrameBuffer.cpp:89-105. */
     color_.assign((size_t)W_ * H_ * 4, 0);
-    ids_.assign((size_t)W_ * H_, 0xFFFFFFFFu);
+    colorMS_.assign((size_t)sampleCount_ * W_ * H_ * 4, 0);
+    idsMS_.assign((size_t)sampleCount_ * W_ * H_, 0xFFFFFFFFu);
+    depth_.resize(tiles_.size() * 256 * sampleCount_);
     pool_->parallel_for(0, (int)tiles_.size(), [&](int i) {
-        for (int j = 0; j < 256; j++) depth_[(size_t)i * 256 + j] = FloatSSE(1.0f);
+        for (int j = 0; j < 256 * sampleCount_; j++) depth_[(size_t)i * 256 * sampleCount_ + j] = FloatSSE(1.0f);
     });
     VertexProcessing(m);
     auto t1 = clk::now();
@@ -1055,13 +1235,18 @@ int Renderer::Render(const oracle_params& p, const oracle_mesh& m, oracle_stats*
     auto t5 = clk::now();
 
     /* Linear depth output and introspection (not timed). */
-    depthLinear_.assign((size_t)W_ * H_, 1.0f);
-    for (uint32_t y = 0; y < H_; y++)
-        for (uint32_t x = 0; x < W_; x++) {
-            const int tX = x >> 5, tY = y >> 5, ix = x & 31, iy = y & 31;
-            const FloatSSE& q = depth_[(size_t)(tY * tileDim_.x + tX) * 256 + ((31 - (iy & ~1)) >> 1) * 16 + (ix >> 1)];
-            depthLinear_[(size_t)(H_ - 1 - y) * W_ + x] = q[(ix & 1) + 2 * (iy & 1)];
-        }
+    const size_t P = (size_t)W_ * H_;
+    depthMS_.assign(P * sampleCount_, 1.0f);
+    for (int sId = 0; sId < sampleCount_; sId++)
+        for (uint32_t y = 0; y < H_; y++)
+            for (uint32_t x = 0; x < W_; x++) {
+                const int tX = x >> 5, tY = y >> 5, ix = x & 31, iy = y & 31;
+                const FloatSSE& q = depth_[((size_t)(tY * tileDim_.x + tX) * sampleCount_ + sId) * 256 +
+                                           ((31 - (iy & ~1)) >> 1) * 16 + (ix >> 1)];
+                depthMS_[sId * P + (size_t)(H_ - 1 - y) * W_ + x] = q[(ix & 1) + 2 * (iy & 1)];
+            }
+    depthLinear_.assign(depthMS_.begin(), depthMS_.begin() + P);
+    ids_.assign(idsMS_.begin(), idsMS_.begin() + P);
     records_.clear();
     std::vector<uint64_t> coreBase(numCores_, 0);
     for (int c = 0; c < numCores_; c++) {
@@ -1155,6 +1340,16 @@ int oracle_read_depth(const oracle_ctx* c, float* d, uint64_t count) {
 int oracle_read_ids(const oracle_ctx* c, uint32_t* ids, uint64_t count) {
     if (count < c->r.ids_.size()) return -1;
     memcpy(ids, c->r.ids_.data(), c->r.ids_.size() * 4);
+    return 0;
+}
+int oracle_read_sample_depth(const oracle_ctx* c, float* depth, uint64_t count) {
+    if (count < c->r.depthMS_.size()) return -1;
+    memcpy(depth, c->r.depthMS_.data(), c->r.depthMS_.size() * 4);
+    return 0;
+}
+int oracle_read_sample_ids(const oracle_ctx* c, uint32_t* ids, uint64_t count) {
+    if (count < c->r.idsMS_.size()) return -1;
+    memcpy(ids, c->r.idsMS_.data(), c->r.idsMS_.size() * 4);
     return 0;
 }
 uint64_t oracle_num_records(const oracle_ctx* c) { return c->r.records_.size(); }

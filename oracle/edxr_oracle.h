@@ -37,6 +37,7 @@ typedef struct oracle_params {
     int32_t hierarchical;  /* Renderer::SetHierarchicalRasterize */
     int32_t num_threads;   /* 0 = hardware concurrency */
     int32_t mutation;      /* 0; non-zero plants a known mistake (pinning tests) */
+    int32_t msaa_log2;     /* Renderer::SetMSAAMode: 0 (1 sample) .. 5 (32 samples) */
 } oracle_params;
 
 typedef struct oracle_mesh {
@@ -89,6 +90,10 @@ int oracle_render(oracle_ctx*, const oracle_params*, const oracle_mesh*, oracle_
 int oracle_read_color(const oracle_ctx*, uint8_t* rgba, uint64_t bytes);
 int oracle_read_depth(const oracle_ctx*, float* depth, uint64_t count);
 int oracle_read_ids(const oracle_ctx*, uint32_t* ids, uint64_t count);
+/* Per-sample depth and winning triangle IDs, [sample][row][x] (rows as
+ * above); count = samples * W * H.  With one sample these equal the above. */
+int oracle_read_sample_depth(const oracle_ctx*, float* depth, uint64_t count);
+int oracle_read_sample_ids(const oracle_ctx*, uint32_t* ids, uint64_t count);
 uint64_t oracle_num_records(const oracle_ctx*);
 int oracle_read_records(const oracle_ctx*, oracle_record*, uint64_t count);
 /* Per-tile reference lists: offsets has num_tiles+1 entries. */

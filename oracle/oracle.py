@@ -22,7 +22,7 @@ class Params(C.Structure):
                 ("raster", C.c_float * 16), ("light", C.c_float * 3), ("shader", C.c_int32),
                 ("shininess", C.c_float), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("albedo", C.c_float * 3), ("clamp_specular", C.c_int32), ("hierarchical", C.c_int32),
-                ("num_threads", C.c_int32), ("mutation", C.c_int32)]
+                ("num_threads", C.c_int32), ("mutation", C.c_int32), ("msaa_log2", C.c_int32)]
 
 
 class MeshC(C.Structure):
@@ -59,7 +59,8 @@ def lib():
         l.oracle_create.restype = C.c_void_p
         l.oracle_destroy.argtypes = [C.c_void_p]
         l.oracle_render.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(MeshC), C.POINTER(Stats)]
-        for n in ("oracle_read_color", "oracle_read_depth", "oracle_read_ids"):
+        for n in ("oracle_read_color", "oracle_read_depth", "oracle_read_ids", "oracle_read_sample_depth",
+                  "oracle_read_sample_ids"):
             getattr(l, n).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
         l.oracle_num_records.argtypes = [C.c_void_p]
         l.oracle_num_records.restype = C.c_uint64
@@ -98,7 +99,7 @@ class Oracle:
 
     def render(self, mesh, width, height, mv, proj, raster, light=(1.0, 1.0, -1.0), shader=SHADER_VC_BLINN_PHONG,
                shininess=32.0, ambient=0.2, kd=3.0, ks=3.0, albedo=(0.9, 0.9, 0.9), clamp_specular=0,
-               hierarchical=1, threads=0, mutation=0):
+               hierarchical=1, threads=0, mutation=0, msaa_log2=0):
         p = Params()
         p.width, p.height = width, height
         for name, m in (("mv", mv), ("proj", proj), ("raster", raster)):
@@ -107,6 +108,7 @@ class Oracle:
         p.shader, p.shininess, p.ambient, p.kd, p.ks = shader, shininess, ambient, kd, ks
         p.albedo[:] = [float(v) for v in albedo]
         p.clamp_specular, p.hierarchical, p.num_threads, p.mutation = clamp_specular, hierarchical, threads, mutation
+        p.msaa_log2 = msaa_log2
         planes = [_f(a) for a in (mesh.px, mesh.py, mesh.pz, mesh.nx, mesh.ny, mesh.nz, mesh.cr, mesh.cg, mesh.cb)]
         idx = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
         self._keep = (planes, idx)
@@ -116,7 +118,7 @@ class Oracle:
         rc = lib().oracle_render(self._ctx, C.byref(p), C.byref(m), C.byref(st))
         if rc != 0:
             raise RuntimeError("oracle_render failed: %d" % rc)
-        self.width, self.height = width, height
+        self.width, self.height, self.samples = width, height, 1 << msaa_log2
         return {n: getattr(st, n) for n, _ in Stats._fields_}
 
     def render_workload(self, w, **kw):
@@ -138,6 +140,18 @@ class Oracle:
     def ids(self):
         out = np.empty((self.height, self.width), np.uint32)
         assert lib().oracle_read_ids(self._ctx, out.ctypes.data, out.size) == 0
+        return out
+
+    def sample_depth(self):
+        """(samples, H, W) per-sample depth."""
+        out = np.empty((self.samples, self.height, self.width), np.float32)
+        assert lib().oracle_read_sample_depth(self._ctx, out.ctypes.data, out.size) == 0
+        return out
+
+    def sample_ids(self):
+        """(samples, H, W) per-sample winning triangle IDs."""
+        out = np.empty((self.samples, self.height, self.width), np.uint32)
+        assert lib().oracle_read_sample_ids(self._ctx, out.ctypes.data, out.size) == 0
         return out
 
     def records(self):

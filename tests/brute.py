@@ -166,3 +166,74 @@ def shade(mesh, ids, L0, L1, s, eye, light, shader, shininess, ambient=0.2, kd=3
 def to_bottom_up(a):
     """Raster rows (y down) -> GetBackBuffer rows (row H-1-y)."""
     return a[::-1].copy()
+
+
+# Sample offsets in 28.4 units per MSAA level (x, y pairs): the reference's
+# FrameBuffer::MultiSampleOffsets table (Core/FrameBuffer.cpp:107-191).
+SAMPLE_OFFSETS = [
+    [(0, 0)],
+    [(4, 4), (-4, -4)],
+    [(-2, -6), (6, -2), (-6, 2), (2, 6)],
+    [(1, -3), (-1, 3), (5, 1), (-3, -5), (-5, 5), (-7, -1), (3, 7), (7, -7)],
+    [(1, 1), (-1, -3), (-3, 2), (4, -1), (-5, -2), (2, 5), (5, 3), (3, -5), (-2, 6), (0, -7), (-4, -6), (-6, 4),
+     (-8, 0), (7, -4), (6, 7), (-7, -8)],
+]
+SAMPLE_OFFSETS.append(SAMPLE_OFFSETS[4] + [
+    (1, 3), (-3, -3), (-3, 0), (6, -2), (-7, -1), (3, 4), (7, 3), (3, -6), (-2, 7), (0, -4), (-2, -5), (-7, 6),
+    (-8, 3), (4, -1), (2, 7), (4, -8)])
+
+
+def rasterize_ms(mesh, width, height, mvp, raster, level):
+    """Multisampling (Core/Rasterizer.h:202-300,355-415): coverage, depth and the
+    sequential LEQUAL test per sample at centre + offset; a winner is shaded with
+    its barycentrics at the pixel centre.  Returns per-sample (ids, depth,
+    centre lambda0, centre lambda1), shape (S, H, W), in raster orientation."""
+    s = setup(mesh, mvp, raster)
+    offs = SAMPLE_OFFSETS[level]
+    S = len(offs)
+    ys, xs = np.mgrid[0:height, 0:width]
+    cx = xs.astype(np.int64) * 16 + 8
+    cy = ys.astype(np.int64) * 16 + 8
+    D = np.full((S, height, width), F(1), np.float32)
+    ids = np.full((S, height, width), 0xFFFFFFFF, np.uint32)
+    L0 = np.zeros((S, height, width), np.float32)
+    L1 = np.zeros((S, height, width), np.float32)
+    X, Y, B, C = s["X"], s["Y"], s["B"], s["C"]
+    for t in range(X.shape[0]):
+        if not s["valid"][t]:
+            continue
+        bias = [-1 if (Y[t, (k + 1) % 3] > Y[t, k]) or (Y[t, k] == Y[t, (k + 1) % 3] and X[t, k] > X[t, (k + 1) % 3])
+                else 0 for k in range(3)]
+        c0 = (wrap32(B[1][t] * (cx - X[t, 2]) + C[1][t] * (cy - Y[t, 2])).astype(F) * s["inv_det"][t])
+        c1 = (wrap32(B[2][t] * (cx - X[t, 2]) + C[2][t] * (cy - Y[t, 2])).astype(F) * s["inv_det"][t])
+        z = s["z"][t]
+        for si, (ox, oy) in enumerate(offs):
+            px, py = cx + ox, cy + oy
+            cov = np.ones((height, width), bool)
+            for k in range(3):
+                a = k
+                e = wrap32(B[k][t] * (px - X[t, a]) + C[k][t] * (py - Y[t, a]) + bias[k])
+                cov &= e >= 0
+            if not cov.any():
+                continue
+            l0 = wrap32(B[1][t] * (px - X[t, 2]) + C[1][t] * (py - Y[t, 2])).astype(F) * s["inv_det"][t]
+            l1 = wrap32(B[2][t] * (px - X[t, 2]) + C[2][t] * (py - Y[t, 2])).astype(F) * s["inv_det"][t]
+            l2 = (F(1) - l0) - l1
+            d = (l0 * z[0] + l1 * z[1]) + l2 * z[2]
+            win = cov & (d <= D[si])
+            D[si] = np.where(win, d, D[si])
+            ids[si] = np.where(win, np.uint32(t), ids[si])
+            L0[si] = np.where(win, c0, L0[si])
+            L1[si] = np.where(win, c1, L1[si])
+    return ids, D, L0, L1, s
+
+
+def resolve(samples_rgba):
+    """FrameBuffer::Resolve with the declared conversions: the mean of
+    b / 255.0f over the samples (summed in sample order), times 1 / S, then
+    FromFloats."""
+    S = samples_rgba.shape[0]
+    acc = np.zeros(samples_rgba.shape[1:], np.float32)
+    for si in range(S):
+        acc = (acc + samples_rgba[si].astype(F) / F(255)).astype(F)
+    return to_byte(acc * (F(1) / F(S)))

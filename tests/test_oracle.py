@@ -443,3 +443,38 @@ def test_scene_sizes_and_winding(built):
     to_eye = np.array([0.0, 0.0, -3.0]) - cen
     assert len(ids) > 100
     assert ((n * cen).sum(1) > 0).all() and ((n * to_eye).sum(1) > 0).all()
+
+
+# ---------------------------------------------------------------- multisampling
+@pytest.mark.parametrize("level", [1, 2, 3, 4, 5])
+def test_oracle_msaa_equals_brute_force(built, level):
+    """Per-sample coverage, depth and winners, centre shading and the resolve
+    (Core/Rasterizer.h:202-300,355-415, Core/FrameBuffer.cpp:70-87) against the
+    per-sample numpy restatement, hierarchical on and off."""
+    for name, mesh, w, h, mv, p, r in SCENES[:2]:
+        mvp = O.mat_mul(p, mv)
+        ids, D, L0, L1, s = brute.rasterize_ms(mesh, w, h, mvp, r, level)
+        cols = np.stack([brute.shade(mesh, ids[k], L0[k], L1[k], s, eye_of(mv), (1.0, 1.0, -1.0), 2, 32.0)
+                         for k in range(ids.shape[0])])
+        res = brute.resolve(cols)
+        for hier in (1, 0):
+            o = O.Oracle()
+            o.render(mesh, w, h, mv, p, r, shader=2, shininess=32.0, msaa_log2=level, hierarchical=hier, threads=3)
+            np.testing.assert_array_equal(o.sample_ids(), ids[:, ::-1])
+            np.testing.assert_array_equal(o.sample_depth().view(np.uint32), D[:, ::-1].view(np.uint32))
+            np.testing.assert_array_equal(o.color(), res[::-1])
+
+
+def test_msaa_half_covered_pixels_known_answer(built):
+    """A rectangle whose left and right edges run through pixel centres: with
+    4 samples (offsets x = -2, 6, -6, 2 sixteenths) the edge pixels have exactly
+    the samples on the inside half covered, so alpha resolves to 2/4 * 255."""
+    w, h = 10, 8
+    tris = [[(2.5, 1.0), (6.5, 1.0), (2.5, 5.0)], [(6.5, 1.0), (6.5, 5.0), (2.5, 5.0)]]
+    o = O.Oracle()
+    identity_render(o, mesh_from_pixels(tris, w, h), w, h, msaa_log2=2)
+    a = o.color()[::-1, :, 3]  # raster orientation
+    # interior pixels fully covered, the edge columns x = 2 and x = 6 half covered
+    assert (a[1:5, 3:6] == 255).all()
+    assert (a[1:5, 2] == int(0.5 * 255 + 0.5)).all() and (a[1:5, 6] == int(0.5 * 255 + 0.5)).all()
+    assert (a[:, :2] == 0).all() and (a[:, 7:] == 0).all()
