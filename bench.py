@@ -66,16 +66,19 @@ RASTER_SHADE_TRAFFIC = {
 }
 
 
-def frame_leg(w, steps, warmup, brief=False, tuning=None):
+def frame_leg(w, steps, warmup, brief=False, tuning=None, msaa=0):
     """Frames of one workload: the timed loop replays the captured frame
     (hipGraph); a second loop of eager frames records HIP events around each
     stage on the library's stream, which time the raster/shade pass for the
     roofline (algorithmic bytes: 4 B bin entry + 40 B minimal setup record per
-    tile reference, 44 B per referenced vertex, 12 B per pixel written)."""
+    tile reference, 44 B per referenced vertex, and per pixel 4 B of colour
+    plus 8 B of depth and triangle ID per sample).  msaa = log2 of the samples
+    per pixel (Renderer::SetMSAAMode)."""
     import torch
     from edxraster_amd import Renderer
-    T, P = w.mesh.num_triangles, w.width * w.height
+    T, P, S = w.mesh.num_triangles, w.width * w.height, 1 << msaa
     r = Renderer(w.width, w.height)
+    r.set_msaa_mode(msaa)
     w.apply(r)
     dm, keep = device_mesh(w.mesh)
     # Launch tuning (triangles per binning chunk, raster workgroups per CU):
@@ -115,16 +118,16 @@ def frame_leg(w, steps, warmup, brief=False, tuning=None):
     r.close()
     stage_avg = {k: v / max(frames, 1) for k, v in stage_ms.items()}
     n_ref = int(st["refs"])
-    b_rs = n_ref * 44 + 44 * v_ref + 12 * P
+    b_rs = n_ref * 44 + 44 * v_ref + (4 + 8 * S) * P
     raster_ms = stage_avg["raster"]
     achieved = b_rs / (raster_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": RASTER_SHADE_TRAFFIC.get(w.name),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": RASTER_SHADE_TRAFFIC.get(w.name) if S == 1 else None,
             "kernel": "k_raster+k_shade (raster/shade pass)", "algorithmic_bytes": b_rs, "kernel_ms": raster_ms}
     out = {"workload": w.name, "ms": ms, "eager_ms": eager_ms, "stage_ms": stage_avg, "roofline": roof,
            "stats": st, "v_ref": v_ref, "tuning": tuning}
     if brief:
-        out = {"workload": w.name, "triangles": T, "frame_ms": ms, "mtris_per_s": T / (ms * 1e-3) / 1e6,
+        out = {"workload": w.name, "samples": S, "triangles": T, "frame_ms": ms, "mtris_per_s": T / (ms * 1e-3) / 1e6,
                "stage_ms": stage_avg, "roofline": roof, "tuning": tuning}
     return out
 
@@ -240,7 +243,9 @@ def main():
     ap.add_argument("--workload", default="grid1m")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the batched and differentiable legs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 10M-triangle, multisampled, batched and differentiable legs")
+    ap.add_argument("--msaa", type=int, default=0, help="log2 samples per pixel of the timed frame (profiling runs)")
     ap.add_argument("--tuning", default=None,
                     help="TRIS_PER_CHUNK,RASTER_BLOCKS_PER_CU instead of autotuning (e.g. for profiling runs)")
     args = ap.parse_args()
@@ -255,7 +260,7 @@ def main():
     w = configs.WORKLOADS[args.workload]()
     T, V, P = w.mesh.num_triangles, w.mesh.num_vertices, w.width * w.height
     tuning = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None
-    fr = frame_leg(w, args.steps, args.warmup, tuning=tuning)
+    fr = frame_leg(w, args.steps, args.warmup, tuning=tuning, msaa=args.msaa)
     ms, stage_avg, st = fr["ms"], fr["stage_ms"], fr["stats"]
     b_frame = 36 * V + 12 * T + 12 * P
 
@@ -272,7 +277,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+int32",
         "data": "synthetic (seeded SplitMix64 scene generated on the host, resident in HBM)",
-        "config": {"workload": w.name, "triangles": T, "vertices": V, "width": w.width, "height": w.height,
+        "config": {"workload": w.name, "samples_per_pixel": 1 << args.msaa, "triangles": T, "vertices": V, "width": w.width, "height": w.height,
                    "shader": "vertex-colour Blinn-Phong s=32", "parallelism": "single GPU",
                    "launch": "hipGraph replay per frame"},
         "frame_ms": ms,
@@ -290,6 +295,10 @@ def main():
         if args.workload != "hf10m":
             h = configs.WORKLOADS["hf10m"]()
             out["frame_10m"] = frame_leg(h, max(5, args.steps // 2), 2, brief=True)
+        # the same frame with 4 samples per pixel, at the tuning picked above
+        t = fr["tuning"]
+        out["msaa_4x"] = frame_leg(w, max(5, args.steps // 2), 2, brief=True, msaa=2,
+                                   tuning=(t["tris_per_chunk"], t["raster_blocks_per_cu"]))
         out["batched"] = batched_leg(max(3, args.steps // 4), 2)
         out["differentiable"] = diff_leg(max(3, args.steps // 4), 2)
     if not args.no_cpu:

@@ -28,6 +28,7 @@ STATE_GRAPH = 4
 STATE_TRIS_PER_CHUNK = 5
 STATE_RASTER_BLOCKS_PER_CU = 6
 STATE_FLOAT_COLOR = 7
+STATE_MSAA = 8
 STAGES = ("setup", "bin_count", "bin_scan", "bin_scatter", "raster")
 
 MEM_HOST = 0
@@ -90,6 +91,7 @@ SIGNATURES = [
     ("edxr_read_color", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("edxr_read_depth", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("edxr_read_triangle_ids", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("edxr_read_sample_buffers", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("edxr_device_buffers", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                       C.POINTER(C.c_void_p)]),
     ("edxr_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),

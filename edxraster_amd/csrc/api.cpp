@@ -129,6 +129,7 @@ struct HostBuf {
 struct JobInput {
     MeshPtrs mesh;
     uint32_t V, T, W, H;
+    uint32_t sampleLevel;
     float mvp[16], raster[16], eye[3], light[3];
     edxr_material mat;
     int32_t shader, hierarchical;
@@ -160,6 +161,7 @@ struct edxr_ctx {
     int32_t profile = 0;
     int32_t useGraph = 0;
     int32_t floatColor = 0;
+    int32_t msaaLog2 = 0;
     hipEvent_t events[kNumStages + 1] = {};
     double stageMs[kNumStages] = {};
     uint64_t profiledFrames = 0;
@@ -238,10 +240,10 @@ int set_size(edxr_ctx* c, uint32_t w, uint32_t h) {
     c->tilesX = tx;
     c->tilesY = ty;
     c->nTiles = tx * ty;
-    const size_t P = (size_t)w * h;
+    const size_t P = (size_t)w * h, S = (size_t)1 << c->msaaLog2;
     hipError_t e;
-    if ((e = c->color.ensure(P)) != hipSuccess || (e = c->depth.ensure(P)) != hipSuccess ||
-        (e = c->ids.ensure(P)) != hipSuccess)
+    if ((e = c->color.ensure(P)) != hipSuccess || (e = c->depth.ensure(S * P)) != hipSuccess ||
+        (e = c->ids.ensure(S * P)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(frame buffers)");
     c->rendered = false;
     return EDXR_OK;
@@ -299,6 +301,8 @@ FrameParams job_params(const JobInput& in, uint32_t tilesX, uint32_t tilesY) {
     P.num_tiles = tilesX * tilesY;
     P.num_vertices = in.V;
     P.num_triangles = in.T;
+    P.sample_level = in.sampleLevel;
+    P.samples = 1u << in.sampleLevel;
     P.mesh = in.mesh;
     return P;
 }
@@ -338,6 +342,7 @@ int build_plan(edxr_ctx* c, const std::vector<JobInput>& in, Plan& pl) {
         pl.pixels += (uint64_t)j.W * j.H;
         Q.max_job_tiles = std::max(Q.max_job_tiles, P.num_tiles);
         Q.max_job_chunks = std::max(Q.max_job_chunks, P.num_chunks);
+        Q.max_samples = std::max(Q.max_samples, P.samples);
         pl.jobs.push_back(P);
         if (vert > 0xF0000000ull || slot + 65536 > kRecordMask) return c->fail(EDXR_E_LIMIT, "too many triangles");
     }
@@ -438,7 +443,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
             (e = c->binCounts.ensure(std::max<uint64_t>(pl.countEntries, 1))) != hipSuccess ||
             (e = c->refs.ensure(c->refCap)) != hipSuccess ||
             (e = c->work.ensure((size_t)Q.total_tiles + c->refCap / kSegRefs + 2)) != hipSuccess ||
-            (e = c->partials.ensure(c->work.cap * kTileSize * kTileSize)) != hipSuccess ||
+            (e = c->partials.ensure(c->work.cap * kTileSize * kTileSize * Q.max_samples)) != hipSuccess ||
             (e = c->verts.ensure((size_t)V + c->extraVertCap)) != hipSuccess ||
             (e = c->sverts.ensure(std::max<uint32_t>(V, 1))) != hipSuccess)
             return c->hip_fail(e, "hipMalloc(pipeline)");
@@ -731,6 +736,13 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
             c->trisPerChunk = (uint32_t)value;
             return EDXR_OK;
         case EDXR_STATE_FLOAT_COLOR: c->floatColor = value ? 1 : 0; return EDXR_OK;
+        case EDXR_STATE_MSAA: {
+            if (value < 0 || value > (int32_t)kMaxSampleLevel)
+                return c->fail(EDXR_E_INVALID, "MSAA level out of range (0..5)");
+            // Renderer::SetMSAAMode resizes the frame buffer (Core/Renderer.cpp:94-98).
+            c->msaaLog2 = value;
+            return set_size(c, c->W, c->H);
+        }
         case EDXR_STATE_RASTER_BLOCKS_PER_CU:
             if (value < 0 || value > 32) return c->fail(EDXR_E_INVALID, "raster blocks per CU out of range (0..32)");
             c->rasterPerCU = (uint32_t)value;
@@ -784,6 +796,7 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     in.mat = c->mat;
     in.shader = c->shader;
     in.hierarchical = c->hierarchical;
+    in.sampleLevel = (uint32_t)c->msaaLog2;
     Plan pl;
     if ((r = build_plan(c, std::vector<JobInput>{in}, pl)) != EDXR_OK) return r;
     c->rendered = false;
@@ -834,6 +847,7 @@ int edxr_render_batch(edxr_ctx* c, const edxr_job* jobs, uint32_t n) {
         d.mat = j.material;
         d.shader = j.shader;
         d.hierarchical = j.hierarchical ? 1 : 0;
+        d.sampleLevel = 0;
     }
     if ((r = stage_batch_meshes(c, jobs, n, in)) != EDXR_OK) return r;
     Plan pl;
@@ -915,6 +929,7 @@ int edxr_backward(edxr_ctx* c, const float* d_rgb, const edxr_grads* g, uint32_t
     if (!c->lastPlanValid) return c->fail(EDXR_E_STATE, "backward before a render");
     const Plan& pl = c->lastPlan;
     if (n != pl.Q.num_jobs) return c->fail(EDXR_E_INVALID, "one edxr_grads per job of the last render");
+    if (pl.Q.max_samples > 1) return c->fail(EDXR_E_STATE, "backward of a multisampled frame is not supported");
     for (const FrameParams& P : pl.jobs) {
         const float* R = P.raster;
         if (R[2] != 0.0f || R[6] != 0.0f || R[12] != 0.0f || R[13] != 0.0f || R[14] != 0.0f || R[15] != 1.0f ||
@@ -1011,6 +1026,18 @@ int edxr_read_triangle_ids(edxr_ctx* c, uint32_t* ids, size_t count) {
     hipError_t e = hipMemcpyAsync(ids, c->ids.p, need * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     return e == hipSuccess ? EDXR_OK : c->hip_fail(e, "read_triangle_ids");
+}
+
+int edxr_read_sample_buffers(edxr_ctx* c, float* depth, uint32_t* ids, size_t count) {
+    if (!c) return EDXR_E_INVALID;
+    if (!c->rendered) return c->fail(EDXR_E_STATE, "nothing rendered");
+    const size_t need = ((size_t)1 << c->msaaLog2) * c->W * c->H;
+    if (count < need) return c->fail(EDXR_E_INVALID, "buffer too small");
+    hipError_t e = hipSuccess;
+    if (depth) e = hipMemcpyAsync(depth, c->depth.p, need * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && ids) e = hipMemcpyAsync(ids, c->ids.p, need * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? EDXR_OK : c->hip_fail(e, "read_sample_buffers");
 }
 
 int edxr_device_buffers(edxr_ctx* c, void** color, void** depth, void** ids) {

@@ -16,6 +16,8 @@ constexpr uint32_t kMaxDim = 8192;
 constexpr uint32_t kRecordBits = 28;         // record index bits in a tile reference
 constexpr uint32_t kRecordMask = (1u << kRecordBits) - 1;
 constexpr int kMaxPoly = 16;                 // clip polygon capacity (convex input needs <= 9)
+constexpr uint32_t kMaxSampleLevel = 5;      // up to 32 samples per pixel
+constexpr uint32_t kPassSamples = 4;         // samples whose keys one raster pass keeps in LDS
 constexpr uint32_t kSegRefs = 2048;          // tile references per raster work item
 constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
@@ -58,6 +60,7 @@ struct FrameParams {
     uint32_t width, height, tiles_x, tiles_y, num_tiles;
     uint32_t num_vertices, num_triangles;
     uint32_t chunk_slots, num_chunks;
+    uint32_t samples, sample_level;  // MSAA: 1 << sample_level samples per pixel
     uint32_t vert_base, slot_base, tile_base, chunk_base;
     uint64_t count_base, group_base, pix_base;
     MeshPtrs mesh;
@@ -73,6 +76,7 @@ struct BatchParams {
     uint32_t vert_blocks, slot_blocks;
     uint32_t max_job_tiles;     // max over jobs of num_tiles
     uint32_t max_job_chunks;    // max over jobs of num_chunks
+    uint32_t max_samples;       // max over jobs of samples
     const uint32_t* vblock_job;
     const uint32_t* sblock_job;
     const uint32_t* chunk_job;
@@ -200,11 +204,11 @@ struct DeviceBuffers {
     uint32_t* work;        // raster work items: global tile | segment << kTileBits
     uint32_t work_cap;
     uint32_t* tile_item;   // per global tile: index of its first work item
-    uint64_t* partials;    // work_cap * 1024 (depth, order) keys, one tile image per work item
+    uint64_t* partials;    // work_cap * samples * 1024 (depth, order) keys, one tile image per item and sample
     uint32_t raster_blocks;
     uint32_t* counters;
-    uint32_t* color;       // every job's pixels at its pix_base
-    float* depth;
+    uint32_t* color;       // every job's pixels at its pix_base (resolved)
+    float* depth;          // samples planes of the job's pixels (single frame), sample-major
     uint32_t* ids;
     float* color_f32;      // optional clamped float RGB (3 per pixel) for the adjoint
 };

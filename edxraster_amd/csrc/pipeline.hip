@@ -116,6 +116,19 @@ __global__ void k_reset(uint32_t* counters, FrameParams* jobs, FrameParams job0,
     if (writeJob && threadIdx.x == 0) jobs[0] = job0;
 }
 
+// Sample offsets in 28.4 units per MSAA level, (x, y) per sample: the table of
+// FrameBuffer::MultiSampleOffsets (Core/FrameBuffer.cpp:107-191).
+__constant__ int2 c_sample_offsets[kMaxSampleLevel + 1][32] = {
+    {{0, 0}},
+    {{4, 4}, {-4, -4}},
+    {{-2, -6}, {6, -2}, {-6, 2}, {2, 6}},
+    {{1, -3}, {-1, 3}, {5, 1}, {-3, -5}, {-5, 5}, {-7, -1}, {3, 7}, {7, -7}},
+    {{1, 1}, {-1, -3}, {-3, 2}, {4, -1}, {-5, -2}, {2, 5}, {5, 3}, {3, -5}, {-2, 6}, {0, -7}, {-4, -6}, {-6, 4},
+     {-8, 0}, {7, -4}, {6, 7}, {-7, -8}},
+    {{1, 1}, {-1, -3}, {-3, 2}, {4, -1}, {-5, -2}, {2, 5}, {5, 3}, {3, -5}, {-2, 6}, {0, -7}, {-4, -6}, {-6, 4},
+     {-8, 0}, {7, -4}, {6, 7}, {-7, -8}, {1, 3}, {-3, -3}, {-3, 0}, {6, -2}, {-7, -1}, {3, 4}, {7, 3}, {3, -6},
+     {-2, 7}, {0, -4}, {-2, -5}, {-7, 6}, {-8, 3}, {4, -1}, {2, 7}, {4, -8}}};
+
 namespace {
 // Job of a block-uniform index i (the block -> job tables are null for a
 // single job).  The result is wave-uniform, so job fields load as scalars.
@@ -992,19 +1005,48 @@ __device__ __forceinline__ void bary_depth(const RTri& t, int px, int py, float&
     d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
 }
 
+// Samples of one raster pass: MSAA level, first sample and count; sample
+// k's keys live in plane k (kTileSize * kTileSize keys apart).
+struct SamplePass {
+    uint32_t level, base, n;
+};
+
 // Depth test of one pixel from its edge values e0..e2 (coverage when
 // useEdges, Core/Rasterizer.h:162) and barycentric numerators n0, n1
-// (Core/RasterTriangle.h:324-337), then the (depth, order) key update.
+// (Core/RasterTriangle.h:324-337), then the (depth, order) key update.  With
+// multisampling each sample of the pass is tested at centre + offset
+// (Core/Rasterizer.h:243-266): e + ox B + oy C, wrapping as the reference.
+template <bool kMS>
 __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2, int n0, int n1, bool useEdges,
-                                          uint64_t* slot) {
-    if (useEdges && (e0 | e1 | e2) < 0) return;
-    const float l0 = (float)n0 * t.inv_det;
-    const float l1 = (float)n1 * t.inv_det;
-    const float l2 = 1.0f - l0 - l1;
-    const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
-    if (!(d <= 1.0f)) return;   // never passes d <= D with D <= 1 (NaN included)
-    const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - t.order);
-    if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+                                          uint64_t* slot, const SamplePass& sp) {
+    if (!kMS) {
+        if (useEdges && (e0 | e1 | e2) < 0) return;
+        const float l0 = (float)n0 * t.inv_det;
+        const float l1 = (float)n1 * t.inv_det;
+        const float l2 = 1.0f - l0 - l1;
+        const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
+        if (!(d <= 1.0f)) return;   // never passes d <= D with D <= 1 (NaN included)
+        const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - t.order);
+        if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+        return;
+    }
+    for (uint32_t k = 0; k < sp.n; k++) {
+        const int2 o = c_sample_offsets[sp.level][sp.base + k];
+        if (useEdges) {
+            const int s0 = e0 + o.x * t.B0 + o.y * t.C0;
+            const int s1 = e1 + o.x * t.B1 + o.y * t.C1;
+            const int s2 = e2 + o.x * t.B2 + o.y * t.C2;
+            if ((s0 | s1 | s2) < 0) continue;
+        }
+        const float l0 = (float)(n0 + o.x * t.B1 + o.y * t.C1) * t.inv_det;
+        const float l1 = (float)(n1 + o.x * t.B2 + o.y * t.C2) * t.inv_det;
+        const float l2 = 1.0f - l0 - l1;
+        const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
+        if (!(d <= 1.0f)) continue;
+        const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - t.order);
+        uint64_t* sl = slot + k * (kTileSize * kTileSize);
+        if (key < *sl) atomicMin((unsigned long long*)sl, (unsigned long long)key);
+    }
 }
 
 // Pixels (qx + dx, qy + dy), dx, dy in {0, 1}, that lie within (maxX, maxY):
@@ -1013,8 +1055,9 @@ __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2,
 // (16x+8, 16y+8) and stepped by 16 B and 16 C to the other pixels, which is
 // exact in wrapping int32 arithmetic (the reference steps the same way,
 // Core/Rasterizer.h:190-198).
+template <bool kMS>
 __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
-                                           uint64_t* keys, int maxX, int maxY) {
+                                           uint64_t* keys, int maxX, int maxY, const SamplePass& sp) {
     const int px = qx * 16 + 8, py = qy * 16 + 8;
     const int e0 = t.B0 * (px - t.x0) + t.C0 * (py - t.y0) + t.pb0;
     const int e1 = t.B1 * (px - t.x1) + t.C1 * (py - t.y1) + t.pb1;
@@ -1025,16 +1068,17 @@ __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool u
     const int sC0 = t.C0 * 16, sC1 = t.C1 * 16, sC2 = t.C2 * 16;
     uint64_t* row = &keys[(qy - tminY) * kTileSize + (qx - tminX)];
     const bool right = qx + 1 <= maxX, down = qy + 1 <= maxY;
-    pixel_key(t, e0, e1, e2, n0, n1, useEdges, row);
-    if (right) pixel_key(t, e0 + sB0, e1 + sB1, e2 + sB2, n0 + sB1, n1 + sB2, useEdges, row + 1);
-    if (down) pixel_key(t, e0 + sC0, e1 + sC1, e2 + sC2, n0 + sC1, n1 + sC2, useEdges, row + kTileSize);
+    pixel_key<kMS>(t, e0, e1, e2, n0, n1, useEdges, row, sp);
+    if (right) pixel_key<kMS>(t, e0 + sB0, e1 + sB1, e2 + sB2, n0 + sB1, n1 + sB2, useEdges, row + 1, sp);
+    if (down) pixel_key<kMS>(t, e0 + sC0, e1 + sC1, e2 + sC2, n0 + sC1, n1 + sC2, useEdges, row + kTileSize, sp);
     if (right && down)
-        pixel_key(t, e0 + sB0 + sC0, e1 + sB1 + sC1, e2 + sB2 + sC2, n0 + sB1 + sC1, n1 + sB2 + sC2, useEdges,
-                  row + kTileSize + 1);
+        pixel_key<kMS>(t, e0 + sB0 + sC0, e1 + sB1 + sC1, e2 + sB2 + sC2, n0 + sB1 + sC1, n1 + sB2 + sC2, useEdges,
+                       row + kTileSize + 1, sp);
 }
+template <bool kMS>
 __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
-                                           uint64_t* keys) {
-    quad_depth(t, qx, qy, useEdges, tminX, tminY, keys, qx + 1, qy + 1);
+                                           uint64_t* keys, const SamplePass& sp) {
+    quad_depth<kMS>(t, qx, qy, useEdges, tminX, tminY, keys, qx + 1, qy + 1, sp);
 }
 
 __device__ __forceinline__ float pow_decl(float x, int n, float s) {
@@ -1116,8 +1160,10 @@ namespace {
 // TrivialAcceptTriangle over the tile and CoarseRasterize's stepped quadrant
 // tests with FineRasterize inside partly covered quadrants
 // (Core/Rasterizer.h:32-200,310-353).
+template <bool kMS>
 __device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RTri& t, uint32_t pay,
-                                                  int tminX, int tminY, int tmaxX, int tmaxY, uint64_t* keys) {
+                                                  int tminX, int tminY, int tmaxX, int tmaxY, uint64_t* keys,
+                                                  const SamplePass& sp) {
     const uint32_t lane = __lane_id();
     const int lx = (int)(lane & 7u) * 2, ly = (int)(lane >> 3) * 2;
     const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
@@ -1126,7 +1172,7 @@ __device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RT
         // TrivialAcceptTriangle over the tile (Core/Rasterizer.h:310-353).
         for (int q = 0; q < 4; q++) {
             const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
-            if (qx <= tmaxX - 1 && qy <= tmaxY - 1) quad_depth(t, qx, qy, false, tminX, tminY, keys);
+            if (qx <= tmaxX - 1 && qy <= tmaxY - 1) quad_depth<kMS>(t, qx, qy, false, tminX, tminY, keys, sp);
         }
     } else {
         // CoarseRasterize (Core/Rasterizer.h:32-111): stepped corner tests per 16x16 quadrant.
@@ -1170,14 +1216,14 @@ __device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RT
             const int qx = tminX + 16 * (q & 1) + lx, qy = tminY + 16 * (q >> 1) + ly;
             if (acc) {
                 if (qx >= bminX && qx <= bmaxX - 1 && qy >= bminY && qy <= bmaxY - 1)
-                    quad_depth(t, qx, qy, false, tminX, tminY, keys);
+                    quad_depth<kMS>(t, qx, qy, false, tminX, tminY, keys, sp);
             } else {
                 int minX = max(bminX, fminX), minY = max(bminY, fminY);
                 const int maxX = min(bmaxX - 1, fmaxX), maxY = min(bmaxY - 1, fmaxY);
                 minX -= minX % 2;
                 minY -= minY % 2;
                 if (qx >= minX && qx <= maxX && qy >= minY && qy <= maxY)
-                    quad_depth(t, qx, qy, true, tminX, tminY, keys);
+                    quad_depth<kMS>(t, qx, qy, true, tminX, tminY, keys, sp);
             }
         }
     }
@@ -1222,6 +1268,7 @@ __device__ __forceinline__ RTri broadcast_rtri(const RTri& t, int l) {
 // key buffer (no workgroup barrier per slice): small triangles take one lane
 // each, medium ones one quad per lane, tile-level ones the whole wave.  The
 // item's keys go to partials[workIndex].
+template <bool kMS>
 __device__ __forceinline__ void raster_item(const BatchParams& Q, const FrameParams* __restrict__ jobs,
                                             const DeviceBuffers& B, uint32_t workIndex, uint64_t* keys,
                                             uint32_t (*sIncl)[kWave]) {
@@ -1232,130 +1279,139 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
     const int tx = (int)(tile % P.tiles_x), ty = (int)(tile / P.tiles_x);
     const int tminX = tx * kTileSize, tminY = ty * kTileSize;
     const int tmaxX = min(tminX + kTileSize, (int)P.width), tmaxY = min(tminY + kTileSize, (int)P.height);
-    for (uint32_t i = threadIdx.x; i < kTileSize * kTileSize; i += kBlock) keys[i] = bg_key();
     const uint32_t start = B.tile_start[gtile];
     const uint32_t n = B.tile_start[gtile + 1] - start;
     const uint32_t segEnd = min(n, (seg + 1) * kSegRefs);
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
-    __syncthreads();
+    // Multisampling keeps kPassSamples samples' keys in LDS and takes the
+    // item's references once per group of samples.
+    const uint32_t S = kMS ? P.samples : 1u;
+    for (uint32_t pass = 0; pass * kPassSamples < S; pass++) {
+        const SamplePass sp{P.sample_level, pass * kPassSamples, min(S - pass * kPassSamples, kPassSamples)};
+        for (uint32_t i = threadIdx.x; i < sp.n * kTileSize * kTileSize; i += kBlock) keys[i] = bg_key();
+        __syncthreads();
 
-    // The next slice's references are loaded before this slice's work.
-    uint32_t payNext = seg * kSegRefs + wave * kWave + lane < segEnd ? B.refs[start + seg * kSegRefs + wave * kWave + lane]
-                                                                      : 0u;
-    for (uint32_t b = seg * kSegRefs + wave * kWave; b < segEnd; b += kBlock) {
-        const uint32_t p = b + lane;
-        const bool valid = p < segEnd;
-        const uint32_t pay = payNext;
-        payNext = p + kBlock < segEnd ? B.refs[start + p + kBlock] : 0u;
-        const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
-        const bool big = (pay >> 31) != 0;
-        // Tile-level paths (RasterizeTile, Core/Renderer.cpp:255-268):
-        // trivially accepted, or big with hierarchical rasterisation on.
-        const bool large = valid && (trivial || (big && P.hierarchical));
-        // Every lane loads its own record; large ones are broadcast below.
-        // References without the big bit below total_slots are compact
-        // unclipped triangles, whose record is rebuilt from their vertices.
-        RTri t;
-        if (valid) {
-            const uint32_t rid = pay & kRecordMask;
-            if (big || rid >= Q.total_slots) load_rtri(B.rrec[rid], t);
-            else screen_rtri(P, B, rid, t);
-        }
-        uint32_t myQ = 0;  // quads of a medium FineRasterize reference
-        int fx = 0, fy = 0, fnx = 1;
-        if (valid && !large) {
-            const int bx0 = min(t.x0, min(t.x1, t.x2)) >> 4, bx1 = max(t.x0, max(t.x1, t.x2)) >> 4;
-            const int by0 = min(t.y0, min(t.y1, t.y2)) >> 4, by1 = max(t.y0, max(t.y1, t.y2)) >> 4;
-            // FineRasterize_SingleSample bbox (Core/Rasterizer.h:132-140).
-            int minX = max(tminX, bx0);
-            const int maxX = min(tmaxX - 1, bx1);
-            int minY = max(tminY, by0);
-            const int maxY = min(tmaxY - 1, by1);
-            if (bx1 - bx0 <= 1 && by1 - by0 <= 1) {
-                // Pixel bbox of at most 2x2: test only its pixels.  The
-                // reference's even-aligned quads add pixels whose centres lie
-                // outside the bbox, hence outside the triangle, and with edges
-                // this short no edge function can wrap.
-                if (maxX >= minX && maxY >= minY) quad_depth(t, minX, minY, true, tminX, tminY, keys, maxX, maxY);
-            } else {
-                minX -= minX % 2;
-                minY -= minY % 2;
-                if (maxX >= minX && maxY >= minY) {
-                    const int nqx = (maxX - minX) / 2 + 1, nqy = (maxY - minY) / 2 + 1;
-                    if (nqx * nqy <= kSmallQuads) {
-                        for (int qy = 0; qy < nqy; qy++)
-                            for (int qx = 0; qx < nqx; qx++)
-                                quad_depth(t, minX + 2 * qx, minY + 2 * qy, true, tminX, tminY, keys);
-                    } else {
-                        myQ = (uint32_t)(nqx * nqy);
-                        fx = minX;
-                        fy = minY;
-                        fnx = nqx;
+        // The next slice's references are loaded before this slice's work.
+        const uint32_t first = seg * kSegRefs + wave * kWave + lane;
+        uint32_t payNext = first < segEnd ? B.refs[start + first] : 0u;
+        for (uint32_t b = seg * kSegRefs + wave * kWave; b < segEnd; b += kBlock) {
+            const uint32_t p = b + lane;
+            const bool valid = p < segEnd;
+            const uint32_t pay = payNext;
+            payNext = p + kBlock < segEnd ? B.refs[start + p + kBlock] : 0u;
+            const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
+            const bool big = (pay >> 31) != 0;
+            // Tile-level paths (RasterizeTile, Core/Renderer.cpp:255-268):
+            // trivially accepted, or big with hierarchical rasterisation on.
+            const bool large = valid && (trivial || (big && P.hierarchical));
+            // Every lane loads its own record; large ones are broadcast below.
+            // References without the big bit below total_slots are compact
+            // unclipped triangles, whose record is rebuilt from their vertices.
+            RTri t;
+            if (valid) {
+                const uint32_t rid = pay & kRecordMask;
+                if (big || rid >= Q.total_slots) load_rtri(B.rrec[rid], t);
+                else screen_rtri(P, B, rid, t);
+            }
+            uint32_t myQ = 0;  // quads of a medium FineRasterize reference
+            int fx = 0, fy = 0, fnx = 1;
+            if (valid && !large) {
+                const int bx0 = min(t.x0, min(t.x1, t.x2)) >> 4, bx1 = max(t.x0, max(t.x1, t.x2)) >> 4;
+                const int by0 = min(t.y0, min(t.y1, t.y2)) >> 4, by1 = max(t.y0, max(t.y1, t.y2)) >> 4;
+                // FineRasterize_SingleSample bbox (Core/Rasterizer.h:132-140).
+                int minX = max(tminX, bx0);
+                const int maxX = min(tmaxX - 1, bx1);
+                int minY = max(tminY, by0);
+                const int maxY = min(tmaxY - 1, by1);
+                if (bx1 - bx0 <= 1 && by1 - by0 <= 1) {
+                    // Pixel bbox of at most 2x2: test only its pixels.  The
+                    // reference's even-aligned quads add pixels whose centres lie
+                    // outside the bbox, hence outside the triangle, and with edges
+                    // this short no edge function can wrap.
+                    if (maxX >= minX && maxY >= minY)
+                        quad_depth<kMS>(t, minX, minY, true, tminX, tminY, keys, maxX, maxY, sp);
+                } else {
+                    minX -= minX % 2;
+                    minY -= minY % 2;
+                    if (maxX >= minX && maxY >= minY) {
+                        const int nqx = (maxX - minX) / 2 + 1, nqy = (maxY - minY) / 2 + 1;
+                        if (nqx * nqy <= kSmallQuads) {
+                            for (int qy = 0; qy < nqy; qy++)
+                                for (int qx = 0; qx < nqx; qx++)
+                                    quad_depth<kMS>(t, minX + 2 * qx, minY + 2 * qy, true, tminX, tminY, keys, sp);
+                        } else {
+                            myQ = (uint32_t)(nqx * nqy);
+                            fx = minX;
+                            fy = minY;
+                            fnx = nqx;
+                        }
                     }
                 }
             }
-        }
-        // Medium references: the wave's FineRasterize quads (bbox clamped to
-        // the tile, Core/Rasterizer.h:126-200) are expanded into (reference,
-        // quad) pairs and tested 64 per pass, one quad per lane; the owner of
-        // pair k is the first lane whose inclusive quad count exceeds k.
-        const uint32_t incl = wave_incl_scan(myQ);
-        const uint32_t totalQ = __builtin_amdgcn_readlane(incl, kWave - 1);
-        if (totalQ) {
-            sIncl[wave][lane] = incl;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (uint32_t base = 0; base < totalQ; base += kWave) {
-                const uint32_t k = base + lane;
-                int lo = 0, hi = kWave - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (sIncl[wave][mid] > k) hi = mid;
-                    else lo = mid + 1;
-                }
-                const int j = k < totalQ ? lo : 0;
-                RTri u;
-                u.x0 = __shfl(t.x0, j); u.y0 = __shfl(t.y0, j);
-                u.x1 = __shfl(t.x1, j); u.y1 = __shfl(t.y1, j);
-                u.x2 = __shfl(t.x2, j); u.y2 = __shfl(t.y2, j);
-                u.B0 = u.y0 - u.y1; u.C0 = u.x1 - u.x0;
-                u.B1 = u.y1 - u.y2; u.C1 = u.x2 - u.x1;
-                u.B2 = u.y2 - u.y0; u.C2 = u.x0 - u.x2;
-                u.pb0 = __shfl(t.pb0, j); u.pb1 = __shfl(t.pb1, j); u.pb2 = __shfl(t.pb2, j);
-                u.inv_det = __shfl(t.inv_det, j);
-                u.z0 = __shfl(t.z0, j); u.z1 = __shfl(t.z1, j); u.z2 = __shfl(t.z2, j);
-                u.order = __shfl(t.order, j);
-                const int ux = __shfl(fx, j), uy = __shfl(fy, j), unx = __shfl(fnx, j);
-                const uint32_t first = __shfl(incl - myQ, j);
-                if (k < totalQ) {
-                    const int kk = (int)(k - first);
-                    quad_depth(u, ux + 2 * (kk % unx), uy + 2 * (kk / unx), true, tminX, tminY, keys);
+            // Medium references: the wave's FineRasterize quads (bbox clamped to
+            // the tile, Core/Rasterizer.h:126-200) are expanded into (reference,
+            // quad) pairs and tested 64 per pass, one quad per lane; the owner of
+            // pair k is the first lane whose inclusive quad count exceeds k.
+            const uint32_t incl = wave_incl_scan(myQ);
+            const uint32_t totalQ = __builtin_amdgcn_readlane(incl, kWave - 1);
+            if (totalQ) {
+                sIncl[wave][lane] = incl;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (uint32_t base = 0; base < totalQ; base += kWave) {
+                    const uint32_t k = base + lane;
+                    int lo = 0, hi = kWave - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (sIncl[wave][mid] > k) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    const int j = k < totalQ ? lo : 0;
+                    RTri u;
+                    u.x0 = __shfl(t.x0, j); u.y0 = __shfl(t.y0, j);
+                    u.x1 = __shfl(t.x1, j); u.y1 = __shfl(t.y1, j);
+                    u.x2 = __shfl(t.x2, j); u.y2 = __shfl(t.y2, j);
+                    u.B0 = u.y0 - u.y1; u.C0 = u.x1 - u.x0;
+                    u.B1 = u.y1 - u.y2; u.C1 = u.x2 - u.x1;
+                    u.B2 = u.y2 - u.y0; u.C2 = u.x0 - u.x2;
+                    u.pb0 = __shfl(t.pb0, j); u.pb1 = __shfl(t.pb1, j); u.pb2 = __shfl(t.pb2, j);
+                    u.inv_det = __shfl(t.inv_det, j);
+                    u.z0 = __shfl(t.z0, j); u.z1 = __shfl(t.z1, j); u.z2 = __shfl(t.z2, j);
+                    u.order = __shfl(t.order, j);
+                    const int ux = __shfl(fx, j), uy = __shfl(fy, j), unx = __shfl(fnx, j);
+                    const uint32_t first = __shfl(incl - myQ, j);
+                    if (k < totalQ) {
+                        const int kk = (int)(k - first);
+                        quad_depth<kMS>(u, ux + 2 * (kk % unx), uy + 2 * (kk / unx), true, tminX, tminY, keys, sp);
+                    }
                 }
             }
+            // Large references: the whole wave takes them one at a time, the
+            // owner lane's record broadcast with readlane (no memory round trip).
+            uint64_t lm = __ballot(large);
+            while (lm) {
+                const int l = __ffsll((unsigned long long)lm) - 1;
+                lm &= lm - 1;
+                const uint32_t ppay = __builtin_amdgcn_readlane(pay, l);
+                raster_large_wave<kMS>(P, broadcast_rtri(t, l), ppay, tminX, tminY, tmaxX, tmaxY, keys, sp);
+            }
         }
-        // Large references: the whole wave takes them one at a time, the
-        // owner lane's record broadcast with readlane (no memory round trip).
-        uint64_t lm = __ballot(large);
-        while (lm) {
-            const int l = __ffsll((unsigned long long)lm) - 1;
-            lm &= lm - 1;
-            const uint32_t ppay = __builtin_amdgcn_readlane(pay, l);
-            raster_large_wave(P, broadcast_rtri(t, l), ppay, tminX, tminY, tmaxX, tmaxY, keys);
-        }
+        __syncthreads();
+        uint64_t* g = B.partials + ((size_t)workIndex * S + sp.base) * (kTileSize * kTileSize);
+        for (uint32_t i = threadIdx.x; i < sp.n * kTileSize * kTileSize; i += kBlock) g[i] = keys[i];
+        __syncthreads();
     }
-    __syncthreads();
-    uint64_t* g = B.partials + (size_t)workIndex * (kTileSize * kTileSize);
-    for (uint32_t i = threadIdx.x; i < kTileSize * kTileSize; i += kBlock) g[i] = keys[i];
 }
 
 }  // namespace
 
 // Persistent raster kernel: workgroups pull (tile, segment) items from the
 // list built by k_bin_tilescan until it is exhausted.
+template <bool kMS>
 __global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                    DeviceBuffers B) {
-    __shared__ uint64_t keys[kTileSize * kTileSize];
+    __shared__ uint64_t keys[(kMS ? kPassSamples : 1) * kTileSize * kTileSize];
     __shared__ uint32_t sIncl[kBlock / kWave][kWave];
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
@@ -1365,7 +1421,7 @@ __global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FramePar
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= nItems) break;
-        raster_item(Q, jobs, B, item, keys, sIncl);
+        raster_item<kMS>(Q, jobs, B, item, keys, sIncl);
         __syncthreads();
     }
 }
@@ -1375,6 +1431,54 @@ __global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FramePar
 // sequential d <= D test is an associative minimum), then perspective-correct
 // interpolation, the pixel shader and one write of colour, depth and
 // triangle ID (Core/Renderer.cpp:272-350).
+namespace {
+// Record and shading-vertex slots of the winner with submission order `order`:
+// clipped pieces from their stored record, unclipped triangles rebuilt from
+// their vertices' screen values (1/det recomputed as setup computed it).
+__device__ __forceinline__ void winner_tri(const FrameParams& P, const DeviceBuffers& B, uint32_t order, RTri& t,
+                                           uint32_t& v0i, uint32_t& v1i, uint32_t& v2i) {
+    const uint32_t tri = order >> 4;  // triangle slot
+    if (order & 1u) {
+        const uint32_t ri = (B.desc[tri] >> 3) + ((order >> 1) & 7u);
+        const ShadeRec s = B.srec[ri];
+        v0i = s.v0;
+        v1i = s.v1;
+        v2i = s.v2;
+        load_rtri(B.rrec[ri], t);
+    } else {
+        const uint32_t* idx = P.mesh.idx + 3 * (size_t)(tri - P.slot_base);
+        v0i = P.vert_base + idx[0];
+        v1i = P.vert_base + idx[1];
+        v2i = P.vert_base + idx[2];
+        const ScreenVert s0 = B.sverts[v0i], s1 = B.sverts[v1i], s2 = B.sverts[v2i];
+        t.x0 = s0.x; t.y0 = s0.y; t.x1 = s1.x; t.y1 = s1.y; t.x2 = s2.x; t.y2 = s2.y;
+        t.B1 = t.y1 - t.y2; t.C1 = t.x2 - t.x1;
+        t.B2 = t.y2 - t.y0; t.C2 = t.x0 - t.x2;
+        t.inv_det = 1.0f / (float)(t.C2 * t.B1 - t.C1 * t.B2);
+        t.z0 = s0.z; t.z1 = s1.z; t.z2 = s2.z;
+    }
+}
+
+// Minimum key of sample s over the tile's segment images (S samples per image).
+__device__ __forceinline__ uint64_t tile_key(const uint64_t* g, uint32_t nseg, uint32_t S, uint32_t s) {
+    constexpr uint32_t kPix = kTileSize * kTileSize;
+    uint64_t key = g[(size_t)s * kPix];
+    uint32_t k = 1;
+    for (; k + 4 <= nseg; k += 4) {  // four independent loads in flight
+        const uint64_t a = g[((size_t)k * S + s) * kPix], b = g[((size_t)(k + 1) * S + s) * kPix];
+        const uint64_t c = g[((size_t)(k + 2) * S + s) * kPix], e = g[((size_t)(k + 3) * S + s) * kPix];
+        const uint64_t ab = a < b ? a : b, ce = c < e ? c : e;
+        const uint64_t m = ab < ce ? ab : ce;
+        key = m < key ? m : key;
+    }
+    for (; k < nseg; k++) {
+        const uint64_t v = g[((size_t)k * S + s) * kPix];
+        key = v < key ? v : key;
+    }
+    return key;
+}
+}  // namespace
+
 __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                  DeviceBuffers B) {
     constexpr uint32_t kPix = kTileSize * kTileSize;
@@ -1387,73 +1491,82 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
     if (x >= (int)P.width || y >= (int)P.height) return;
     const uint32_t start = B.tile_start[gtile];
     const uint32_t n = B.tile_start[gtile + 1] - start;
-    uint64_t key = bg_key();
-    if (n) {
-        const uint32_t nseg = tile_segments(n);
-        const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * kPix + threadIdx.x;
-        key = g[0];
-        uint32_t s = 1;
-        for (; s + 4 <= nseg; s += 4) {  // four independent loads in flight
-            const uint64_t a = g[(size_t)s * kPix], b = g[(size_t)(s + 1) * kPix];
-            const uint64_t c = g[(size_t)(s + 2) * kPix], e = g[(size_t)(s + 3) * kPix];
-            const uint64_t ab = a < b ? a : b, ce = c < e ? c : e;
-            const uint64_t m = ab < ce ? ab : ce;
-            key = m < key ? m : key;
-        }
-        for (; s < nseg; s++) {
-            const uint64_t v = g[(size_t)s * kPix];
-            key = v < key ? v : key;
-        }
-    }
+    const uint32_t nseg = tile_segments(n), S = P.samples;
+    const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * S * kPix + threadIdx.x;
     const size_t o = P.pix_base + (size_t)(P.height - 1 - y) * P.width + x;
-    if (key == bg_key()) {
-        B.color[o] = 0u;
-        B.depth[o] = 1.0f;
-        B.ids[o] = 0xFFFFFFFFu;
-        if (B.color_f32) B.color_f32[3 * o] = B.color_f32[3 * o + 1] = B.color_f32[3 * o + 2] = 0.0f;
+    if (S == 1) {
+        const uint64_t key = n ? tile_key(g, nseg, 1, 0) : bg_key();
+        if (key == bg_key()) {
+            B.color[o] = 0u;
+            B.depth[o] = 1.0f;
+            B.ids[o] = 0xFFFFFFFFu;
+            if (B.color_f32) B.color_f32[3 * o] = B.color_f32[3 * o + 1] = B.color_f32[3 * o + 2] = 0.0f;
+            return;
+        }
+        // The key's low word is the winner's submission order (make_order).
+        const uint32_t order = 0xFFFFFFFEu - (uint32_t)key;
+        RTri t;
+        uint32_t v0i, v1i, v2i;
+        winner_tri(P, B, order, t, v0i, v1i, v2i);
+        float l0, l1, d;
+        bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
+        const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
+        const float3 c = shade_pixel(P, l0, l1, v0, v1, v2);
+        B.color[o] = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
+        if (B.color_f32) {
+            B.color_f32[3 * o] = clamp01(c.x);
+            B.color_f32[3 * o + 1] = clamp01(c.y);
+            B.color_f32[3 * o + 2] = clamp01(c.z);
+        }
+        B.depth[o] = d;
+        B.ids[o] = (order >> 4) - P.slot_base;
         return;
     }
-    // The key's low word is the winner's submission order (make_order).
-    const uint32_t order = 0xFFFFFFFEu - (uint32_t)key;
-    const uint32_t tri = order >> 4;  // triangle slot
-    uint32_t v0i, v1i, v2i, ri;
-    if (order & 1u) {
-        ri = (B.desc[tri] >> 3) + ((order >> 1) & 7u);
-        const ShadeRec s = B.srec[ri];
-        v0i = s.v0;
-        v1i = s.v1;
-        v2i = s.v2;
-    } else {
-        ri = tri;
-        const uint32_t* idx = P.mesh.idx + 3 * (size_t)(tri - P.slot_base);
-        v0i = P.vert_base + idx[0];
-        v1i = P.vert_base + idx[1];
-        v2i = P.vert_base + idx[2];
+    // Multisampling: per-sample winners and depths (the sequential LEQUAL test
+    // per sample, Core/FrameBuffer.cpp:54-68); a winner is shaded once at the
+    // pixel centre (Core/Rasterizer.h:270-285, Core/Renderer.cpp:305-350); the
+    // resolve averages the samples' colours (Core/FrameBuffer.cpp:70-87) as
+    // Color(b) = b / 255 per channel, summed in sample order, times 1 / S, then
+    // FromFloats.
+    const size_t plane = (size_t)P.width * P.height;
+    const int cx = x * 16 + 8, cy = y * 16 + 8;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t cachedOrder = 0xFFFFFFFFu, cachedColor = 0u;
+    for (uint32_t s = 0; s < S; s++) {
+        const uint64_t key = n ? tile_key(g, nseg, S, s) : bg_key();
+        uint32_t rgba = 0u;
+        if (key == bg_key()) {
+            B.depth[o + s * plane] = 1.0f;
+            B.ids[o + s * plane] = 0xFFFFFFFFu;
+        } else {
+            const uint32_t order = 0xFFFFFFFEu - (uint32_t)key;
+            RTri t;
+            uint32_t v0i, v1i, v2i;
+            winner_tri(P, B, order, t, v0i, v1i, v2i);
+            const int2 off = c_sample_offsets[P.sample_level][s];
+            float l0, l1, d;
+            bary_depth(t, cx + off.x, cy + off.y, l0, l1, d);
+            B.depth[o + s * plane] = d;
+            B.ids[o + s * plane] = (order >> 4) - P.slot_base;
+            if (order != cachedOrder) {
+                bary_depth(t, cx, cy, l0, l1, d);
+                const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
+                const float3 c = shade_pixel(P, l0, l1, v0, v1, v2);
+                cachedColor = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
+                cachedOrder = order;
+            }
+            rgba = cachedColor;
+        }
+        for (int ch = 0; ch < 4; ch++) acc[ch] += (float)((rgba >> (8 * ch)) & 255u) / 255.0f;
     }
-    RTri t;
-    if (order & 1u) {
-        load_rtri(B.rrec[ri], t);
-    } else {
-        // An unclipped record is its vertices' screen values and 1/det.
-        const ScreenVert s0 = B.sverts[v0i], s1 = B.sverts[v1i], s2 = B.sverts[v2i];
-        t.x0 = s0.x; t.y0 = s0.y; t.x1 = s1.x; t.y1 = s1.y; t.x2 = s2.x; t.y2 = s2.y;
-        t.B1 = t.y1 - t.y2; t.C1 = t.x2 - t.x1;
-        t.B2 = t.y2 - t.y0; t.C2 = t.x0 - t.x2;
-        t.inv_det = 1.0f / (float)(t.C2 * t.B1 - t.C1 * t.B2);
-        t.z0 = s0.z; t.z1 = s1.z; t.z2 = s2.z;
-    }
-    float l0, l1, d;
-    bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
-    const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
-    const float3 c = shade_pixel(P, l0, l1, v0, v1, v2);
-    B.color[o] = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
+    const float invS = 1.0f / (float)S;
+    B.color[o] = to_byte(acc[0] * invS) | (to_byte(acc[1] * invS) << 8) | (to_byte(acc[2] * invS) << 16) |
+                 (to_byte(acc[3] * invS) << 24);
     if (B.color_f32) {
-        B.color_f32[3 * o] = clamp01(c.x);
-        B.color_f32[3 * o + 1] = clamp01(c.y);
-        B.color_f32[3 * o + 2] = clamp01(c.z);
+        B.color_f32[3 * o] = clamp01(acc[0] * invS);
+        B.color_f32[3 * o + 1] = clamp01(acc[1] * invS);
+        B.color_f32[3 * o + 2] = clamp01(acc[2] * invS);
     }
-    B.depth[o] = d;
-    B.ids[o] = tri - P.slot_base;
 }
 
 // ---------------------------------------------------------------------------
@@ -1499,7 +1612,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     mark(3);
     if (Q.total_chunks) hipLaunchKernelGGL(k_bin_scatter, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
     mark(4);
-    hipLaunchKernelGGL(k_raster, dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
+    if (Q.max_samples > 1) hipLaunchKernelGGL((k_raster<true>), dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
+    else hipLaunchKernelGGL((k_raster<false>), dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
     hipLaunchKernelGGL(k_shade, dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
     mark(5);
     return hipGetLastError();

@@ -179,6 +179,19 @@ class Renderer:
     def set_hierarchical_rasterize(self, on):
         _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_HIERARCHICAL, 1 if on else 0))
 
+    # Renderer::SetMSAAMode (Core/Renderer.cpp:94-98)
+    def set_msaa_mode(self, sample_count_log2):
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_MSAA, int(sample_count_log2)))
+        self.samples = 1 << int(sample_count_log2)
+
+    def read_sample_buffers(self):
+        """(depth, ids), each (samples, H, W), rows as read_depth."""
+        S = getattr(self, "samples", 1)
+        depth = np.empty((S, self.height, self.width), np.float32)
+        ids = np.empty((S, self.height, self.width), np.uint32)
+        _check(self._ctx, L.lib().edxr_read_sample_buffers(self._ctx, depth.ctypes.data, ids.ctypes.data, depth.size))
+        return depth, ids
+
     def set_bin_chunks(self, n):
         _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_BIN_CHUNKS, int(n)))
 

@@ -79,7 +79,12 @@ enum edxr_state_key {
     EDXR_STATE_RASTER_BLOCKS_PER_CU = 6,
     /* 1 = also write the clamped float RGB image (3 floats per pixel, the
      * function edxr_backward differentiates); default 0. */
-    EDXR_STATE_FLOAT_COLOR = 7
+    EDXR_STATE_FLOAT_COLOR = 7,
+    /* Renderer::SetMSAAMode (Core/Renderer.cpp:94-98): log2 of the samples per
+     * pixel, 0 (1 sample) .. 5 (32 samples); default 0.  Sample positions are
+     * the reference's table (Core/FrameBuffer.cpp:107-191); the colour buffer
+     * is resolved, depth and IDs are kept per sample. */
+    EDXR_STATE_MSAA = 8
 };
 
 /* Pipeline stages timed when EDXR_STATE_PROFILE is on. */
@@ -184,6 +189,11 @@ int edxr_read_color(edxr_ctx* ctx, uint8_t* rgba, size_t bytes);
 int edxr_read_depth(edxr_ctx* ctx, float* depth, size_t count);
 /* Winning input-triangle index per pixel (0xFFFFFFFF = none), same layout. */
 int edxr_read_triangle_ids(edxr_ctx* ctx, uint32_t* ids, size_t count);
+/* Per-sample depth and winning triangle IDs of the last single frame,
+ * [sample][row][x] with rows as edxr_read_depth; count >= samples * W * H.
+ * Either pointer may be NULL.  Sample 0 is what edxr_read_depth and
+ * edxr_read_triangle_ids return. */
+int edxr_read_sample_buffers(edxr_ctx* ctx, float* depth, uint32_t* ids, size_t count);
 /* Device addresses of the three frame buffers (valid until resize/destroy). */
 int edxr_device_buffers(edxr_ctx* ctx, void** color, void** depth, void** ids);
 int edxr_get_stats(edxr_ctx* ctx, edxr_stats* out);

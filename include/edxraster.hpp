@@ -164,10 +164,8 @@ class Renderer {
         check(edxr_set_pixel_shader(ctx_, ps.Kind(), &m));
     }
     void SetHierarchicalRasterize(bool on) { check(edxr_set_state(ctx_, EDXR_STATE_HIERARCHICAL, on ? 1 : 0)); }
-    /* Single-sample only: any other level is rejected. */
-    void SetMSAAMode(int sampleCountLog2) {
-        if (sampleCountLog2 != 0) throw std::runtime_error("edxraster: MSAA is not implemented");
-    }
+    /* Renderer::SetMSAAMode (Core/Renderer.cpp:94-98): 0 (1 sample) .. 5 (32). */
+    void SetMSAAMode(int sampleCountLog2) { check(edxr_set_state(ctx_, EDXR_STATE_MSAA, sampleCountLog2)); }
     void RenderMesh(const Mesh& mesh) {
         const edxr_mesh v = mesh.View();
         check(edxr_render_mesh(ctx_, &v));
