@@ -431,6 +431,8 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
     const uint32_t refWant = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * Q.total_tiles + 1024, 0xFFFFFFF0ull);
     c->refCap = std::max(c->refCap, refWant);
 
+    // Persistent raster grid: enough resident workgroups per CU to hide latency.
+    const uint32_t rasterBlocks = c->numCUs * (c->rasterPerCU ? c->rasterPerCU : 8);
     c->stats = edxr_stats{};
     for (uint32_t attempt = 0; attempt < 8; attempt++) {
         if ((uint64_t)T + c->extraRecCap > kRecordMask) return c->fail(EDXR_E_LIMIT, "record capacity exceeded");
@@ -442,7 +444,8 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
             (e = c->clipQueue.ensure(c->clipQueueCap)) != hipSuccess ||
             (e = c->binCounts.ensure(std::max<uint64_t>(pl.countEntries, 1))) != hipSuccess ||
             (e = c->refs.ensure(c->refCap)) != hipSuccess ||
-            (e = c->work.ensure((size_t)Q.total_tiles + c->refCap / kSegRefs + 2)) != hipSuccess ||
+            (e = c->work.ensure((size_t)Q.total_tiles + std::max(rasterBlocks, c->refCap / kSegRefs) + 2)) !=
+                hipSuccess ||
             (e = c->partials.ensure(c->work.cap * kTileSize * kTileSize * Q.max_samples)) != hipSuccess ||
             (e = c->verts.ensure((size_t)V + c->extraVertCap)) != hipSuccess ||
             (e = c->sverts.ensure(std::max<uint32_t>(V, 1))) != hipSuccess)
@@ -469,8 +472,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.work_cap = (uint32_t)c->work.cap;
         B.tile_item = c->tileItem.p;
         B.partials = c->partials.p;
-        // Persistent raster grid: enough resident workgroups per CU to hide latency.
-        B.raster_blocks = c->numCUs * (c->rasterPerCU ? c->rasterPerCU : 8);
+        B.raster_blocks = rasterBlocks;
         B.counters = c->counters.p;
         B.color = color;
         B.depth = depth;

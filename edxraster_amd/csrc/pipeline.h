@@ -18,7 +18,9 @@ constexpr uint32_t kRecordMask = (1u << kRecordBits) - 1;
 constexpr int kMaxPoly = 16;                 // clip polygon capacity (convex input needs <= 9)
 constexpr uint32_t kMaxSampleLevel = 5;      // up to 32 samples per pixel
 constexpr uint32_t kPassSamples = 4;         // samples whose keys one raster pass keeps in LDS
-constexpr uint32_t kSegRefs = 2048;          // tile references per raster work item
+constexpr uint32_t kSegLog2Min = 8;          // raster work items hold 256 .. 2048 tile references,
+constexpr uint32_t kSegLog2Max = 11;         // chosen per frame from the total (k_bin_tilescan)
+constexpr uint32_t kSegRefs = 1u << kSegLog2Max;
 constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
 constexpr uint32_t kJobAlign = 1024;         // job slot alignment: vertex/setup blocks never span jobs
@@ -167,6 +169,7 @@ enum CounterSlot {
     kCntTotalRefs,
     kCntWorkItems,       // raster work items (tile segments)
     kCntWorkNext,        // next work item to claim
+    kCntSegLog2,         // log2 of the tile references per work item of this frame
     kCntStriped = 16,    // high-rate counters: kStripes slots each, summed on the host
     kCntSlots = kCntStriped + 2 * 32
 };

@@ -806,35 +806,51 @@ __device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* wsum, uint32_
 }
 
 // Raster work items of a tile with n references (an empty tile has none).
-__device__ __forceinline__ uint32_t tile_segments(uint32_t n) { return (n + kSegRefs - 1) / kSegRefs; }
+__device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) {
+    return (n + (1u << segLog2) - 1) >> segLog2;
+}
 }  // namespace
 
 // Exclusive scan of the tile totals over every job's tiles (global tiles <=
 // 2^18, one block of 1024), and the raster work list: one item per (tile,
-// segment of <= kSegRefs refs).
+// segment of <= 2^segLog2 refs).  The segment size is the smallest of 256 ..
+// 2048 that gives no more items than the raster grid has workgroups (beyond
+// one per tile), so a frame with few references is still spread over the
+// whole grid instead of being bound by one workgroup walking a deep tile.
 __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuffers B) {
     __shared__ uint32_t wsum[16];
+    __shared__ uint32_t sMax;
     const uint32_t nt = Q.total_tiles;
     const uint32_t per = (nt + 1023) / 1024;
     const uint32_t t0 = min(nt, threadIdx.x * per), t1 = min(nt, t0 + per);
-    uint32_t localRefs = 0, localSegs = 0;
-    bool segsFit = true;
+    if (threadIdx.x == 0) sMax = 0;
+    __syncthreads();
+    uint32_t localRefs = 0, localMax = 0;
     for (uint32_t t = t0; t < t1; t++) {
         const uint32_t n = B.tile_totals[t];
         localRefs += n;
-        localSegs += tile_segments(n);
-        segsFit &= tile_segments(n) <= (1u << (32 - kTileBits));
+        localMax = max(localMax, n);
     }
-    uint32_t total, totalSegs;
-    uint32_t run = scan1024(localRefs, wsum, total);
+    atomicMax(&sMax, localMax);
+    uint32_t total;
+    uint32_t run = scan1024(localRefs, wsum, total);  // ends with a barrier: sMax is final
+    const uint32_t maxN = sMax;
+    uint32_t segLog2 = kSegLog2Min;
+    while (segLog2 < kSegLog2Max && (total >> segLog2) > B.raster_blocks) segLog2++;
+    // a tile's segment index must fit above the tile bits of a work item
+    while (segLog2 < kSegLog2Max && tile_segments(maxN, segLog2) > (1u << (32 - kTileBits))) segLog2++;
+    const bool segsFit = tile_segments(maxN, segLog2) <= (1u << (32 - kTileBits));
+    uint32_t localSegs = 0;
+    for (uint32_t t = t0; t < t1; t++) localSegs += tile_segments(B.tile_totals[t], segLog2);
+    uint32_t totalSegs;
     uint32_t seg = scan1024(localSegs, wsum, totalSegs);
-    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && __syncthreads_and(segsFit);
+    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && segsFit;
     for (uint32_t t = t0; t < t1; t++) {
         const uint32_t n = B.tile_totals[t];
         B.tile_start[t] = run;
         run += n;
         if (fits) {
-            const uint32_t ns = tile_segments(n);
+            const uint32_t ns = tile_segments(n, segLog2);
             B.tile_item[t] = seg;
             for (uint32_t s = 0; s < ns; s++) B.work[seg + s] = t | (s << kTileBits);
             seg += ns;
@@ -844,6 +860,7 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
         B.tile_start[nt] = total;
         B.counters[kCntTotalRefs] = total;
         B.counters[kCntWorkItems] = totalSegs;
+        B.counters[kCntSegLog2] = segLog2;
         if (!fits) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvRefs);
     }
 }
@@ -941,7 +958,12 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int kSmallQuads = 4;
+// Medium references with at least this many quads are walked by the whole
+// wave (the triangle in scalar registers, 64 quads per pass); fewer are packed
+// with other references' quads into shared passes.
+constexpr uint32_t kWaveQuads = 40;
+// Packed quads of one wave's slice: at most 64 references of < kWaveQuads.
+constexpr uint32_t kOwnerCap = kWave * (kWaveQuads - 1);
 
 struct RTri {
     int x0, y0, x1, y1, x2, y2;
@@ -1263,15 +1285,15 @@ __device__ __forceinline__ RTri broadcast_rtri(const RTri& t, int l) {
     return u;
 }
 
-// One raster work item: references [seg * kSegRefs, (seg + 1) * kSegRefs) of a
-// tile.  Each wave streams its own 64-reference slices through the tile's LDS
+// One raster work item: references [seg << segLog2, (seg + 1) << segLog2) of
+// a tile.  Each wave streams its own 64-reference slices through the tile's LDS
 // key buffer (no workgroup barrier per slice): small triangles take one lane
 // each, medium ones one quad per lane, tile-level ones the whole wave.  The
 // item's keys go to partials[workIndex].
 template <bool kMS>
 __device__ __forceinline__ void raster_item(const BatchParams& Q, const FrameParams* __restrict__ jobs,
-                                            const DeviceBuffers& B, uint32_t workIndex, uint64_t* keys,
-                                            uint32_t (*sIncl)[kWave]) {
+                                            const DeviceBuffers& B, uint32_t workIndex, uint32_t segLog2,
+                                            uint64_t* keys, uint16_t (*sOwner)[kOwnerCap]) {
     const uint32_t workItem = __builtin_amdgcn_readfirstlane(B.work[workIndex]);
     const uint32_t gtile = workItem & (kMaxBatchTiles - 1), seg = workItem >> kTileBits;
     const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
@@ -1281,7 +1303,7 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
     const int tmaxX = min(tminX + kTileSize, (int)P.width), tmaxY = min(tminY + kTileSize, (int)P.height);
     const uint32_t start = B.tile_start[gtile];
     const uint32_t n = B.tile_start[gtile + 1] - start;
-    const uint32_t segEnd = min(n, (seg + 1) * kSegRefs);
+    const uint32_t segBegin = seg << segLog2, segEnd = min(n, (seg + 1) << segLog2);
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     // Multisampling keeps kPassSamples samples' keys in LDS and takes the
     // item's references once per group of samples.
@@ -1291,14 +1313,19 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
         for (uint32_t i = threadIdx.x; i < sp.n * kTileSize * kTileSize; i += kBlock) keys[i] = bg_key();
         __syncthreads();
 
-        // The next slice's references are loaded before this slice's work.
-        const uint32_t first = seg * kSegRefs + wave * kWave + lane;
-        uint32_t payNext = first < segEnd ? B.refs[start + first] : 0u;
-        for (uint32_t b = seg * kSegRefs + wave * kWave; b < segEnd; b += kBlock) {
+        // Each wave takes slices of L references (64, or a quarter of a short
+        // segment so that every wave has work); the next slice's references
+        // are loaded before this slice's work.
+        const uint32_t segLen = segEnd - segBegin;
+        const uint32_t L = segLen >= kBlock ? kWave : (segLen + kBlock / kWave - 1) / (kBlock / kWave);
+        const uint32_t stride = L * (kBlock / kWave);
+        const uint32_t first = segBegin + wave * L + lane;
+        uint32_t payNext = lane < L && first < segEnd ? B.refs[start + first] : 0u;
+        for (uint32_t b = segBegin + wave * L; b < segEnd; b += stride) {
             const uint32_t p = b + lane;
-            const bool valid = p < segEnd;
+            const bool valid = lane < L && p < segEnd;
             const uint32_t pay = payNext;
-            payNext = p + kBlock < segEnd ? B.refs[start + p + kBlock] : 0u;
+            payNext = lane < L && p + stride < segEnd ? B.refs[start + p + stride] : 0u;
             const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
             const bool big = (pay >> 31) != 0;
             // Tile-level paths (RasterizeTile, Core/Renderer.cpp:255-268):
@@ -1313,7 +1340,7 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
                 if (big || rid >= Q.total_slots) load_rtri(B.rrec[rid], t);
                 else screen_rtri(P, B, rid, t);
             }
-            uint32_t myQ = 0;  // quads of a medium FineRasterize reference
+            uint32_t myQ = 0;  // quads of a FineRasterize reference beyond the tiny path
             int fx = 0, fy = 0, fnx = 1;
             if (valid && !large) {
                 const int bx0 = min(t.x0, min(t.x1, t.x2)) >> 4, bx1 = max(t.x0, max(t.x1, t.x2)) >> 4;
@@ -1335,39 +1362,65 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
                     minY -= minY % 2;
                     if (maxX >= minX && maxY >= minY) {
                         const int nqx = (maxX - minX) / 2 + 1, nqy = (maxY - minY) / 2 + 1;
-                        if (nqx * nqy <= kSmallQuads) {
-                            for (int qy = 0; qy < nqy; qy++)
-                                for (int qx = 0; qx < nqx; qx++)
-                                    quad_depth<kMS>(t, minX + 2 * qx, minY + 2 * qy, true, tminX, tminY, keys, sp);
-                        } else {
-                            myQ = (uint32_t)(nqx * nqy);
-                            fx = minX;
-                            fy = minY;
-                            fnx = nqx;
-                        }
+                        myQ = (uint32_t)(nqx * nqy);
+                        fx = minX;
+                        fy = minY;
+                        fnx = nqx;
                     }
                 }
             }
-            // Medium references: the wave's FineRasterize quads (bbox clamped to
-            // the tile, Core/Rasterizer.h:126-200) are expanded into (reference,
-            // quad) pairs and tested 64 per pass, one quad per lane; the owner of
-            // pair k is the first lane whose inclusive quad count exceeds k.
+            // Medium references of kWaveQuads quads or more: the whole wave walks
+            // each one's FineRasterize quads (bbox clamped to the tile,
+            // Core/Rasterizer.h:126-200), 64 per pass.
+            uint64_t wm = __ballot(myQ >= kWaveQuads);
+            while (wm) {
+                const int l = __ffsll((unsigned long long)wm) - 1;
+                wm &= wm - 1;
+                const RTri u = broadcast_rtri(t, l);
+                const int ux = __builtin_amdgcn_readlane(fx, l), uy = __builtin_amdgcn_readlane(fy, l);
+                const uint32_t unx = __builtin_amdgcn_readlane((uint32_t)fnx, l);
+                const uint32_t uq = __builtin_amdgcn_readlane(myQ, l);
+                // quad k = lane + 64 i at (k % unx, k / unx), stepped per pass
+                uint32_t cx = lane % unx, cy = lane / unx;
+                const uint32_t dx = kWave % unx, dy = kWave / unx;
+                for (uint32_t k = lane; k < uq; k += kWave) {
+                    quad_depth<kMS>(u, ux + 2 * (int)cx, uy + 2 * (int)cy, true, tminX, tminY, keys, sp);
+                    cx += dx;
+                    cy += dy;
+                    if (cx >= unx) {
+                        cx -= unx;
+                        cy++;
+                    }
+                }
+            }
+            if (myQ >= kWaveQuads) myQ = 0;
+            // Other references with quads: the wave's quads are expanded into
+            // (reference, quad) pairs and tested 64 per pass, one quad per lane.
+            // Pair k's entry in the wave's owner table holds the owner lane and
+            // the quad's column and row in the owner's bbox.
             const uint32_t incl = wave_incl_scan(myQ);
             const uint32_t totalQ = __builtin_amdgcn_readlane(incl, kWave - 1);
             if (totalQ) {
-                sIncl[wave][lane] = incl;
+                {
+                    uint16_t* own = &sOwner[wave][incl - myQ];
+                    uint32_t cx = 0, cy = 0;
+                    for (uint32_t i = 0; __ballot(i < myQ); i++) {
+                        if (i < myQ) {
+                            own[i] = (uint16_t)(lane | (cx << 6) | (cy << 10));
+                            if (++cx == (uint32_t)fnx) {
+                                cx = 0;
+                                cy++;
+                            }
+                        }
+                    }
+                }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (uint32_t base = 0; base < totalQ; base += kWave) {
                     const uint32_t k = base + lane;
-                    int lo = 0, hi = kWave - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (sIncl[wave][mid] > k) hi = mid;
-                        else lo = mid + 1;
-                    }
-                    const int j = k < totalQ ? lo : 0;
+                    const uint32_t e = k < totalQ ? sOwner[wave][k] : 0u;
+                    const int j = (int)(e & 63u);
                     RTri u;
                     u.x0 = __shfl(t.x0, j); u.y0 = __shfl(t.y0, j);
                     u.x1 = __shfl(t.x1, j); u.y1 = __shfl(t.y1, j);
@@ -1379,12 +1432,10 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
                     u.inv_det = __shfl(t.inv_det, j);
                     u.z0 = __shfl(t.z0, j); u.z1 = __shfl(t.z1, j); u.z2 = __shfl(t.z2, j);
                     u.order = __shfl(t.order, j);
-                    const int ux = __shfl(fx, j), uy = __shfl(fy, j), unx = __shfl(fnx, j);
-                    const uint32_t first = __shfl(incl - myQ, j);
-                    if (k < totalQ) {
-                        const int kk = (int)(k - first);
-                        quad_depth<kMS>(u, ux + 2 * (kk % unx), uy + 2 * (kk / unx), true, tminX, tminY, keys, sp);
-                    }
+                    const int ux = __shfl(fx, j), uy = __shfl(fy, j);
+                    if (k < totalQ)
+                        quad_depth<kMS>(u, ux + 2 * (int)((e >> 6) & 15u), uy + 2 * (int)(e >> 10), true, tminX,
+                                        tminY, keys, sp);
                 }
             }
             // Large references: the whole wave takes them one at a time, the
@@ -1412,16 +1463,16 @@ template <bool kMS>
 __global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                    DeviceBuffers B) {
     __shared__ uint64_t keys[(kMS ? kPassSamples : 1) * kTileSize * kTileSize];
-    __shared__ uint32_t sIncl[kBlock / kWave][kWave];
+    __shared__ uint16_t sOwner[kBlock / kWave][kOwnerCap];
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
-    const uint32_t nItems = B.counters[kCntWorkItems];
+    const uint32_t nItems = B.counters[kCntWorkItems], segLog2 = B.counters[kCntSegLog2];
     for (;;) {
         if (threadIdx.x == 0) sItem = atomicAdd(&B.counters[kCntWorkNext], 1u);
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= nItems) break;
-        raster_item<kMS>(Q, jobs, B, item, keys, sIncl);
+        raster_item<kMS>(Q, jobs, B, item, segLog2, keys, sOwner);
         __syncthreads();
     }
 }
@@ -1491,7 +1542,7 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
     if (x >= (int)P.width || y >= (int)P.height) return;
     const uint32_t start = B.tile_start[gtile];
     const uint32_t n = B.tile_start[gtile + 1] - start;
-    const uint32_t nseg = tile_segments(n), S = P.samples;
+    const uint32_t nseg = tile_segments(n, B.counters[kCntSegLog2]), S = P.samples;
     const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * S * kPix + threadIdx.x;
     const size_t o = P.pix_base + (size_t)(P.height - 1 - y) * P.width + x;
     if (S == 1) {
