@@ -16,6 +16,8 @@ EDXR_E_NOMEM = -3
 EDXR_E_INDEX = -4
 EDXR_E_LIMIT = -5
 EDXR_E_STATE = -6
+EDXR_E_IO = -7
+EDXR_E_FORMAT = -8
 
 SHADER_LAMBERT_ALBEDO = 0
 SHADER_BLINN_PHONG_REF = 1
@@ -29,6 +31,7 @@ STATE_TRIS_PER_CHUNK = 5
 STATE_RASTER_BLOCKS_PER_CU = 6
 STATE_FLOAT_COLOR = 7
 STATE_MSAA = 8
+STATE_WRITE_FRAMES = 9
 STAGES = ("setup", "bin_count", "bin_scan", "bin_scatter", "raster")
 
 MEM_HOST = 0
@@ -114,6 +117,11 @@ SIGNATURES = [
     ("edxr_scene_sphere_grid", C.c_int, [C.POINTER(HostMesh), C.c_uint32, C.c_uint32, C.c_uint64]),
     ("edxr_scene_merge", C.c_int, [C.POINTER(HostMesh), C.POINTER(HostMesh)]),
     ("edxr_scene_free", None, [C.POINTER(HostMesh)]),
+    ("edxr_load_obj", C.c_int, [C.c_char_p, F, F, F, C.POINTER(HostMesh)]),
+    ("edxr_write_bmp", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("edxr_set_frame_directory", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("edxr_write_frame_to_file", C.c_int, [C.c_void_p]),
+    ("edxr_frame_count", C.c_int64, [C.c_void_p]),
     ("edxr_look_at", None, [F, F, F, F]),
     ("edxr_perspective", None, [C.c_float, C.c_float, C.c_float, C.c_float, F]),
     ("edxr_raster_matrix", None, [C.c_uint32, C.c_uint32, F]),

@@ -8,6 +8,8 @@
 // counters; the frame is then re-run with larger buffers.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -162,6 +164,9 @@ struct edxr_ctx {
     int32_t useGraph = 0;
     int32_t floatColor = 0;
     int32_t msaaLog2 = 0;
+    int32_t writeFrames = 0;      // Renderer::SetWriteFrames
+    int64_t frameCount = 0;       // RenderStates::FrameCount
+    std::string frameDir = ".";   // Application::GetBaseDirectory
     hipEvent_t events[kNumStages + 1] = {};
     double stageMs[kNumStages] = {};
     uint64_t profiledFrames = 0;
@@ -738,6 +743,7 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
             c->trisPerChunk = (uint32_t)value;
             return EDXR_OK;
         case EDXR_STATE_FLOAT_COLOR: c->floatColor = value ? 1 : 0; return EDXR_OK;
+        case EDXR_STATE_WRITE_FRAMES: c->writeFrames = value ? 1 : 0; return EDXR_OK;
         case EDXR_STATE_MSAA: {
             if (value < 0 || value > (int32_t)kMaxSampleLevel)
                 return c->fail(EDXR_E_INVALID, "MSAA level out of range (0..5)");
@@ -816,7 +822,34 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     c->lastV = V;
     c->lastChunks = pl.Q.total_chunks;
     c->lastMesh = M;
+    // RenderMesh's epilogue (Core/Renderer.cpp:114-117).
+    if (c->writeFrames && (r = edxr_write_frame_to_file(c)) != EDXR_OK) return r;
+    c->frameCount++;
     return EDXR_OK;
+}
+
+int edxr_set_frame_directory(edxr_ctx* c, const char* dir) {
+    if (!c || !dir || !dir[0]) return EDXR_E_INVALID;
+    c->frameDir = dir;
+    return EDXR_OK;
+}
+
+int64_t edxr_frame_count(edxr_ctx* c) { return c ? c->frameCount : EDXR_E_INVALID; }
+
+int edxr_write_frame_to_file(edxr_ctx* c) {
+    if (!c) return EDXR_E_INVALID;
+    if (!c->rendered) return c->fail(EDXR_E_STATE, "nothing rendered");
+    const std::string dir = c->frameDir + "/Frames";
+    struct stat st;
+    if (stat(dir.c_str(), &st) != 0 && mkdir(dir.c_str(), 0755) != 0)
+        return c->fail(EDXR_E_IO, "cannot create the Frames directory");
+    char name[32];
+    snprintf(name, sizeof(name), "/Frame%05lld.bmp", (long long)c->frameCount);
+    std::vector<uint8_t> rgba(4ull * c->W * c->H);
+    int r = edxr_read_color(c, rgba.data(), rgba.size());
+    if (r != EDXR_OK) return r;
+    r = edxr_write_bmp((dir + name).c_str(), rgba.data(), c->W, c->H);
+    return r == EDXR_OK ? r : c->fail(r, "cannot write the frame bitmap");
 }
 
 int edxr_render_batch(edxr_ctx* c, const edxr_job* jobs, uint32_t n) {

@@ -8,8 +8,10 @@
 // RasterTriangle::Setup, EDXRaster/Core/RasterTriangle.h:49-51).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -252,3 +254,308 @@ void edxr_raster_matrix(uint32_t width, uint32_t height, float out[16]) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Wavefront OBJ input: Mesh::LoadMesh (EDXRaster/Utils/Mesh.cpp:11-34) through
+// EDXUtil's ObjMesh::LoadFromObj, which is not available.  Declared semantics:
+//   - 'v', 'vn', 'vt', 'f' (v, v/t, v//n, v/t/n; 1-based or negative
+//     relative indices), 'mtllib' and 'usemtl'; the .mtl file supplies
+//     'newmtl' and 'Kd' (the material colour, ConstantTexture2D in the
+//     reference); other statements are skipped;
+//   - a polygon face is split into a fan (0, i, i + 1), keeping the file's
+//     vertex order (front faces of a right-handed export are clockwise on
+//     screen in the left-handed, y-down raster space);
+//   - one output vertex per distinct (position, normal, material) triple;
+//     faces without normals get area-weighted vertex normals of their
+//     position (computed in double);
+//   - world transform p' = T(pos) Rz Ry Rx S(scl) p (rotation in degrees),
+//     normals n' = normalize(Rz Ry Rx (n / scl));
+//   - vertex colour = the face material's Kd, (0.9, 0.9, 0.9) without one
+//     (the constant albedo Mesh::LoadPlane gives, Utils/Mesh.cpp:46).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ObjMaterial {
+    std::string name;
+    float kd[3] = {0.9f, 0.9f, 0.9f};
+};
+
+bool read_file(const std::string& path, std::string& out) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof(buf), f)) > 0) out.append(buf, n);
+    fclose(f);
+    return true;
+}
+
+std::string dir_of(const std::string& path) {
+    const size_t s = path.find_last_of("/\\");
+    return s == std::string::npos ? std::string() : path.substr(0, s + 1);
+}
+
+// Splits text into lines with leading blanks removed and comments cut.
+template <class F>
+void for_each_line(const std::string& text, F f) {
+    size_t i = 0;
+    while (i < text.size()) {
+        size_t e = text.find('\n', i);
+        if (e == std::string::npos) e = text.size();
+        std::string line = text.substr(i, e - i);
+        i = e + 1;
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        while (!line.empty() && (line.back() == '\r' || line.back() == ' ' || line.back() == '\t')) line.pop_back();
+        size_t b = 0;
+        while (b < line.size() && (line[b] == ' ' || line[b] == '\t')) b++;
+        if (b < line.size()) f(line.substr(b));
+    }
+}
+
+void load_mtl(const std::string& path, std::vector<ObjMaterial>& mats) {
+    std::string text;
+    if (!read_file(path, text)) return;  // a missing library leaves the default colour
+    for_each_line(text, [&](const std::string& l) {
+        char key[64] = {0}, name[512] = {0};
+        if (sscanf(l.c_str(), "%63s", key) != 1) return;
+        if (!strcmp(key, "newmtl") && sscanf(l.c_str(), "%*s %511s", name) == 1) {
+            mats.push_back(ObjMaterial());
+            mats.back().name = name;
+        } else if (!strcmp(key, "Kd") && !mats.empty()) {
+            float r, g, b;
+            if (sscanf(l.c_str(), "%*s %f %f %f", &r, &g, &b) == 3) {
+                mats.back().kd[0] = r;
+                mats.back().kd[1] = g;
+                mats.back().kd[2] = b;
+            }
+        }
+    });
+}
+
+// Resolves an OBJ index (1-based, or negative = relative to the end).
+bool obj_index(long v, size_t count, uint32_t& out) {
+    if (v > 0 && (size_t)v <= count) {
+        out = (uint32_t)(v - 1);
+        return true;
+    }
+    if (v < 0 && (size_t)(-v) <= count) {
+        out = (uint32_t)(count + v);
+        return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+int edxr_load_obj(const char* path, const float pos[3], const float scl[3], const float rot[3],
+                  edxr_host_mesh* out) {
+    if (!path || !out) return EDXR_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    std::string text;
+    if (!read_file(path, text)) return EDXR_E_IO;
+    std::vector<double> P, N;  // file positions and normals, xyz each
+    std::vector<ObjMaterial> mats;
+    struct Corner {
+        uint32_t p, n, m;  // n = UINT32_MAX: none
+    };
+    std::vector<Corner> corners;  // triangle corners, 3 per triangle
+    uint32_t cur = UINT32_MAX;    // current material (UINT32_MAX: default)
+    bool bad = false;
+    const std::string dir = dir_of(path);
+    for_each_line(text, [&](const std::string& l) {
+        if (bad) return;
+        char key[64] = {0};
+        if (sscanf(l.c_str(), "%63s", key) != 1) return;
+        if (!strcmp(key, "v") || !strcmp(key, "vn")) {
+            double x, y, z;
+            if (sscanf(l.c_str(), "%*s %lf %lf %lf", &x, &y, &z) != 3) {
+                bad = true;
+                return;
+            }
+            std::vector<double>& d = key[1] ? N : P;
+            d.push_back(x);
+            d.push_back(y);
+            d.push_back(z);
+        } else if (!strcmp(key, "mtllib")) {
+            char name[512] = {0};
+            if (sscanf(l.c_str(), "%*s %511s", name) == 1) load_mtl(dir + name, mats);
+        } else if (!strcmp(key, "usemtl")) {
+            char name[512] = {0};
+            cur = UINT32_MAX;
+            if (sscanf(l.c_str(), "%*s %511s", name) == 1)
+                for (size_t i = 0; i < mats.size(); i++)
+                    if (mats[i].name == name) cur = (uint32_t)i;
+        } else if (!strcmp(key, "f")) {
+            std::vector<Corner> poly;
+            const char* s = l.c_str() + 1;
+            while (*s) {
+                while (*s == ' ' || *s == '\t') s++;
+                if (!*s) break;
+                char* e;
+                const long vi = strtol(s, &e, 10);
+                if (e == s) {
+                    bad = true;
+                    return;
+                }
+                s = e;
+                long ni = 0;
+                if (*s == '/') {
+                    s++;
+                    if (*s != '/') {
+                        strtol(s, &e, 10);  // texture coordinate (unused)
+                        s = e;
+                    }
+                    if (*s == '/') {
+                        s++;
+                        ni = strtol(s, &e, 10);
+                        if (e == s) {
+                            bad = true;
+                            return;
+                        }
+                        s = e;
+                    }
+                }
+                Corner c{0, UINT32_MAX, cur};
+                if (!obj_index(vi, P.size() / 3, c.p) || (ni && !obj_index(ni, N.size() / 3, c.n))) {
+                    bad = true;
+                    return;
+                }
+                poly.push_back(c);
+            }
+            if (poly.size() < 3) {
+                bad = true;
+                return;
+            }
+            for (size_t i = 1; i + 1 < poly.size(); i++) {
+                corners.push_back(poly[0]);
+                corners.push_back(poly[i]);
+                corners.push_back(poly[i + 1]);
+            }
+        }
+    });
+    if (bad) return EDXR_E_FORMAT;
+
+    // Area-weighted vertex normals for corners without one.
+    std::vector<double> AN(P.size(), 0.0);
+    for (size_t t = 0; t + 2 < corners.size(); t += 3) {
+        const double* a = &P[3 * corners[t].p];
+        const double* b = &P[3 * corners[t + 1].p];
+        const double* c = &P[3 * corners[t + 2].p];
+        const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        for (int k = 0; k < 3; k++)
+            for (int j = 0; j < 3; j++) AN[3 * corners[t + k].p + j] += n[j];
+    }
+
+    // Distinct (position, normal, material) triples become output vertices,
+    // numbered by first appearance in the file.
+    std::vector<uint32_t> vid(corners.size());
+    std::vector<uint32_t> first;  // corner index of each output vertex
+    {
+        std::vector<uint32_t> order(corners.size());
+        for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            const Corner &x = corners[a], &y = corners[b];
+            if (x.p != y.p) return x.p < y.p;
+            if (x.n != y.n) return x.n < y.n;
+            return x.m < y.m;
+        });
+        std::vector<uint32_t> rep(corners.size());
+        for (size_t k = 0; k < order.size(); k++) {
+            const Corner& c = corners[order[k]];
+            const bool same = k && corners[order[k - 1]].p == c.p && corners[order[k - 1]].n == c.n &&
+                              corners[order[k - 1]].m == c.m;
+            rep[order[k]] = same ? rep[order[k - 1]] : order[k];
+        }
+        std::vector<uint32_t> idOf(corners.size(), UINT32_MAX);
+        for (size_t i = 0; i < corners.size(); i++) {
+            if (idOf[rep[i]] == UINT32_MAX) {
+                idOf[rep[i]] = (uint32_t)first.size();
+                first.push_back((uint32_t)i);
+            }
+            vid[i] = idOf[rep[i]];
+        }
+    }
+    if (!alloc_mesh(out, first.size(), corners.size() / 3)) {
+        edxr_scene_free(out);
+        return EDXR_E_NOMEM;
+    }
+    const double d2r = 3.14159265358979323846 / 180.0;
+    const double cx = cos(rot[0] * d2r), sx = sin(rot[0] * d2r), cy = cos(rot[1] * d2r), sy = sin(rot[1] * d2r),
+                 cz = cos(rot[2] * d2r), sz = sin(rot[2] * d2r);
+    // R = Rz Ry Rx
+    const double R[3][3] = {{cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx},
+                            {sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx},
+                            {-sy, cy * sx, cy * cx}};
+    for (size_t v = 0; v < first.size(); v++) {
+        const Corner& c = corners[first[v]];
+        double p[3], n[3];
+        for (int j = 0; j < 3; j++) {
+            p[j] = P[3 * c.p + j] * scl[j];
+            n[j] = (c.n != UINT32_MAX ? N[3 * c.n + j] : AN[3 * c.p + j]) / scl[j];
+        }
+        double wp[3], wn[3];
+        for (int i = 0; i < 3; i++) {
+            wp[i] = R[i][0] * p[0] + R[i][1] * p[1] + R[i][2] * p[2] + pos[i];
+            wn[i] = R[i][0] * n[0] + R[i][1] * n[1] + R[i][2] * n[2];
+        }
+        const double len = sqrt(wn[0] * wn[0] + wn[1] * wn[1] + wn[2] * wn[2]);
+        const double inv = len > 0.0 ? 1.0 / len : 0.0;
+        out->px[v] = (float)wp[0];
+        out->py[v] = (float)wp[1];
+        out->pz[v] = (float)wp[2];
+        out->nx[v] = (float)(wn[0] * inv);
+        out->ny[v] = (float)(wn[1] * inv);
+        out->nz[v] = (float)(wn[2] * inv);
+        static const float kDefaultKd[3] = {0.9f, 0.9f, 0.9f};
+        const float* kd = c.m != UINT32_MAX ? mats[c.m].kd : kDefaultKd;
+        out->cr[v] = kd[0];
+        out->cg[v] = kd[1];
+        out->cb[v] = kd[2];
+    }
+    for (size_t i = 0; i < corners.size(); i++) out->indices[i] = vid[i];
+    return EDXR_OK;
+}
+
+// Bitmap::SaveBitmapFile (used by Renderer::WriteFrameToFile,
+// EDXRaster/Core/Renderer.cpp:352-358), restated: an uncompressed 24-bit BMP
+// (BITMAPINFOHEADER, BI_RGB), rows bottom-up as the back buffer holds them,
+// BGR byte order, each row padded to 4 bytes; alpha is dropped.
+int edxr_write_bmp(const char* path, const uint8_t* rgba, uint32_t width, uint32_t height) {
+    if (!path || !rgba || !width || !height) return EDXR_E_INVALID;
+    const uint32_t row = (3 * width + 3) & ~3u;
+    const uint64_t image = (uint64_t)row * height;
+    if (image + 54 > 0xFFFFFFFFull) return EDXR_E_LIMIT;
+    uint8_t h[54] = {0};
+    auto put32 = [&](int at, uint32_t v) {
+        for (int k = 0; k < 4; k++) h[at + k] = (uint8_t)(v >> (8 * k));
+    };
+    h[0] = 'B';
+    h[1] = 'M';
+    put32(2, (uint32_t)(54 + image));
+    put32(10, 54);
+    put32(14, 40);
+    put32(18, width);
+    put32(22, height);
+    h[26] = 1;   // planes
+    h[28] = 24;  // bits per pixel
+    put32(34, (uint32_t)image);
+    put32(38, 2835);  // 72 dpi
+    put32(42, 2835);
+    FILE* f = fopen(path, "wb");
+    if (!f) return EDXR_E_IO;
+    bool ok = fwrite(h, 1, 54, f) == 54;
+    std::vector<uint8_t> line(row, 0);
+    for (uint32_t y = 0; ok && y < height; y++) {
+        const uint8_t* s = rgba + 4ull * width * y;
+        for (uint32_t x = 0; x < width; x++) {
+            line[3 * x] = s[4 * x + 2];
+            line[3 * x + 1] = s[4 * x + 1];
+            line[3 * x + 2] = s[4 * x];
+        }
+        ok = fwrite(line.data(), 1, row, f) == row;
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? EDXR_OK : EDXR_E_IO;
+}

@@ -8,6 +8,7 @@ follows ``EDX::RasterRenderer::Mesh`` (EDXRaster/Utils/Mesh.h:17-69) with SoA
 fp32 arrays.  Every render goes through libedxraster.so (HIP, gfx950).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -79,6 +80,14 @@ class Mesh:
     def load_heightfield(n, amplitude=0.2, height_seed=3, color_seed=33):
         hm = L.HostMesh()
         _check(None, L.lib().edxr_scene_heightfield(C.byref(hm), n, amplitude, height_seed, color_seed))
+        return Mesh._from_host(hm)
+
+    # Mesh::LoadMesh (EDXRaster/Utils/Mesh.cpp:11-34)
+    @staticmethod
+    def load_obj(path, pos=(0.0, 0.0, 0.0), scl=(1.0, 1.0, 1.0), rot=(0.0, 0.0, 0.0)):
+        hm = L.HostMesh()
+        p, s, r = (np.asarray(v, dtype=np.float32) for v in (pos, scl, rot))
+        _check(None, L.lib().edxr_load_obj(os.fsencode(path), L.fptr(p), L.fptr(s), L.fptr(r), C.byref(hm)))
         return Mesh._from_host(hm)
 
     @staticmethod
@@ -178,6 +187,20 @@ class Renderer:
     # Renderer::SetHierarchicalRasterize (Core/Renderer.h:49)
     def set_hierarchical_rasterize(self, on):
         _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_HIERARCHICAL, 1 if on else 0))
+
+    # Renderer::SetWriteFrames (Core/Renderer.h:50); frames go to <directory>/Frames/
+    def set_write_frames(self, on, directory=None):
+        if directory is not None:
+            _check(self._ctx, L.lib().edxr_set_frame_directory(self._ctx, os.fsencode(directory)))
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_WRITE_FRAMES, 1 if on else 0))
+
+    # Renderer::WriteFrameToFile (Core/Renderer.cpp:352-358)
+    def write_frame_to_file(self):
+        _check(self._ctx, L.lib().edxr_write_frame_to_file(self._ctx))
+
+    @property
+    def frame_count(self):
+        return int(L.lib().edxr_frame_count(self._ctx))
 
     # Renderer::SetMSAAMode (Core/Renderer.cpp:94-98)
     def set_msaa_mode(self, sample_count_log2):

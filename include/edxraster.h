@@ -39,7 +39,9 @@ enum edxr_status {
     EDXR_E_NOMEM = -3,     /* device allocation failed */
     EDXR_E_INDEX = -4,     /* an index >= num_vertices */
     EDXR_E_LIMIT = -5,     /* size beyond a documented limit */
-    EDXR_E_STATE = -6      /* call out of order (e.g. backward before forward) */
+    EDXR_E_STATE = -6,     /* call out of order (e.g. backward before forward) */
+    EDXR_E_IO = -7,        /* a file could not be read or written */
+    EDXR_E_FORMAT = -8     /* malformed input file */
 };
 
 enum edxr_shader_kind {
@@ -84,7 +86,11 @@ enum edxr_state_key {
      * pixel, 0 (1 sample) .. 5 (32 samples); default 0.  Sample positions are
      * the reference's table (Core/FrameBuffer.cpp:107-191); the colour buffer
      * is resolved, depth and IDs are kept per sample. */
-    EDXR_STATE_MSAA = 8
+    EDXR_STATE_MSAA = 8,
+    /* Renderer::SetWriteFrames (Core/Renderer.h:50): after every render_mesh
+     * write <frame directory>/Frames/Frame%05i.bmp, numbered by the frame
+     * counter (Core/Renderer.cpp:114-117, 352-358).  Default off. */
+    EDXR_STATE_WRITE_FRAMES = 9
 };
 
 /* Pipeline stages timed when EDXR_STATE_PROFILE is on. */
@@ -249,6 +255,17 @@ int64_t edxr_debug_read_records(edxr_ctx* ctx, uint32_t* out, uint64_t cap);
  * Returns the reference count or a negative error. */
 int64_t edxr_debug_read_bins(edxr_ctx* ctx, uint64_t* offsets, uint32_t* refs3, uint64_t cap);
 
+/* Directory that WriteFrameToFile writes under (Application::GetBaseDirectory
+ * in the reference): frames go to <dir>/Frames/, created when missing.
+ * Default ".". */
+int edxr_set_frame_directory(edxr_ctx* ctx, const char* dir);
+/* Renderer::WriteFrameToFile (Core/Renderer.cpp:352-358): the last frame's
+ * back buffer as <dir>/Frames/Frame%05i.bmp with the current frame counter,
+ * which render_mesh advances after every frame (Core/Renderer.cpp:117). */
+int edxr_write_frame_to_file(edxr_ctx* ctx);
+/* Number of frames rendered by render_mesh on this context (RenderStates::FrameCount). */
+int64_t edxr_frame_count(edxr_ctx* ctx);
+
 /* ---- scenes (Mesh::LoadSphere / LoadPlane stand-ins; host memory) ---- */
 typedef struct edxr_host_mesh {
     float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb;
@@ -268,6 +285,16 @@ int edxr_scene_sphere_grid(edxr_host_mesh* out, uint32_t grid, uint32_t segments
 /* Appends b to a (a's indices unchanged, b's shifted). */
 int edxr_scene_merge(edxr_host_mesh* a, const edxr_host_mesh* b);
 void edxr_scene_free(edxr_host_mesh* m);
+/* Mesh::LoadMesh (Utils/Mesh.cpp:11-34): a Wavefront OBJ file (with its .mtl
+ * colours) placed by p' = T(pos) Rz Ry Rx S(scl) p, rotation in degrees.
+ * Polygons are fanned in file order; one vertex per distinct (position,
+ * normal, material); missing normals are area-weighted; colour = the
+ * material's Kd, 0.9 grey without one.  Free with edxr_scene_free. */
+int edxr_load_obj(const char* path, const float pos[3], const float scl[3], const float rot[3],
+                  edxr_host_mesh* out);
+/* Bitmap::SaveBitmapFile: 24-bit uncompressed BMP of a bottom-up RGBA8 image
+ * (rows kept in order, BGR bytes, alpha dropped). */
+int edxr_write_bmp(const char* path, const uint8_t* rgba, uint32_t width, uint32_t height);
 /* Left-handed look-at view matrix and D3D-style projection (z in [0,1]),
  * raster matrix x_r = (x+1)W/2, y_r = (1-y)H/2.  Computed in double. */
 void edxr_look_at(const float eye[3], const float at[3], const float up[3], float out[16]);

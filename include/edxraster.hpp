@@ -106,6 +106,12 @@ class Mesh {
         Release();
         check(edxr_scene_heightfield(&m_, n, amplitude, heightSeed, colorSeed));
     }
+    /* Mesh::LoadMesh (Utils/Mesh.cpp:11-34): a Wavefront OBJ (+ .mtl colours)
+     * placed by translation pos, rotation rot (degrees) and scale scl. */
+    void LoadMesh(const float pos[3], const float scl[3], const float rot[3], const char* path) {
+        Release();
+        check(edxr_load_obj(path, pos, scl, rot, &m_));
+    }
     void Release() { edxr_scene_free(&m_); }
 
     uint32_t GetVertexCount() const { return m_.num_vertices; }
@@ -164,6 +170,12 @@ class Renderer {
         check(edxr_set_pixel_shader(ctx_, ps.Kind(), &m));
     }
     void SetHierarchicalRasterize(bool on) { check(edxr_set_state(ctx_, EDXR_STATE_HIERARCHICAL, on ? 1 : 0)); }
+    /* Renderer::SetWriteFrames (Core/Renderer.h:50): every RenderMesh writes
+     * Frames/Frame%05i.bmp under the frame directory ("." by default). */
+    void SetWriteFrames(bool on) { check(edxr_set_state(ctx_, EDXR_STATE_WRITE_FRAMES, on ? 1 : 0)); }
+    void SetFrameDirectory(const char* dir) { check(edxr_set_frame_directory(ctx_, dir)); }
+    /* Renderer::WriteFrameToFile (Core/Renderer.cpp:352-358). */
+    void WriteFrameToFile() { check(edxr_write_frame_to_file(ctx_)); }
     /* Renderer::SetMSAAMode (Core/Renderer.cpp:94-98): 0 (1 sample) .. 5 (32). */
     void SetMSAAMode(int sampleCountLog2) { check(edxr_set_state(ctx_, EDXR_STATE_MSAA, sampleCountLog2)); }
     void RenderMesh(const Mesh& mesh) {

@@ -1,0 +1,180 @@
+"""Host-side file formats: Mesh::LoadMesh's OBJ input (Utils/Mesh.cpp:11-34)
+and WriteFrameToFile's BMP output (Core/Renderer.cpp:352-358).
+
+EDXUtil's ObjMesh and Bitmap are absent, so the semantics are the ones the
+header declares; these tests pin them by closed forms (transforms, fan
+triangulation, plane normals), by round trips (mesh -> OBJ text -> mesh,
+image -> BMP -> decoded image) and by the BMP file layout itself.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from edxraster_amd import Mesh, configs
+from edxraster_amd import _lib as L
+
+
+def write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+    return str(path)
+
+
+def decode_bmp(path):
+    """(H, W, 3) RGB rows bottom-up, from a 24-bit BI_RGB file."""
+    data = open(path, "rb").read()
+    assert data[:2] == b"BM"
+    size, off = struct.unpack_from("<I", data, 2)[0], struct.unpack_from("<I", data, 10)[0]
+    hdr, w, h, planes, bpp, comp, img = struct.unpack_from("<IiiHHII", data, 14)
+    assert (hdr, planes, bpp, comp) == (40, 1, 24, 0)
+    assert size == len(data) and off == 54
+    row = (3 * w + 3) & ~3
+    assert img == row * h and len(data) == 54 + row * h
+    out = np.zeros((h, w, 3), np.uint8)
+    for y in range(h):
+        line = np.frombuffer(data, np.uint8, 3 * w, 54 + y * row).reshape(w, 3)
+        out[y] = line[:, ::-1]
+    return out
+
+
+def test_obj_quad_fan_transform_and_normals(built, tmp_path):
+    obj = write(tmp_path / "quad.obj", """# a unit square in z = 0, one polygon
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+f 1 2 3 4
+""")
+    m = Mesh.load_obj(obj)
+    assert m.num_vertices == 4 and m.num_triangles == 2
+    np.testing.assert_array_equal(m.indices, [0, 1, 2, 0, 2, 3])
+    np.testing.assert_array_equal(m.positions(), [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]])
+    # counter-clockwise seen from +z: the area-weighted normal is +z everywhere
+    np.testing.assert_array_equal(np.stack([m.nx, m.ny, m.nz], 1), np.tile([0, 0, 1], (4, 1)))
+    np.testing.assert_array_equal(np.stack([m.cr, m.cg, m.cb], 1), np.full((4, 3), np.float32(0.9)))
+    # p' = T R S p with R = Rz(90): (1,0,0) -> S (2,0,0) -> R (0,2,0) -> T (1,4,3)
+    t = Mesh.load_obj(obj, pos=(1, 2, 3), scl=(2, 1, 1), rot=(0, 0, 90))
+    np.testing.assert_allclose(t.positions()[1], [1, 4, 3], atol=1e-6)
+    np.testing.assert_allclose(t.positions()[3], [0, 2, 3], atol=1e-6)  # (0,1,0) -> (-1,0,0) + T
+    np.testing.assert_allclose(np.stack([t.nx, t.ny, t.nz], 1)[0], [0, 0, 1], atol=1e-7)
+    # Rx(90) turns the +z normal to -y; Ry(90) turns +z to +x
+    np.testing.assert_allclose(np.stack([*[getattr(Mesh.load_obj(obj, rot=(90, 0, 0)), c)[0] for c in ("nx", "ny", "nz")]]),
+                               [0, -1, 0], atol=1e-7)
+    np.testing.assert_allclose(np.stack([*[getattr(Mesh.load_obj(obj, rot=(0, 90, 0)), c)[0] for c in ("nx", "ny", "nz")]]),
+                               [1, 0, 0], atol=1e-7)
+
+
+def test_obj_materials_indices_and_normals(built, tmp_path):
+    write(tmp_path / "two.mtl", """newmtl red
+Kd 1 0 0
+newmtl blue
+Kd 0 0 0.5
+""")
+    obj = write(tmp_path / "two.obj", """mtllib two.mtl
+v 0 0 0
+v 1 0 0
+v 0 1 0
+v 1 1 0
+vn 0 0 -1
+vt 0.5 0.5
+usemtl red
+f 1//1 2//1 3//1
+usemtl blue
+f -2/1/1 -1/1/1 -3/1/1
+usemtl missing
+f 1 2 4
+""")
+    m = Mesh.load_obj(obj)
+    # vertex 2 (index 1) and 3 (index 2) appear under both materials: distinct output vertices
+    assert m.num_triangles == 3
+    assert m.num_vertices == 3 + 3 + 3
+    cols = np.stack([m.cr, m.cg, m.cb], 1)
+    np.testing.assert_array_equal(cols[m.indices[0:3]], np.tile([1, 0, 0], (3, 1)))
+    np.testing.assert_array_equal(cols[m.indices[3:6]], np.tile(np.float32([0, 0, 0.5]), (3, 1)))
+    np.testing.assert_array_equal(cols[m.indices[6:9]], np.full((3, 3), np.float32(0.9)))
+    # negative indices: -2 -> v3, -1 -> v4, -3 -> v2
+    np.testing.assert_array_equal(m.positions()[m.indices[3:6]], [[0, 1, 0], [1, 1, 0], [1, 0, 0]])
+    # given normals are used as written
+    np.testing.assert_array_equal(np.stack([m.nx, m.ny, m.nz], 1)[m.indices[0:6]], np.tile([0, 0, -1], (6, 1)))
+
+
+def test_obj_round_trip_of_a_sphere(built, tmp_path):
+    s = Mesh.load_sphere(24, 12, 1.5, (0.25, -0.5, 2.0), color_seed=5)
+    lines = ["v %.9g %.9g %.9g" % tuple(p) for p in s.positions()]
+    lines += ["vn %.9g %.9g %.9g" % (a, b, c) for a, b, c in zip(s.nx, s.ny, s.nz)]
+    lines += ["f %d//%d %d//%d %d//%d" % (a + 1, a + 1, b + 1, b + 1, c + 1, c + 1) for a, b, c in s.indices.reshape(-1, 3)]
+    m = Mesh.load_obj(write(tmp_path / "s.obj", "\n".join(lines) + "\n"))
+    assert (m.num_vertices, m.num_triangles) <= (s.num_vertices, s.num_triangles)
+    # vertices renumbered by first use; map back through the index buffers
+    remap = np.full(m.num_vertices, -1)
+    remap[m.indices] = s.indices
+    np.testing.assert_array_equal(m.positions(), s.positions()[remap])
+    n_in = np.stack([s.nx, s.ny, s.nz], 1)[remap]
+    np.testing.assert_allclose(np.stack([m.nx, m.ny, m.nz], 1), n_in / np.linalg.norm(n_in, axis=1)[:, None],
+                               atol=1e-7)
+    np.testing.assert_array_equal(remap[m.indices], s.indices)
+
+
+def test_obj_errors(built, tmp_path):
+    with pytest.raises(RuntimeError):
+        Mesh.load_obj(tmp_path / "absent.obj")
+    with pytest.raises(RuntimeError):
+        Mesh.load_obj(write(tmp_path / "bad.obj", "v 0 0 0\nv 1 0 0\nf 1 2\n"))
+    with pytest.raises(RuntimeError):
+        Mesh.load_obj(write(tmp_path / "range.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 4\n"))
+    hm = L.HostMesh()
+    z = np.zeros(3, np.float32)
+    assert L.lib().edxr_load_obj(os.fsencode(tmp_path / "absent.obj"), L.fptr(z), L.fptr(z + 1), L.fptr(z),
+                                 hm) == L.EDXR_E_IO
+
+
+@pytest.mark.parametrize("w,h", [(3, 2), (4, 1), (17, 5)])
+def test_bmp_layout_and_round_trip(built, tmp_path, w, h):
+    rng = np.random.default_rng(w * 31 + h)
+    rgba = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    path = tmp_path / "f.bmp"
+    assert L.lib().edxr_write_bmp(os.fsencode(path), rgba.ctypes.data, w, h) == 0
+    np.testing.assert_array_equal(decode_bmp(path), rgba[:, :, :3])
+    data = open(path, "rb").read()
+    # first stored pixel is row 0 of the bottom-up buffer, as B, G, R
+    assert tuple(data[54:57]) == (rgba[0, 0, 2], rgba[0, 0, 1], rgba[0, 0, 0])
+
+
+@pytest.mark.gpu
+def test_obj_mesh_renders_like_the_oracle_and_frames_are_written(built, tmp_path):
+    """An OBJ scene (two materials, polygons, generated normals) through the
+    GPU pipeline against the oracle, with SetWriteFrames on: each frame's BMP
+    equals the back buffer, numbered by the frame counter."""
+    from oracle.oracle import Oracle
+    from edxraster_amd import Renderer
+    from tests.test_gpu_parity import assert_frame_parity
+    g = Mesh.load_heightfield(40, 0.3, 7, 9)
+    write(tmp_path / "hf.mtl", "newmtl a\nKd 0.8 0.3 0.2\nnewmtl b\nKd 0.2 0.6 0.9\n")
+    lines = ["mtllib hf.mtl"] + ["v %.9g %.9g %.9g" % tuple(p) for p in g.positions()]
+    tris = g.indices.reshape(-1, 3)
+    for k, (a, b, c) in enumerate(tris):
+        if k % 2 == 0:
+            lines.append("usemtl %s" % ("a" if (k // 64) % 2 else "b"))
+        lines.append("f %d %d %d" % (a + 1, b + 1, c + 1))
+    m = Mesh.load_obj(write(tmp_path / "hf.obj", "\n".join(lines) + "\n"), rot=(-30, 0, 0))
+    w = configs.Workload("obj_heightfield", 320, 240, m, *configs.camera(320, 240, (0.0, 1.2, -2.2), (0, 0, 0)))
+    r = Renderer(w.width, w.height)
+    w.apply(r)
+    r.set_write_frames(True, str(tmp_path))
+    for _ in range(2):
+        r.render_mesh(w.mesh)
+    o = Oracle()
+    o.render_workload(w)
+    info = assert_frame_parity(r, o, "obj")
+    assert info["covered"] > 1000
+    assert r.frame_count == 2
+    frames = sorted(os.listdir(tmp_path / "Frames"))
+    assert frames == ["Frame00000.bmp", "Frame00001.bmp"]
+    rgb = r.get_back_buffer()[:, :, :3]
+    for f in frames:
+        np.testing.assert_array_equal(decode_bmp(tmp_path / "Frames" / f), rgb)
+    r.write_frame_to_file()  # explicit write: the counter has advanced to 2
+    assert os.path.exists(tmp_path / "Frames" / "Frame00002.bmp")
+    r.close()
