@@ -20,6 +20,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+SHADER_NAMES = {0: "Lambertian, constant albedo", 1: "Blinn-Phong s=200", 2: "vertex-colour Blinn-Phong s=32",
+                3: "textured Lambertian, trilinear"}
 
 
 def device_mesh(mesh):
@@ -28,11 +30,14 @@ def device_mesh(mesh):
     from edxraster_amd import _lib as L
     planes = [torch.from_numpy(a).to("cuda") for a in mesh.planes()]
     idx = torch.from_numpy(mesh.indices.view(np.int32)).to("cuda")
+    uv = [None if a is None else torch.from_numpy(a).to("cuda") for a in (mesh.tu, mesh.tv)]
+    tex = None if mesh.tex_ids is None else torch.from_numpy(mesh.tex_ids.view(np.int32)).to("cuda")
     torch.cuda.synchronize()
-    ptr = lambda t: C.cast(C.c_void_p(t.data_ptr()), L.F)
-    m = L.Mesh(*[ptr(t) for t in planes], C.cast(C.c_void_p(idx.data_ptr()), L.U32), mesh.num_vertices,
-               mesh.num_triangles, L.MEM_DEVICE)
-    return m, (planes, idx)
+    ptr = lambda t: None if t is None else C.cast(C.c_void_p(t.data_ptr()), L.F)
+    uptr = lambda t: None if t is None else C.cast(C.c_void_p(t.data_ptr()), L.U32)
+    m = L.Mesh(*[ptr(t) for t in planes], uptr(idx), mesh.num_vertices, mesh.num_triangles, L.MEM_DEVICE,
+               ptr(uv[0]), ptr(uv[1]), uptr(tex))
+    return m, (planes, idx, uv, tex)
 
 
 def cpu_baseline(w, budget_s):
@@ -278,7 +283,7 @@ def main():
         "dtype": "fp32+int32",
         "data": "synthetic (seeded SplitMix64 scene generated on the host, resident in HBM)",
         "config": {"workload": w.name, "samples_per_pixel": 1 << args.msaa, "triangles": T, "vertices": V, "width": w.width, "height": w.height,
-                   "shader": "vertex-colour Blinn-Phong s=32", "parallelism": "single GPU",
+                   "shader": SHADER_NAMES.get(w.shader, str(w.shader)), "parallelism": "single GPU",
                    "launch": "hipGraph replay per frame"},
         "frame_ms": ms,
         "gpix_s": P / (ms * 1e-3) / 1e9,
@@ -299,6 +304,10 @@ def main():
         t = fr["tuning"]
         out["msaa_4x"] = frame_leg(w, max(5, args.steps // 2), 2, brief=True, msaa=2,
                                    tuning=(t["tris_per_chunk"], t["raster_blocks_per_cu"]))
+        # the 10M frame with the textured Lambertian shader (trilinear, two 1024^2 slots)
+        if args.workload != "hf10m_tex":
+            ht = configs.WORKLOADS["hf10m_tex"]()
+            out["textured_10m"] = frame_leg(ht, max(5, args.steps // 2), 2, brief=True)
         out["batched"] = batched_leg(max(3, args.steps // 4), 2)
         out["differentiable"] = diff_leg(max(3, args.steps // 4), 2)
     if not args.no_cpu:

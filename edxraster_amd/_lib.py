@@ -22,6 +22,11 @@ EDXR_E_FORMAT = -8
 SHADER_LAMBERT_ALBEDO = 0
 SHADER_BLINN_PHONG_REF = 1
 SHADER_VC_BLINN_PHONG = 2
+SHADER_TEXTURED_LAMBERT = 3
+
+FILTER_NEAREST = 0
+FILTER_LINEAR = 1
+FILTER_TRILINEAR = 2
 
 STATE_HIERARCHICAL = 1
 STATE_BIN_CHUNKS = 2
@@ -32,6 +37,7 @@ STATE_RASTER_BLOCKS_PER_CU = 6
 STATE_FLOAT_COLOR = 7
 STATE_MSAA = 8
 STATE_WRITE_FRAMES = 9
+STATE_TEXTURE_FILTER = 10
 STAGES = ("setup", "bin_count", "bin_scan", "bin_scatter", "raster")
 
 MEM_HOST = 0
@@ -49,7 +55,12 @@ class Material(C.Structure):
 class Mesh(C.Structure):
     _fields_ = [("px", F), ("py", F), ("pz", F), ("nx", F), ("ny", F), ("nz", F),
                 ("cr", F), ("cg", F), ("cb", F), ("indices", U32),
-                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("memory", C.c_int32)]
+                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("memory", C.c_int32),
+                ("tu", F), ("tv", F), ("tex_ids", U32)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.c_void_p), ("color", C.c_float * 3)]
 
 
 class Stats(C.Structure):
@@ -120,6 +131,7 @@ SIGNATURES = [
     ("edxr_load_obj", C.c_int, [C.c_char_p, F, F, F, C.POINTER(HostMesh)]),
     ("edxr_write_bmp", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("edxr_set_frame_directory", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("edxr_set_textures", C.c_int, [C.c_void_p, C.POINTER(Texture), C.c_uint32]),
     ("edxr_write_frame_to_file", C.c_int, [C.c_void_p]),
     ("edxr_frame_count", C.c_int64, [C.c_void_p]),
     ("edxr_look_at", None, [F, F, F, F]),

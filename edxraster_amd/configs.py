@@ -29,6 +29,8 @@ class Workload:
     shader: int = L.SHADER_VC_BLINN_PHONG
     shininess: float = 32.0
     info: dict = field(default_factory=dict)
+    textures: list = None               # texture slots (textured shader)
+    tex_filter: int = L.FILTER_TRILINEAR
 
     def material(self):
         m = default_material(self.shader)
@@ -45,6 +47,9 @@ class Workload:
         renderer.set_transform(self.mv, self.proj, self.raster)
         renderer.set_light(self.light)
         renderer.set_pixel_shader(self.shader, self.material())
+        if self.textures is not None:
+            renderer.set_textures(self.textures)
+            renderer.set_texture_filter(self.tex_filter)
 
 
 def camera(width, height, eye, at, up=(0.0, 1.0, 0.0)):
@@ -84,10 +89,64 @@ def heightfield_4k(width=3840, height=2160, n=2236):
     return Workload("heightfield_2236_3840x2160", width, height, mesh, mv, p, r)
 
 
+def heightfield_4k_textured(width=3840, height=2160, n=2236, tex_filter=L.FILTER_TRILINEAR):
+    """The 10M-triangle heightfield with the textured Lambertian shader:
+    coordinates repeating 32 times across, alternating between two 1024^2
+    checker textures by triangle pair; trilinear filtering by default."""
+    w = heightfield_4k(width, height, n)
+    m = w.mesh
+    m.tu = ((m.px + 1.0) * 16.0).astype(np.float32)
+    m.tv = ((m.pz + 1.0) * 16.0).astype(np.float32)
+    m.tex_ids = ((np.arange(m.num_triangles) // 2) % 2).astype(np.uint32)
+    w.name = "heightfield_2236_textured_3840x2160"
+    w.shader, w.shininess = L.SHADER_TEXTURED_LAMBERT, 0.0
+    w.textures = [checker_texture(1024, 16, 21), checker_texture(1024, 32, 22)]
+    w.tex_filter = tex_filter
+    return w
+
+
 def heightfield_small(n=64, width=320, height=240, seed=7):
     mesh = Mesh.load_heightfield(n, 0.2, height_seed=seed, color_seed=seed + 30)
     mv, p, r = camera(width, height, (0.0, 1.2, -1.6), (0.0, 0.0, 0.0))
     return Workload("heightfield_%d_%dx%d" % (n, width, height), width, height, mesh, mv, p, r)
+
+
+def checker_texture(size, cells, seed):
+    """RGBA8 checkerboard of `cells` x `cells` squares in seeded colours, with
+    a seeded per-texel noise of +-16 so that mip levels differ."""
+    rng = np.random.default_rng(seed)
+    pal = rng.integers(0, 256, (2, 3))
+    yy, xx = np.mgrid[0:size, 0:size]
+    sq = ((xx * cells // size) + (yy * cells // size)) % 2
+    rgb = pal[sq] + rng.integers(-16, 17, (size, size, 3))
+    img = np.empty((size, size, 4), np.uint8)
+    img[:, :, :3] = np.clip(rgb, 0, 255)
+    img[:, :, 3] = 255
+    return img
+
+
+def textured_scene(width=640, height=480, n=96, tex_filter=L.FILTER_TRILINEAR, sphere=(48, 24)):
+    """LambertianAlbedoPixelShader with texture slots: a heightfield with its
+    texture coordinates repeating 3 times across, alternating between two
+    checker textures (256^2 and 100x60) by triangle pair, and a UV sphere
+    (u = longitude, v = latitude) on a constant-colour slot, seen obliquely so
+    that the trilinear filter spans several mip levels."""
+    hf = Mesh.load_heightfield(n, 0.15, height_seed=5, color_seed=6)
+    hf.tu = ((hf.px + 1.0) * 1.5).astype(np.float32)
+    hf.tv = ((hf.pz + 1.0) * 1.5).astype(np.float32)
+    T = hf.num_triangles
+    hf.tex_ids = ((np.arange(T) // 2) % 2).astype(np.uint32)
+    sp = Mesh.load_sphere(sphere[0], sphere[1], 0.35, (0.1, 0.45, 0.2), color_seed=8)
+    d = np.stack([sp.px - 0.1, sp.py - 0.45, sp.pz - 0.2], 1) / 0.35
+    sp.tu = (np.arctan2(d[:, 2], d[:, 0]) / (2 * np.pi) + 0.5).astype(np.float32)
+    sp.tv = (np.arccos(np.clip(d[:, 1], -1, 1)) / np.pi).astype(np.float32)
+    sp.tex_ids = np.full(sp.num_triangles, 2, np.uint32)
+    mesh = Mesh.merge(hf, sp)
+    mv, p, r = camera(width, height, (0.0, 0.9, -1.9), (0.0, 0.0, 0.3))
+    tex = [checker_texture(256, 8, 11), checker_texture(64, 4, 12)[:60, :, :].copy(), (0.8, 0.55, 0.3)]
+    tex[1] = np.ascontiguousarray(np.concatenate([tex[1], tex[1][:, :36]], axis=1))  # 100 x 60
+    return Workload("textured_%d_%dx%d" % (n, width, height), width, height, mesh, mv, p, r,
+                    shader=L.SHADER_TEXTURED_LAMBERT, shininess=0.0, textures=tex, tex_filter=tex_filter)
 
 
 def _unit(rng):
@@ -157,4 +216,5 @@ WORKLOADS = {
     "small": small_sphere,
     "grid1m": sphere_grid_1080p,
     "hf10m": heightfield_4k,
+    "hf10m_tex": heightfield_4k_textured,
 }

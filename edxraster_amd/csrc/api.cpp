@@ -130,6 +130,10 @@ struct HostBuf {
 // Host description of one job before placement in the batch arrays.
 struct JobInput {
     MeshPtrs mesh;
+    const TexDesc* tex = nullptr;
+    const float4* texels = nullptr;
+    uint32_t numTex = 0;
+    int32_t texFilter = 0;
     uint32_t V, T, W, H;
     uint32_t sampleLevel;
     float mvp[16], raster[16], eye[3], light[3];
@@ -172,8 +176,12 @@ struct edxr_ctx {
     uint64_t profiledFrames = 0;
 
     // owned copies of host meshes
-    DevBuf<float> mesh[9];
-    DevBuf<uint32_t> meshIdx;
+    DevBuf<float> mesh[11];   // the 9 SoA planes, then tu, tv
+    DevBuf<uint32_t> meshIdx, meshTex;
+    DevBuf<TexDesc> texDesc;  // texture slots (edxr_set_textures)
+    DevBuf<float4> texels;
+    uint32_t numTex = 0;
+    int32_t texFilter = kFilterTrilinear;
     DevBuf<uint8_t> batchMesh;
     // pipeline buffers
     DevBuf<RasterRec> rrec;
@@ -273,7 +281,16 @@ int check_mesh(edxr_ctx* c, const edxr_mesh* m, int32_t shader) {
     if (shader == EDXR_SHADER_VC_BLINN_PHONG && (!m->cr || !m->cg || !m->cb))
         return c->fail(EDXR_E_INVALID, "vertex colours required by EDXR_SHADER_VC_BLINN_PHONG");
     if (m->memory != EDXR_MEM_HOST && m->memory != EDXR_MEM_DEVICE) return c->fail(EDXR_E_INVALID, "bad mesh memory");
+    if (shader == EDXR_SHADER_TEXTURED_LAMBERT) {
+        if (!m->tu || !m->tv || (m->num_triangles && !m->tex_ids))
+            return c->fail(EDXR_E_INVALID, "texture coordinates and slots required by EDXR_SHADER_TEXTURED_LAMBERT");
+        if (!c->numTex) return c->fail(EDXR_E_STATE, "no textures set (edxr_set_textures)");
+    }
     return EDXR_OK;
+}
+
+MeshPtrs mesh_ptrs(const float* const f[11], const uint32_t* idx, const uint32_t* tex) {
+    return MeshPtrs{f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], idx, f[9], f[10], tex};
 }
 
 // Per-job constants; the light is normalised on the host as Math::Normalize
@@ -309,6 +326,10 @@ FrameParams job_params(const JobInput& in, uint32_t tilesX, uint32_t tilesY) {
     P.sample_level = in.sampleLevel;
     P.samples = 1u << in.sampleLevel;
     P.mesh = in.mesh;
+    P.tex = in.tex;
+    P.texels = in.texels;
+    P.num_tex = in.numTex;
+    P.tex_filter = (uint32_t)in.texFilter;
     return P;
 }
 
@@ -348,6 +369,7 @@ int build_plan(edxr_ctx* c, const std::vector<JobInput>& in, Plan& pl) {
         Q.max_job_tiles = std::max(Q.max_job_tiles, P.num_tiles);
         Q.max_job_chunks = std::max(Q.max_job_chunks, P.num_chunks);
         Q.max_samples = std::max(Q.max_samples, P.samples);
+        if (P.shader == kTexturedLambert) Q.any_textured = 1;
         pl.jobs.push_back(P);
         if (vert > 0xF0000000ull || slot + 65536 > kRecordMask) return c->fail(EDXR_E_LIMIT, "too many triangles");
     }
@@ -517,7 +539,8 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         const uint32_t ov = h[kCntOverflow];
         if (ov & kOvPolygon) return c->fail(EDXR_E_LIMIT, "clip polygon exceeded 9 vertices");
         if (!ov) {
-            if (h[kCntBadIndex]) return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices");
+            if (h[kCntBadIndex])
+                return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices or texture slot >= texture count");
             uint64_t rejected = 0, mainRecords = 0, tin = 0;
             for (uint32_t s = 0; s < kStripes; s++) {
                 rejected += h[kCntStriped + kStRejected * kStripes + s];
@@ -567,10 +590,11 @@ int stage_batch_meshes(edxr_ctx* c, const edxr_job* jobs, uint32_t n, std::vecto
     for (uint32_t k = 0; k < n; k++) {
         const edxr_mesh& m = jobs[k].mesh;
         if (m.memory != EDXR_MEM_HOST) continue;
-        const float* planes[9] = {m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb};
+        const float* planes[11] = {m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb, m.tu, m.tv};
         for (const float* p : planes)
             if (p) bytes += al(4ull * m.num_vertices);
         bytes += al(12ull * m.num_triangles);
+        if (m.tex_ids) bytes += al(4ull * m.num_triangles);
     }
     if (!bytes) return EDXR_OK;
     hipError_t e;
@@ -579,9 +603,9 @@ int stage_batch_meshes(edxr_ctx* c, const edxr_job* jobs, uint32_t n, std::vecto
     for (uint32_t k = 0; k < n; k++) {
         const edxr_mesh& m = jobs[k].mesh;
         if (m.memory != EDXR_MEM_HOST) continue;
-        const float* planes[9] = {m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb};
-        const float* dst[9];
-        for (int q = 0; q < 9; q++) {
+        const float* planes[11] = {m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb, m.tu, m.tv};
+        const float* dst[11];
+        for (int q = 0; q < 11; q++) {
             dst[q] = nullptr;
             if (!planes[q]) continue;
             uint8_t* d = c->batchMesh.p + off;
@@ -598,8 +622,17 @@ int stage_batch_meshes(edxr_ctx* c, const edxr_job* jobs, uint32_t n, std::vecto
                 hipSuccess)
             return c->hip_fail(e, "hipMemcpyAsync(batch indices)");
         off += al(12ull * m.num_triangles);
-        in[k].mesh = MeshPtrs{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8],
-                              (const uint32_t*)di};
+        const uint32_t* dt = nullptr;
+        if (m.tex_ids) {
+            uint8_t* d = c->batchMesh.p + off;
+            if (m.num_triangles &&
+                (e = hipMemcpyAsync(d, m.tex_ids, 4ull * m.num_triangles, hipMemcpyHostToDevice, c->stream)) !=
+                    hipSuccess)
+                return c->hip_fail(e, "hipMemcpyAsync(batch texture slots)");
+            dt = (const uint32_t*)d;
+            off += al(4ull * m.num_triangles);
+        }
+        in[k].mesh = mesh_ptrs(dst, (const uint32_t*)di, dt);
     }
     return EDXR_OK;
 }
@@ -614,7 +647,7 @@ void edxr_default_material(int32_t kind, edxr_material* m) {
     m->kd = 3.0f;
     m->ks = 3.0f;
     m->clamp_specular = 0;
-    if (kind == EDXR_SHADER_LAMBERT_ALBEDO) {
+    if (kind == EDXR_SHADER_LAMBERT_ALBEDO || kind == EDXR_SHADER_TEXTURED_LAMBERT) {
         m->shininess = 0.0f;
         m->albedo[0] = m->albedo[1] = m->albedo[2] = 0.9f;
     } else {
@@ -659,6 +692,9 @@ void edxr_destroy(edxr_ctx* c) {
     drop_graph(c);
     for (auto& b : c->mesh) b.release();
     c->meshIdx.release();
+    c->meshTex.release();
+    c->texDesc.release();
+    c->texels.release();
     c->batchMesh.release();
     c->rrec.release();
     c->srec.release();
@@ -712,7 +748,7 @@ int edxr_set_light(edxr_ctx* c, const float dir[3]) {
 }
 
 int edxr_set_pixel_shader(edxr_ctx* c, int32_t kind, const edxr_material* m) {
-    if (!c || kind < EDXR_SHADER_LAMBERT_ALBEDO || kind > EDXR_SHADER_VC_BLINN_PHONG) return EDXR_E_INVALID;
+    if (!c || kind < EDXR_SHADER_LAMBERT_ALBEDO || kind > EDXR_SHADER_TEXTURED_LAMBERT) return EDXR_E_INVALID;
     c->shader = kind;
     if (m) c->mat = *m;
     else edxr_default_material(kind, &c->mat);
@@ -744,6 +780,11 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
             return EDXR_OK;
         case EDXR_STATE_FLOAT_COLOR: c->floatColor = value ? 1 : 0; return EDXR_OK;
         case EDXR_STATE_WRITE_FRAMES: c->writeFrames = value ? 1 : 0; return EDXR_OK;
+        case EDXR_STATE_TEXTURE_FILTER:
+            if (value < EDXR_FILTER_NEAREST || value > EDXR_FILTER_TRILINEAR)
+                return c->fail(EDXR_E_INVALID, "unknown texture filter");
+            c->texFilter = value;
+            return EDXR_OK;
         case EDXR_STATE_MSAA: {
             if (value < 0 || value > (int32_t)kMaxSampleLevel)
                 return c->fail(EDXR_E_INVALID, "MSAA level out of range (0..5)");
@@ -771,11 +812,12 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
 
     MeshPtrs M;
     if (m->memory == EDXR_MEM_DEVICE) {
-        M = MeshPtrs{m->px, m->py, m->pz, m->nx, m->ny, m->nz, m->cr, m->cg, m->cb, m->indices};
+        const float* const f[11] = {m->px, m->py, m->pz, m->nx, m->ny, m->nz, m->cr, m->cg, m->cb, m->tu, m->tv};
+        M = mesh_ptrs(f, m->indices, m->tex_ids);
     } else {
-        const float* src[9] = {m->px, m->py, m->pz, m->nx, m->ny, m->nz, m->cr, m->cg, m->cb};
-        const float* dst[9];
-        for (int k = 0; k < 9; k++) {
+        const float* src[11] = {m->px, m->py, m->pz, m->nx, m->ny, m->nz, m->cr, m->cg, m->cb, m->tu, m->tv};
+        const float* dst[11];
+        for (int k = 0; k < 11; k++) {
             dst[k] = nullptr;
             if (!src[k]) continue;
             if ((e = c->mesh[k].ensure(V)) != hipSuccess) return c->hip_fail(e, "hipMalloc(mesh)");
@@ -788,7 +830,15 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
         if (T && (e = hipMemcpyAsync(c->meshIdx.p, m->indices, 12ull * T, hipMemcpyHostToDevice, c->stream)) !=
                      hipSuccess)
             return c->hip_fail(e, "hipMemcpyAsync(indices)");
-        M = MeshPtrs{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7], dst[8], c->meshIdx.p};
+        const uint32_t* dt = nullptr;
+        if (m->tex_ids) {
+            if ((e = c->meshTex.ensure(T)) != hipSuccess) return c->hip_fail(e, "hipMalloc(texture slots)");
+            if (T && (e = hipMemcpyAsync(c->meshTex.p, m->tex_ids, 4ull * T, hipMemcpyHostToDevice, c->stream)) !=
+                         hipSuccess)
+                return c->hip_fail(e, "hipMemcpyAsync(texture slots)");
+            dt = c->meshTex.p;
+        }
+        M = mesh_ptrs(dst, c->meshIdx.p, dt);
     }
 
     JobInput in;
@@ -805,6 +855,10 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     in.shader = c->shader;
     in.hierarchical = c->hierarchical;
     in.sampleLevel = (uint32_t)c->msaaLog2;
+    in.tex = c->texDesc.p;
+    in.texels = c->texels.p;
+    in.numTex = c->numTex;
+    in.texFilter = c->texFilter;
     Plan pl;
     if ((r = build_plan(c, std::vector<JobInput>{in}, pl)) != EDXR_OK) return r;
     c->rendered = false;
@@ -825,6 +879,71 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     // RenderMesh's epilogue (Core/Renderer.cpp:114-117).
     if (c->writeFrames && (r = edxr_write_frame_to_file(c)) != EDXR_OK) return r;
     c->frameCount++;
+    return EDXR_OK;
+}
+
+// Texture slots with their mip chains (RGBA floats, b / 255 at level 0; each
+// further level halves the sides, floor, at least 1, texel = 0.25 * (((a + b)
+// + c) + d) of its 2x2 parent texels, a side of 1 repeating its texel).
+int edxr_set_textures(edxr_ctx* c, const edxr_texture* tex, uint32_t n) {
+    if (!c || (n && !tex)) return EDXR_E_INVALID;
+    std::vector<TexDesc> desc(n);
+    std::vector<float4> texels;
+    for (uint32_t k = 0; k < n; k++) {
+        const edxr_texture& t = tex[k];
+        TexDesc& d = desc[k];
+        memset(&d, 0, sizeof(d));
+        d.color[0] = t.color[0];
+        d.color[1] = t.color[1];
+        d.color[2] = t.color[2];
+        if (!t.rgba) continue;  // ConstantTexture2D
+        if (!t.width || !t.height || t.width > 16384 || t.height > 16384)
+            return c->fail(EDXR_E_LIMIT, "texture size out of range");
+        d.width = t.width;
+        d.height = t.height;
+        d.offset = (uint32_t)texels.size();
+        uint32_t w = t.width, h = t.height;
+        const size_t base = texels.size();
+        texels.resize(base + (size_t)w * h);
+        for (size_t i = 0; i < (size_t)w * h; i++)
+            texels[base + i] = make_float4(t.rgba[4 * i] / 255.0f, t.rgba[4 * i + 1] / 255.0f,
+                                           t.rgba[4 * i + 2] / 255.0f, t.rgba[4 * i + 3] / 255.0f);
+        d.levels = 1;
+        size_t prev = base;
+        while (w > 1 || h > 1) {
+            const uint32_t w2 = std::max(1u, w >> 1), h2 = std::max(1u, h >> 1);
+            const size_t at = texels.size();
+            texels.resize(at + (size_t)w2 * h2);
+            for (uint32_t j = 0; j < h2; j++)
+                for (uint32_t i = 0; i < w2; i++) {
+                    const uint32_t i0 = 2 * i, i1 = std::min(2 * i + 1, w - 1);
+                    const uint32_t j0 = 2 * j, j1 = std::min(2 * j + 1, h - 1);
+                    const float4 a = texels[prev + (size_t)j0 * w + i0], b = texels[prev + (size_t)j0 * w + i1];
+                    const float4 cc = texels[prev + (size_t)j1 * w + i0], e = texels[prev + (size_t)j1 * w + i1];
+                    texels[at + (size_t)j * w2 + i] =
+                        make_float4(0.25f * (((a.x + b.x) + cc.x) + e.x), 0.25f * (((a.y + b.y) + cc.y) + e.y),
+                                    0.25f * (((a.z + b.z) + cc.z) + e.z), 0.25f * (((a.w + b.w) + cc.w) + e.w));
+                }
+            prev = at;
+            w = w2;
+            h = h2;
+            d.levels++;
+        }
+        if (texels.size() > 0xFFFFFFFFull) return c->fail(EDXR_E_LIMIT, "texture memory out of range");
+    }
+    hipError_t e;
+    if ((e = c->texDesc.ensure(std::max<uint32_t>(n, 1))) != hipSuccess ||
+        (e = c->texels.ensure(std::max<size_t>(texels.size(), 1))) != hipSuccess)
+        return c->hip_fail(e, "hipMalloc(textures)");
+    if (n && (e = hipMemcpyAsync(c->texDesc.p, desc.data(), sizeof(TexDesc) * n, hipMemcpyHostToDevice, c->stream)) !=
+                 hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(texture slots)");
+    if (!texels.empty() && (e = hipMemcpyAsync(c->texels.p, texels.data(), sizeof(float4) * texels.size(),
+                                               hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        return c->hip_fail(e, "hipMemcpyAsync(texels)");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "set_textures");
+    c->numTex = n;
+    drop_graph(c);  // texture pointers are part of the captured job table
     return EDXR_OK;
 }
 
@@ -859,7 +978,7 @@ int edxr_render_batch(edxr_ctx* c, const edxr_job* jobs, uint32_t n) {
     int r;
     for (uint32_t k = 0; k < n; k++) {
         const edxr_job& j = jobs[k];
-        if (j.shader < EDXR_SHADER_LAMBERT_ALBEDO || j.shader > EDXR_SHADER_VC_BLINN_PHONG)
+        if (j.shader < EDXR_SHADER_LAMBERT_ALBEDO || j.shader > EDXR_SHADER_TEXTURED_LAMBERT)
             return c->fail(EDXR_E_INVALID, "bad shader kind in job " + std::to_string(k));
         if ((r = check_mesh(c, &j.mesh, j.shader)) != EDXR_OK) {
             c->err += " (job " + std::to_string(k) + ")";
@@ -867,7 +986,12 @@ int edxr_render_batch(edxr_ctx* c, const edxr_job* jobs, uint32_t n) {
         }
         JobInput& d = in[k];
         const edxr_mesh& m = j.mesh;
-        d.mesh = MeshPtrs{m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb, m.indices};
+        const float* const f[11] = {m.px, m.py, m.pz, m.nx, m.ny, m.nz, m.cr, m.cg, m.cb, m.tu, m.tv};
+        d.mesh = mesh_ptrs(f, m.indices, m.tex_ids);
+        d.tex = c->texDesc.p;
+        d.texels = c->texels.p;
+        d.numTex = c->numTex;
+        d.texFilter = c->texFilter;
         d.V = m.num_vertices;
         d.T = m.num_triangles;
         d.W = j.width;
@@ -965,6 +1089,9 @@ int edxr_backward(edxr_ctx* c, const float* d_rgb, const edxr_grads* g, uint32_t
     const Plan& pl = c->lastPlan;
     if (n != pl.Q.num_jobs) return c->fail(EDXR_E_INVALID, "one edxr_grads per job of the last render");
     if (pl.Q.max_samples > 1) return c->fail(EDXR_E_STATE, "backward of a multisampled frame is not supported");
+    for (const FrameParams& P : pl.jobs)
+        if (P.shader == kTexturedLambert)
+            return c->fail(EDXR_E_STATE, "backward of the textured shader is not supported");
     for (const FrameParams& P : pl.jobs) {
         const float* R = P.raster;
         if (R[2] != 0.0f || R[6] != 0.0f || R[12] != 0.0f || R[13] != 0.0f || R[14] != 0.0f || R[15] != 1.0f ||

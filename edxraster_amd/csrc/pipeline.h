@@ -31,11 +31,22 @@ constexpr uint32_t kMaxChunks = 4096;
 // otherwise (first record << 3) | record count (0..7).
 constexpr uint32_t kDescCompact = 1u << 31;
 
-enum ShaderKind { kLambertAlbedo = 0, kBlinnPhongRef = 1, kVcBlinnPhong = 2 };
+enum ShaderKind { kLambertAlbedo = 0, kBlinnPhongRef = 1, kVcBlinnPhong = 2, kTexturedLambert = 3 };
+enum TextureFilter { kFilterNearest = 0, kFilterLinear = 1, kFilterTrilinear = 2 };
 
 struct MeshPtrs {
     const float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb;
     const uint32_t* idx;
+    const float *tu, *tv;      // texture coordinates (textured shader only)
+    const uint32_t* tex;       // texture slot per triangle (textured shader only)
+};
+
+// One texture slot: its mip chain in the context's texel array (RGBA floats,
+// level l of size max(1, w >> l) x max(1, h >> l) at offset + level_offset(l)),
+// or a constant colour (levels == 0).
+struct TexDesc {
+    uint32_t width, height, levels, offset;
+    float color[4];
 };
 
 // Constants of one render job (one mesh, camera, shader and render target),
@@ -66,6 +77,9 @@ struct FrameParams {
     uint32_t vert_base, slot_base, tile_base, chunk_base;
     uint64_t count_base, group_base, pix_base;
     MeshPtrs mesh;
+    const TexDesc* tex;   // texture slots (textured shader), num_tex of them
+    const float4* texels;
+    uint32_t num_tex, tex_filter;
 };
 
 // Batch-wide sizes and the block -> job tables (null for a single job).
@@ -79,6 +93,7 @@ struct BatchParams {
     uint32_t max_job_tiles;     // max over jobs of num_tiles
     uint32_t max_job_chunks;    // max over jobs of num_chunks
     uint32_t max_samples;       // max over jobs of samples
+    uint32_t any_textured;      // 1 if a job uses the textured shader
     const uint32_t* vblock_job;
     const uint32_t* sblock_job;
     const uint32_t* chunk_job;
@@ -86,14 +101,14 @@ struct BatchParams {
 };
 
 // Post-transform vertex for shading (ProjectedVertex, Core/Shader.h:13-25,
-// with RGB instead of UV): one 48-byte AoS record so a vertex fetch touches
-// one cache line instead of nine SoA planes.  Indices < V are the input
+// with an RGB colour beside the texture coordinate): one 48-byte AoS record so
+// a vertex fetch touches one cache line instead of eleven SoA planes.  Indices < V are the input
 // vertices (written by k_vertex); V + e are vertices created by clipping
 // (Core/Clipper.h:136-151).
 struct __attribute__((aligned(16))) ShadeVert {
     float px, py, pz, nx;
     float ny, nz, cr, cg;
-    float cb, inv_w, pad0, pad1;
+    float cb, inv_w, u, v;
 };
 static_assert(sizeof(ShadeVert) == 48, "ShadeVert layout");
 

@@ -165,8 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_vertex(BatchParams Q, const FramePar
     // vertex, so every unclipped triangle's setup reuses them.
     const float4 c = xform4(P.mvp, o.px, o.py, o.pz, 1.0f);
     o.inv_w = 1.0f / c.w;
-    o.pad0 = 0.0f;
-    o.pad1 = 0.0f;
+    o.u = M.tu ? M.tu[v] : 0.0f;
+    o.v = M.tv ? M.tv[v] : 0.0f;
     verts[slot] = o;
     ScreenVert sv;
     const int2 xy = snap_point(P, c.x / c.w, c.y / c.w, c.z / c.w);
@@ -228,6 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
 #pragma unroll
     for (uint32_t u = 0; u < kSetupSlots; u++) {
         bad[u] = valid[u] && (i0[u] >= P.num_vertices || i1[u] >= P.num_vertices || i2[u] >= P.num_vertices);
+        if (P.shader == kTexturedLambert && valid[u] && M.tex[s[u] - P.slot_base] >= P.num_tex) bad[u] = true;
         if (valid[u] && !bad[u]) {
             sv[u][0] = B.sverts[P.vert_base + i0[u]];
             sv[u][1] = B.sverts[P.vert_base + i1[u]];
@@ -465,8 +466,8 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
                 o.cg = comb(a.cg, b.cg, cc.cg);
                 o.cb = comb(a.cb, b.cb, cc.cb);
                 o.inv_w = 1.0f / pos.w;
-                o.pad0 = 0.0f;
-                o.pad1 = 0.0f;
+                o.u = comb(a.u, b.u, cc.u);
+                o.v = comb(a.v, b.v, cc.v);
                 B.verts[vid] = o;
             }
             sPos[out][j][lane] = make_float4(pos.x / pos.w, pos.y / pos.w, pos.z / pos.w, pos.z * (1.0f / pos.w));
@@ -1173,6 +1174,131 @@ __device__ __forceinline__ float3 shade_pixel(const FrameParams& P, float b0, fl
     return make_float3(r, g, b);
 }
 
+// ---- Texture slots (Texture2D<Color>::Sample, EDXUtil; declared semantics:
+// texel (i, j) centred at ((i + 0.5) / w, (j + 0.5) / h), coordinates wrap,
+// float-to-index conversion with NaN -> 0 and a clamp to +-2^30). ----
+__device__ __forceinline__ int tex_index(float x) {
+    return x != x ? 0 : (int)fminf(fmaxf(x, -1073741824.0f), 1073741823.0f);
+}
+__device__ __forceinline__ float4 tex_texel(const FrameParams& P, uint32_t off, int w, int h, int i, int j) {
+    i %= w;
+    if (i < 0) i += w;
+    j %= h;
+    if (j < 0) j += h;
+    return P.texels[off + (uint32_t)j * (uint32_t)w + (uint32_t)i];
+}
+// Bilinear filter over one level of size w x h at offset off.
+__device__ __forceinline__ float3 tex_bilinear(const FrameParams& P, uint32_t off, int w, int h, float u, float v) {
+    const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int i = tex_index(x0), j = tex_index(y0);
+    const float4 c00 = tex_texel(P, off, w, h, i, j), c10 = tex_texel(P, off, w, h, i + 1, j);
+    const float4 c01 = tex_texel(P, off, w, h, i, j + 1), c11 = tex_texel(P, off, w, h, i + 1, j + 1);
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    return make_float3(gy * (gx * c00.x + fx * c10.x) + fy * (gx * c01.x + fx * c11.x),
+                       gy * (gx * c00.y + fx * c10.y) + fy * (gx * c01.y + fx * c11.y),
+                       gy * (gx * c00.z + fx * c10.z) + fy * (gx * c01.z + fx * c11.z));
+}
+// Offset of mip level l of a slot.
+__device__ __forceinline__ uint32_t tex_level(const TexDesc& d, uint32_t l, int& w, int& h) {
+    uint32_t off = d.offset;
+    w = (int)d.width;
+    h = (int)d.height;
+    for (uint32_t k = 0; k < l; k++) {
+        off += (uint32_t)w * (uint32_t)h;
+        w = max(1, w >> 1);
+        h = max(1, h >> 1);
+    }
+    return off;
+}
+// Sample(uv, differentials): nearest / bilinear on level 0, or trilinear with
+// lod = log2 of the longer of the two texel-space differentials.
+__device__ __forceinline__ float3 tex_sample(const FrameParams& P, const TexDesc& d, float u, float v, float dux,
+                                             float dvx, float duy, float dvy) {
+    if (d.levels == 0) return make_float3(d.color[0], d.color[1], d.color[2]);
+    const int w = (int)d.width, h = (int)d.height;
+    if (P.tex_filter == kFilterNearest) {
+        const float4 c = tex_texel(P, d.offset, w, h, tex_index(floorf(u * (float)w)), tex_index(floorf(v * (float)h)));
+        return make_float3(c.x, c.y, c.z);
+    }
+    if (P.tex_filter == kFilterLinear) return tex_bilinear(P, d.offset, w, h, u, v);
+    const float ax = dux * (float)w, ay = dvx * (float)h, bx = duy * (float)w, by = dvy * (float)h;
+    const float rho2 = fmaxf(ax * ax + ay * ay, bx * bx + by * by);
+    float lod = rho2 > 0.0f ? 0.5f * log2f(rho2) : 0.0f;
+    lod = fminf(fmaxf(lod, 0.0f), (float)(d.levels - 1));
+    const float fl = floorf(lod);
+    const uint32_t l0 = (uint32_t)fl, l1 = min(l0 + 1, d.levels - 1);
+    const float t = lod - fl;
+    int w0, h0, w1, h1;
+    const uint32_t o0 = tex_level(d, l0, w0, h0), o1 = tex_level(d, l1, w1, h1);
+    const float3 a = tex_bilinear(P, o0, w0, h0, u, v), b = tex_bilinear(P, o1, w1, h1, u, v);
+    const float s = 1.0f - t;
+    return make_float3(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z);
+}
+
+// Perspective-correct texture coordinate of the triangle at pixel (x, y)
+// (Fragment::Interpolate, Core/Shader.h:142-170, at that pixel's centre).
+__device__ __forceinline__ float2 interp_uv(const RTri& t, int x, int y, const ShadeVert& v0, const ShadeVert& v1,
+                                           const ShadeVert& v2) {
+    const int px = x * 16 + 8, py = y * 16 + 8;
+    float b0 = (float)(t.B1 * (px - t.x2) + t.C1 * (py - t.y2)) * t.inv_det;
+    float b1 = (float)(t.B2 * (px - t.x2) + t.C2 * (py - t.y2)) * t.inv_det;
+    float b2 = 1.0f - b0 - b1;
+    b0 *= v0.inv_w;
+    b1 *= v1.inv_w;
+    b2 *= v2.inv_w;
+    const float invB = 1.0f / (b0 + b1 + b2);
+    b0 *= invB;
+    b1 *= invB;
+    b2 = 1.0f - b0 - b1;
+    return make_float2(b0 * v0.u + b1 * v1.u + b2 * v2.u, b0 * v0.v + b1 * v1.v + b2 * v2.v);
+}
+
+// LambertianAlbedoPixelShader (Core/Shader.h:209-244) for pixel (x, y) of the
+// 2x2 quad at (x & ~1, y & ~1): the albedo is the triangle's texture sampled
+// at the pixel's coordinate with the quad's differentials (lane 1 - lane 0,
+// lane 2 - lane 0; uncovered lanes extrapolate the triangle's plane).
+__device__ __forceinline__ float3 shade_textured(const FrameParams& P, const RTri& t, int x, int y, float l0, float l1,
+                                                 const ShadeVert& v0, const ShadeVert& v1, const ShadeVert& v2,
+                                                 uint32_t slot) {
+    const int qx = x & ~1, qy = y & ~1;
+    const float2 q0 = interp_uv(t, qx, qy, v0, v1, v2), q1 = interp_uv(t, qx + 1, qy, v0, v1, v2),
+                 q2 = interp_uv(t, qx, qy + 1, v0, v1, v2);
+    const uint32_t lane = (uint32_t)(x & 1) | ((uint32_t)(y & 1) << 1);
+    const float2 uv = lane == 0 ? q0 : lane == 1 ? q1 : lane == 2 ? q2 : interp_uv(t, x, y, v0, v1, v2);
+    const float3 albedo = tex_sample(P, P.tex[slot], uv.x, uv.y, q1.x - q0.x, q1.y - q0.y, q2.x - q0.x, q2.y - q0.y);
+    // the normal and the diffuse term as shade_pixel's Lambertian path
+    float b2 = 1.0f - l0 - l1;
+    float b0 = l0 * v0.inv_w, b1 = l1 * v1.inv_w;
+    b2 *= v2.inv_w;
+    const float invB = 1.0f / (b0 + b1 + b2);
+    b0 *= invB;
+    b1 *= invB;
+    b2 = 1.0f - b0 - b1;
+    const float Nx = b0 * v0.nx + b1 * v1.nx + b2 * v2.nx;
+    const float Ny = b0 * v0.ny + b1 * v1.ny + b2 * v2.ny;
+    const float Nz = b0 * v0.nz + b1 * v1.nz + b2 * v2.nz;
+    const float w = 1.0f / sqrtf(Nx * Nx + Ny * Ny + Nz * Nz);
+    const float nx = Nx * w, ny = Ny * w, nz = Nz * w;
+    float diffuseAmount = P.light[0] * nx + P.light[1] * ny + P.light[2] * nz;
+    diffuseAmount = diffuseAmount < 0.0f ? 0.0f : diffuseAmount;
+    const float diffuse = (diffuseAmount + P.ambient) * P.kd * (float)(1.0 / 3.14159265358979323846);
+    return make_float3(diffuse * albedo.x, diffuse * albedo.y, diffuse * albedo.z);
+}
+
+// The pixel shader of the job for the winner at pixel (x, y); kTex: a job of
+// the launch may use the textured shader (its code is only compiled in then,
+// keeping the other shaders' register budget).
+template <bool kTex>
+__device__ __forceinline__ float3 shade_any(const FrameParams& P, const RTri& t, uint32_t order, int x, int y,
+                                            float l0, float l1, const ShadeVert& v0, const ShadeVert& v1,
+                                            const ShadeVert& v2) {
+    if (kTex && P.shader == kTexturedLambert)
+        return shade_textured(P, t, x, y, l0, l1, v0, v1, v2, P.mesh.tex[(order >> 4) - P.slot_base]);
+    return shade_pixel(P, l0, l1, v0, v1, v2);
+}
+
 }  // namespace
 
 namespace {
@@ -1530,6 +1656,7 @@ __device__ __forceinline__ uint64_t tile_key(const uint64_t* g, uint32_t nseg, u
 }
 }  // namespace
 
+template <bool kTex>
 __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                  DeviceBuffers B) {
     constexpr uint32_t kPix = kTileSize * kTileSize;
@@ -1562,7 +1689,7 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
         float l0, l1, d;
         bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
         const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
-        const float3 c = shade_pixel(P, l0, l1, v0, v1, v2);
+        const float3 c = shade_any<kTex>(P, t, order, x, y, l0, l1, v0, v1, v2);
         B.color[o] = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
         if (B.color_f32) {
             B.color_f32[3 * o] = clamp01(c.x);
@@ -1602,7 +1729,7 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
             if (order != cachedOrder) {
                 bary_depth(t, cx, cy, l0, l1, d);
                 const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
-                const float3 c = shade_pixel(P, l0, l1, v0, v1, v2);
+                const float3 c = shade_any<kTex>(P, t, order, x, y, l0, l1, v0, v1, v2);
                 cachedColor = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
                 cachedOrder = order;
             }
@@ -1665,7 +1792,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     mark(4);
     if (Q.max_samples > 1) hipLaunchKernelGGL((k_raster<true>), dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
     else hipLaunchKernelGGL((k_raster<false>), dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
-    hipLaunchKernelGGL(k_shade, dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
+    if (Q.any_textured) hipLaunchKernelGGL((k_shade<true>), dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
+    else hipLaunchKernelGGL((k_shade<false>), dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
     mark(5);
     return hipGetLastError();
 }

@@ -27,14 +27,19 @@ def _mat(m):
 
 
 class Mesh:
-    """SoA mesh: positions, normals, RGB colours (float32, V each) and uint32 indices (3T)."""
+    """SoA mesh: positions, normals, RGB colours (float32, V each) and uint32
+    indices (3T); optionally texture coordinates tu, tv (V each) and a texture
+    slot per triangle (tex_ids, T) for the textured shader."""
 
-    def __init__(self, px, py, pz, nx, ny, nz, cr, cg, cb, indices):
+    def __init__(self, px, py, pz, nx, ny, nz, cr, cg, cb, indices, tu=None, tv=None, tex_ids=None):
         f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
         self.px, self.py, self.pz = f(px), f(py), f(pz)
         self.nx, self.ny, self.nz = f(nx), f(ny), f(nz)
         self.cr, self.cg, self.cb = f(cr), f(cg), f(cb)
         self.indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        self.tu = None if tu is None else f(tu)
+        self.tv = None if tv is None else f(tv)
+        self.tex_ids = None if tex_ids is None else np.ascontiguousarray(tex_ids, dtype=np.uint32).reshape(-1)
 
     @property
     def num_vertices(self):
@@ -53,7 +58,8 @@ class Mesh:
     def c_struct(self):
         """edxr_mesh over this mesh's host arrays (keep the Mesh alive meanwhile)."""
         return L.Mesh(*[L.fptr(a) for a in self.planes()], L.uptr(self.indices),
-                      self.num_vertices, self.num_triangles, L.MEM_HOST)
+                      self.num_vertices, self.num_triangles, L.MEM_HOST,
+                      L.fptr(self.tu), L.fptr(self.tv), L.uptr(self.tex_ids))
 
     @staticmethod
     def _from_host(hm):
@@ -100,7 +106,14 @@ class Mesh:
     def merge(a, b):
         planes = [np.concatenate([x, y]) for x, y in zip(a.planes(), b.planes())]
         idx = np.concatenate([a.indices, b.indices + np.uint32(a.num_vertices)])
-        return Mesh(*planes, idx)
+        extra = {}
+        for n, size in (("tu", "num_vertices"), ("tv", "num_vertices"), ("tex_ids", "num_triangles")):
+            x, y = getattr(a, n), getattr(b, n)
+            if x is not None or y is not None:
+                z = lambda m, v: v if v is not None else np.zeros(getattr(m, size), np.uint32 if n == "tex_ids"
+                                                                  else np.float32)
+                extra[n] = np.concatenate([z(a, x), z(b, y)])
+        return Mesh(*planes, idx, **extra)
 
 
 def look_at(eye, at, up=(0.0, 1.0, 0.0)):
@@ -201,6 +214,25 @@ class Renderer:
     @property
     def frame_count(self):
         return int(L.lib().edxr_frame_count(self._ctx))
+
+    # Mesh::GetTextures bound as RenderStates::TextureSlots: each slot an
+    # (H, W, 4) uint8 image (rows from v = 0) or an RGB colour triple.
+    def set_textures(self, textures):
+        arr = (L.Texture * max(1, len(textures)))()
+        self._textures = []
+        for k, t in enumerate(textures):
+            if isinstance(t, np.ndarray):
+                img = np.ascontiguousarray(t, dtype=np.uint8)
+                assert img.ndim == 3 and img.shape[2] == 4
+                self._textures.append(img)
+                arr[k].width, arr[k].height, arr[k].rgba = img.shape[1], img.shape[0], img.ctypes.data
+            else:
+                arr[k].color[:] = [float(c) for c in t]
+        _check(self._ctx, L.lib().edxr_set_textures(self._ctx, arr, len(textures)))
+
+    # Renderer::SetTextureFilter (RenderStates::TexFilter): L.FILTER_*
+    def set_texture_filter(self, f):
+        _check(self._ctx, L.lib().edxr_set_state(self._ctx, L.STATE_TEXTURE_FILTER, int(f)))
 
     # Renderer::SetMSAAMode (Core/Renderer.cpp:94-98)
     def set_msaa_mode(self, sample_count_log2):

@@ -51,7 +51,19 @@ enum edxr_shader_kind {
     /* BlinnPhongPixelShader (Core/Shader.h:246-282): albedo 1, shininess 200. */
     EDXR_SHADER_BLINN_PHONG_REF = 1,
     /* Vertex-colour Blinn-Phong: diffuse * interpolated RGB + specular. */
-    EDXR_SHADER_VC_BLINN_PHONG = 2
+    EDXR_SHADER_VC_BLINN_PHONG = 2,
+    /* LambertianAlbedoPixelShader (Core/Shader.h:209-244) with the albedo
+     * sampled from the triangle's texture slot (edxr_set_textures) at the
+     * interpolated texture coordinate, with the 2x2 quad's differentials and
+     * the EDXR_STATE_TEXTURE_FILTER filter.  The mesh needs tu, tv, tex_ids. */
+    EDXR_SHADER_TEXTURED_LAMBERT = 3
+};
+
+/* Texture filters (RenderStates::TexFilter, Core/RenderStates.h:22). */
+enum edxr_texture_filter {
+    EDXR_FILTER_NEAREST = 0,
+    EDXR_FILTER_LINEAR = 1,     /* bilinear, level 0 */
+    EDXR_FILTER_TRILINEAR = 2   /* default (RenderStates::DefaultSettings) */
 };
 
 /* Pixel-shader parameters.  The reference values are ambient 0.2, kd 3,
@@ -90,7 +102,9 @@ enum edxr_state_key {
     /* Renderer::SetWriteFrames (Core/Renderer.h:50): after every render_mesh
      * write <frame directory>/Frames/Frame%05i.bmp, numbered by the frame
      * counter (Core/Renderer.cpp:114-117, 352-358).  Default off. */
-    EDXR_STATE_WRITE_FRAMES = 9
+    EDXR_STATE_WRITE_FRAMES = 9,
+    /* Renderer::SetTextureFilter: EDXR_FILTER_*; default trilinear. */
+    EDXR_STATE_TEXTURE_FILTER = 10
 };
 
 /* Pipeline stages timed when EDXR_STATE_PROFILE is on. */
@@ -107,7 +121,10 @@ enum edxr_memory { EDXR_MEM_HOST = 0, EDXR_MEM_DEVICE = 1 };
 
 /* One mesh (Utils/Mesh.h:17-69, Utils/InputBuffer.h:43-194), SoA fp32.
  * Positions, normals and indices are required; colours may be NULL except for
- * EDXR_SHADER_VC_BLINN_PHONG (they are then read as (0,0,0)). */
+ * EDXR_SHADER_VC_BLINN_PHONG (they are then read as (0,0,0)).  Texture
+ * coordinates and per-triangle texture slots (Mesh::GetTextureIds) are
+ * needed by EDXR_SHADER_TEXTURED_LAMBERT only; they follow `memory` so that
+ * initialisers of the earlier fields leave them NULL. */
 typedef struct edxr_mesh {
     const float *px, *py, *pz;
     const float *nx, *ny, *nz;
@@ -116,7 +133,21 @@ typedef struct edxr_mesh {
     uint32_t num_vertices;
     uint32_t num_triangles;
     int32_t memory;              /* EDXR_MEM_HOST or EDXR_MEM_DEVICE */
+    const float *tu, *tv;        /* texture coordinate per vertex */
+    const uint32_t *tex_ids;     /* texture slot per triangle */
 } edxr_mesh;
+
+/* One texture slot (Texture2D<Color>): an RGBA8 image, rows from v = 0 up,
+ * (ImageTexture<Color, Color4b>), or, with rgba NULL, a constant colour
+ * (ConstantTexture2D<Color>).  Texel (i, j) is centred at
+ * ((i + 0.5) / width, (j + 0.5) / height); coordinates wrap; a byte b reads
+ * as b / 255.  Mip levels halve each side (floor, at least 1) with 2x2 box
+ * averages (edge texels repeated for odd sizes). */
+typedef struct edxr_texture {
+    uint32_t width, height;
+    const uint8_t *rgba;         /* host memory, width * height * 4 bytes */
+    float color[3];
+} edxr_texture;
 
 /* Per-frame counters (filled after every render). */
 typedef struct edxr_stats {
@@ -254,6 +285,11 @@ int64_t edxr_debug_read_records(edxr_ctx* ctx, uint32_t* out, uint64_t cap);
  * {tri_id, sub, flags} (flags: bit0-2 accept edge, bit3 trivial, bit4 big).
  * Returns the reference count or a negative error. */
 int64_t edxr_debug_read_bins(edxr_ctx* ctx, uint64_t* offsets, uint32_t* refs3, uint64_t cap);
+
+/* Texture slots of the context (Mesh::GetTextures, bound by RenderMesh as
+ * RenderStates::TextureSlots): copied to the device with their mip chains.
+ * Replaces the previous set; count 0 clears it. */
+int edxr_set_textures(edxr_ctx* ctx, const edxr_texture* textures, uint32_t count);
 
 /* Directory that WriteFrameToFile writes under (Application::GetBaseDirectory
  * in the reference): frames go to <dir>/Frames/, created when missing.

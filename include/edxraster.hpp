@@ -117,7 +117,7 @@ class Mesh {
     uint32_t GetVertexCount() const { return m_.num_vertices; }
     uint32_t GetTriangleCount() const { return m_.num_triangles; }
     edxr_mesh View() const {
-        edxr_mesh v;
+        edxr_mesh v = {};  // texture coordinates and slots stay NULL
         v.px = m_.px; v.py = m_.py; v.pz = m_.pz;
         v.nx = m_.nx; v.ny = m_.ny; v.nz = m_.nz;
         v.cr = m_.cr; v.cg = m_.cg; v.cb = m_.cb;

@@ -164,6 +164,103 @@ static inline uint8_t ToByte(float c) {
     return (uint8_t)(int)(c * 255.0f + 0.5f);
 }
 
+/* ------------------------------------------------------------------ */
+/* Texture slots: Texture2D<Color>::Sample of EDXUtil (absent).  Declared   */
+/* semantics (parity unpinned against EDXUtil): texel (i, j) centred at     */
+/* ((i + 0.5) / w, (j + 0.5) / h); coordinates wrap; byte b reads b / 255;  */
+/* mip level k + 1 halves the sides (floor, at least 1), each texel the     */
+/* 0.25 * (((a + b) + c) + d) mean of its 2x2 parents (a side of 1 repeats); */
+/* float-to-index conversion maps NaN to 0 and clamps to +-2^30.            */
+/* ------------------------------------------------------------------ */
+struct TextureSlot {
+    bool constant = true;
+    float color[3] = {0.0f, 0.0f, 0.0f};
+    std::vector<std::vector<float>> levels;  /* RGBA floats per level */
+    std::vector<int> lw, lh;
+};
+
+static TextureSlot MakeSlot(const oracle_texture& t) {
+    TextureSlot s;
+    for (int k = 0; k < 3; k++) s.color[k] = t.color[k];
+    if (!t.rgba) return s;
+    s.constant = false;
+    int w = (int)t.width, h = (int)t.height;
+    std::vector<float> l0((size_t)w * h * 4);
+    for (size_t i = 0; i < l0.size(); i++) l0[i] = (float)t.rgba[i] / 255.0f;
+    s.levels.push_back(std::move(l0));
+    s.lw.push_back(w);
+    s.lh.push_back(h);
+    while (w > 1 || h > 1) {
+        const int w2 = std::max(1, w / 2), h2 = std::max(1, h / 2);
+        const std::vector<float>& a = s.levels.back();
+        std::vector<float> n((size_t)w2 * h2 * 4);
+        for (int y = 0; y < h2; y++)
+            for (int x = 0; x < w2; x++) {
+                const int x0 = 2 * x, x1 = std::min(2 * x + 1, w - 1), y0 = 2 * y, y1 = std::min(2 * y + 1, h - 1);
+                for (int ch = 0; ch < 4; ch++) {
+                    const float p00 = a[((size_t)y0 * w + x0) * 4 + ch], p10 = a[((size_t)y0 * w + x1) * 4 + ch];
+                    const float p01 = a[((size_t)y1 * w + x0) * 4 + ch], p11 = a[((size_t)y1 * w + x1) * 4 + ch];
+                    n[((size_t)y * w2 + x) * 4 + ch] = 0.25f * (((p00 + p10) + p01) + p11);
+                }
+            }
+        s.levels.push_back(std::move(n));
+        s.lw.push_back(w2);
+        s.lh.push_back(h2);
+        w = w2;
+        h = h2;
+    }
+    return s;
+}
+
+static inline int TexIndex(float x) {
+    if (x != x) return 0;
+    return (int)std::min(std::max(x, -1073741824.0f), 1073741823.0f);
+}
+
+static inline const float* Texel(const TextureSlot& s, int level, int i, int j) {
+    const int w = s.lw[level], h = s.lh[level];
+    i %= w;
+    if (i < 0) i += w;
+    j %= h;
+    if (j < 0) j += h;
+    return &s.levels[level][((size_t)j * w + i) * 4];
+}
+
+static Vec3 Bilinear(const TextureSlot& s, int level, float u, float v) {
+    const float x = u * (float)s.lw[level] - 0.5f, y = v * (float)s.lh[level] - 0.5f;
+    const float x0 = std::floor(x), y0 = std::floor(y);
+    const float fx = x - x0, fy = y - y0, gx = 1.0f - fx, gy = 1.0f - fy;
+    const int i = TexIndex(x0), j = TexIndex(y0);
+    const float *c00 = Texel(s, level, i, j), *c10 = Texel(s, level, i + 1, j);
+    const float *c01 = Texel(s, level, i, j + 1), *c11 = Texel(s, level, i + 1, j + 1);
+    float r[3];
+    for (int ch = 0; ch < 3; ch++)
+        r[ch] = gy * (gx * c00[ch] + fx * c10[ch]) + fy * (gx * c01[ch] + fx * c11[ch]);
+    return Vec3{r[0], r[1], r[2]};
+}
+
+/* Sample(texCoord, differentials[2]) with the state's filter. */
+static Vec3 SampleSlot(const TextureSlot& s, int filter, float u, float v, float dux, float dvx, float duy, float dvy) {
+    if (s.constant) return Vec3{s.color[0], s.color[1], s.color[2]};
+    const int w = s.lw[0], h = s.lh[0];
+    if (filter == ORACLE_FILTER_NEAREST) {
+        const float* c = Texel(s, 0, TexIndex(std::floor(u * (float)w)), TexIndex(std::floor(v * (float)h)));
+        return Vec3{c[0], c[1], c[2]};
+    }
+    if (filter == ORACLE_FILTER_LINEAR) return Bilinear(s, 0, u, v);
+    /* trilinear: lod = log2 of the longer texel-space differential */
+    const float ax = dux * (float)w, ay = dvx * (float)h, bx = duy * (float)w, by = dvy * (float)h;
+    const float rho2 = std::max(ax * ax + ay * ay, bx * bx + by * by);
+    float lod = rho2 > 0.0f ? 0.5f * std::log2(rho2) : 0.0f;
+    const int nl = (int)s.levels.size();
+    lod = std::min(std::max(lod, 0.0f), (float)(nl - 1));
+    const float fl = std::floor(lod);
+    const int l0 = (int)fl, l1 = std::min(l0 + 1, nl - 1);
+    const float t = lod - fl, g = 1.0f - t;
+    const Vec3 a = Bilinear(s, l0, u, v), b = Bilinear(s, l1, u, v);
+    return Vec3{g * a.x + t * b.x, g * a.y + t * b.y, g * a.z + t * b.z};
+}
+
 /* 28.4 snapping: RasterTriangle.h:35-40 converts x * 16.0 to int by
  * truncation.  The product is exact; out-of-range values saturate and NaN
  * gives 0 (declared: the reference has undefined behaviour there). */
@@ -310,11 +407,12 @@ class Pool {
 /* ------------------------------------------------------------------ */
 /* Pipeline types                                                        */
 /* ------------------------------------------------------------------ */
-/* Core/Shader.h:13-25; the UV pair is replaced by an RGB vertex colour. */
+/* Core/Shader.h:13-25, with an RGB vertex colour beside the UV pair. */
 struct ProjectedVertex {
     Vec4 projectedPos;
     float invW = 0.0f;
     Vec3 position, normal, color;
+    float u = 0.0f, v = 0.0f;
 };
 
 /* Core/RasterTriangle.h:11-24 plus the input triangle index and fan index. */
@@ -611,6 +709,8 @@ class Renderer {
     void Resolve();
 
     oracle_params p_{};
+    const uint32_t* texIds_ = nullptr;   /* Mesh::GetTextureIds (textured shader) */
+    std::vector<TextureSlot> slots_;     /* RenderStates::TextureSlots */
     Mat4 mvp_{}, mvInv_{}, raster_{};
     uint32_t W_ = 0, H_ = 0;
     Vec2i tileDim_{0, 0};
@@ -666,6 +766,8 @@ void Renderer::VertexProcessing(const oracle_mesh& m) {
         o.position = {m.px[i], m.py[i], m.pz[i]};
         o.normal = {m.nx[i], m.ny[i], m.nz[i]};
         o.color = {m.cr[i], m.cg[i], m.cb[i]};
+        if (m.tu) o.u = m.tu[i];
+        if (m.tv) o.v = m.tv[i];
         o.invW = 0.0f;
     });
 }
@@ -730,6 +832,8 @@ void Renderer::Clipping(const oracle_mesh& m) {
                             t.position = comb(a.position, b.position, c.position);
                             t.normal = comb(a.normal, b.normal, c.normal);
                             t.color = comb(a.color, b.color, c.color);
+                            t.u = (w.x * a.u + w.y * b.u) + w.z * c.u;
+                            t.v = (w.x * a.v + w.y * b.v) + w.z * c.v;
                             vb.push_back(t);
                             cs.clip_vertices++;
                         }
@@ -1114,6 +1218,8 @@ void Renderer::FragmentProcessing() {
         const Vec3SSE position = b0 * Splat(v0.position) + b1 * Splat(v1.position) + b2 * Splat(v2.position);
         const Vec3SSE normal = b0 * Splat(v0.normal) + b1 * Splat(v1.normal) + b2 * Splat(v2.normal);
         const Vec3SSE color = b0 * Splat(v0.color) + b1 * Splat(v1.color) + b2 * Splat(v2.color);
+        const FloatSSE tu = b0 * FloatSSE(v0.u) + b1 * FloatSSE(v1.u) + b2 * FloatSSE(v2.u);
+        const FloatSSE tv = b0 * FloatSSE(v0.v) + b1 * FloatSSE(v1.v) + b2 * FloatSSE(v2.v);
 
         /* Shade */
         FloatSSE w = Rsqrt(Dot(normal, normal));
@@ -1126,6 +1232,22 @@ void Renderer::FragmentProcessing() {
         Vec3SSE out;
         if (p_.shader == ORACLE_SHADER_LAMBERT_ALBEDO) {
             out = {diffuse * FloatSSE(p_.albedo[0]), diffuse * FloatSSE(p_.albedo[1]), diffuse * FloatSSE(p_.albedo[2])};
+        } else if (p_.shader == ORACLE_SHADER_TEXTURED_LAMBERT) {
+            /* Core/Shader.h:226-241: differentials from lanes 1 - 0 and 2 - 0,
+             * one Sample per lane of the fragment's texture slot. */
+            alignas(16) float u[4], v[4], a[3][4];
+            _mm_store_ps(u, tu.v);
+            _mm_store_ps(v, tv.v);
+            const TextureSlot& slot = slots_[texIds_[f.triId]];
+            for (int k = 0; k < 4; k++) {
+                const Vec3 c = SampleSlot(slot, p_.tex_filter, u[k], v[k], u[1] - u[0], v[1] - v[0], u[2] - u[0],
+                                          v[2] - v[0]);
+                a[0][k] = c.x;
+                a[1][k] = c.y;
+                a[2][k] = c.z;
+            }
+            out = {diffuse * FloatSSE(_mm_load_ps(a[0])), diffuse * FloatSSE(_mm_load_ps(a[1])),
+                   diffuse * FloatSSE(_mm_load_ps(a[2]))};
         } else {
             Vec3SSE eyeDir = Splat(eyePos) - position;
             w = Rsqrt(Dot(eyeDir, eyeDir));
@@ -1204,6 +1326,14 @@ int Renderer::Render(const oracle_params& p, const oracle_mesh& m, oracle_stats*
     for (uint64_t i = 0; i < 3ull * m.num_triangles; i++)
         if (m.indices[i] >= m.num_vertices) return -2;
     if (p.msaa_log2 < 0 || p.msaa_log2 > 5) return -1;
+    if (p.shader == ORACLE_SHADER_TEXTURED_LAMBERT) {
+        if (!m.tu || !m.tv || (m.num_triangles && !m.tex_ids) || !p.num_textures) return -1;
+        for (uint32_t t = 0; t < m.num_triangles; t++)
+            if (m.tex_ids[t] >= p.num_textures) return -2;
+        slots_.clear();
+        for (uint32_t k = 0; k < p.num_textures; k++) slots_.push_back(MakeSlot(p.textures[k]));
+        texIds_ = m.tex_ids;
+    }
     p_ = p;
     Initialize(p.width, p.height, p.num_threads);
     msLevel_ = p.msaa_log2;

@@ -22,8 +22,20 @@ extern "C" {
 enum {
     ORACLE_SHADER_LAMBERT_ALBEDO = 0, /* LambertianAlbedoPixelShader, constant texture */
     ORACLE_SHADER_BLINN_PHONG_REF = 1, /* BlinnPhongPixelShader (albedo 1) */
-    ORACLE_SHADER_VC_BLINN_PHONG = 2  /* vertex-colour Blinn-Phong */
+    ORACLE_SHADER_VC_BLINN_PHONG = 2, /* vertex-colour Blinn-Phong */
+    ORACLE_SHADER_TEXTURED_LAMBERT = 3 /* LambertianAlbedoPixelShader, texture slots */
 };
+
+enum { ORACLE_FILTER_NEAREST = 0, ORACLE_FILTER_LINEAR = 1, ORACLE_FILTER_TRILINEAR = 2 };
+
+/* A texture slot (Texture2D<Color>, EDXUtil, absent; the declared semantics
+ * of the library header, restated independently here): RGBA8 rows from v = 0,
+ * or with rgba NULL a constant colour.  Parity unpinned against EDXUtil. */
+typedef struct oracle_texture {
+    uint32_t width, height;
+    const uint8_t* rgba;
+    float color[3];
+} oracle_texture;
 
 typedef struct oracle_params {
     uint32_t width, height;
@@ -38,6 +50,9 @@ typedef struct oracle_params {
     int32_t num_threads;   /* 0 = hardware concurrency */
     int32_t mutation;      /* 0; non-zero plants a known mistake (pinning tests) */
     int32_t msaa_log2;     /* Renderer::SetMSAAMode: 0 (1 sample) .. 5 (32 samples) */
+    const oracle_texture* textures; /* texture slots (textured shader) */
+    uint32_t num_textures;
+    int32_t tex_filter;    /* ORACLE_FILTER_* */
 } oracle_params;
 
 typedef struct oracle_mesh {
@@ -46,6 +61,8 @@ typedef struct oracle_mesh {
     const float *cr, *cg, *cb;
     const uint32_t *indices; /* 3 per triangle */
     uint32_t num_vertices, num_triangles;
+    const float *tu, *tv;    /* texture coordinates (textured shader) */
+    const uint32_t *tex_ids; /* texture slot per triangle (textured shader) */
 } oracle_mesh;
 
 typedef struct oracle_stats {

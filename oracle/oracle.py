@@ -14,7 +14,12 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 F = C.POINTER(C.c_float)
 U32 = C.POINTER(C.c_uint32)
 
-SHADER_LAMBERT_ALBEDO, SHADER_BLINN_PHONG_REF, SHADER_VC_BLINN_PHONG = 0, 1, 2
+SHADER_LAMBERT_ALBEDO, SHADER_BLINN_PHONG_REF, SHADER_VC_BLINN_PHONG, SHADER_TEXTURED_LAMBERT = 0, 1, 2, 3
+FILTER_NEAREST, FILTER_LINEAR, FILTER_TRILINEAR = 0, 1, 2
+
+
+class TextureC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.c_void_p), ("color", C.c_float * 3)]
 
 
 class Params(C.Structure):
@@ -22,12 +27,14 @@ class Params(C.Structure):
                 ("raster", C.c_float * 16), ("light", C.c_float * 3), ("shader", C.c_int32),
                 ("shininess", C.c_float), ("ambient", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
                 ("albedo", C.c_float * 3), ("clamp_specular", C.c_int32), ("hierarchical", C.c_int32),
-                ("num_threads", C.c_int32), ("mutation", C.c_int32), ("msaa_log2", C.c_int32)]
+                ("num_threads", C.c_int32), ("mutation", C.c_int32), ("msaa_log2", C.c_int32),
+                ("textures", C.POINTER(TextureC)), ("num_textures", C.c_uint32), ("tex_filter", C.c_int32)]
 
 
 class MeshC(C.Structure):
     _fields_ = [("px", F), ("py", F), ("pz", F), ("nx", F), ("ny", F), ("nz", F), ("cr", F), ("cg", F),
-                ("cb", F), ("indices", U32), ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32)]
+                ("cb", F), ("indices", U32), ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32),
+                ("tu", F), ("tv", F), ("tex_ids", U32)]
 
 
 class Stats(C.Structure):
@@ -99,7 +106,7 @@ class Oracle:
 
     def render(self, mesh, width, height, mv, proj, raster, light=(1.0, 1.0, -1.0), shader=SHADER_VC_BLINN_PHONG,
                shininess=32.0, ambient=0.2, kd=3.0, ks=3.0, albedo=(0.9, 0.9, 0.9), clamp_specular=0,
-               hierarchical=1, threads=0, mutation=0, msaa_log2=0):
+               hierarchical=1, threads=0, mutation=0, msaa_log2=0, textures=(), tex_filter=FILTER_TRILINEAR):
         p = Params()
         p.width, p.height = width, height
         for name, m in (("mv", mv), ("proj", proj), ("raster", raster)):
@@ -109,11 +116,26 @@ class Oracle:
         p.albedo[:] = [float(v) for v in albedo]
         p.clamp_specular, p.hierarchical, p.num_threads, p.mutation = clamp_specular, hierarchical, threads, mutation
         p.msaa_log2 = msaa_log2
+        # texture slots: (H, W, 4) uint8 images or RGB colour triples
+        tex = (TextureC * max(1, len(textures)))()
+        imgs = []
+        for k, t in enumerate(textures):
+            if isinstance(t, np.ndarray):
+                img = np.ascontiguousarray(t, dtype=np.uint8)
+                imgs.append(img)
+                tex[k].width, tex[k].height, tex[k].rgba = img.shape[1], img.shape[0], img.ctypes.data
+            else:
+                tex[k].color[:] = [float(c) for c in t]
+        p.textures, p.num_textures, p.tex_filter = tex, len(textures), tex_filter
         planes = [_f(a) for a in (mesh.px, mesh.py, mesh.pz, mesh.nx, mesh.ny, mesh.nz, mesh.cr, mesh.cg, mesh.cb)]
         idx = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
-        self._keep = (planes, idx)
+        uv = [None if getattr(mesh, n, None) is None else _f(getattr(mesh, n)) for n in ("tu", "tv")]
+        tid = getattr(mesh, "tex_ids", None)
+        tid = None if tid is None else np.ascontiguousarray(tid, dtype=np.uint32)
+        self._keep = (planes, idx, uv, tid, tex, imgs)
         m = MeshC(*[a.ctypes.data_as(F) for a in planes], idx.ctypes.data_as(U32), planes[0].shape[0],
-                  idx.shape[0] // 3)
+                  idx.shape[0] // 3, *[None if a is None else a.ctypes.data_as(F) for a in uv],
+                  None if tid is None else tid.ctypes.data_as(U32))
         st = Stats()
         rc = lib().oracle_render(self._ctx, C.byref(p), C.byref(m), C.byref(st))
         if rc != 0:
@@ -123,6 +145,9 @@ class Oracle:
 
     def render_workload(self, w, **kw):
         m = w.material()
+        if getattr(w, "textures", None) is not None:
+            kw.setdefault("textures", w.textures)
+            kw.setdefault("tex_filter", w.tex_filter)
         return self.render(w.mesh, w.width, w.height, w.mv, w.proj, w.raster, light=w.light, shader=w.shader,
                            shininess=m.shininess, ambient=m.ambient, kd=m.kd, ks=m.ks, albedo=tuple(m.albedo),
                            clamp_specular=m.clamp_specular, **kw)
