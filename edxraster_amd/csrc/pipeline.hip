@@ -178,10 +178,6 @@ __global__ __launch_bounds__(kBlock) void k_vertex(BatchParams Q, const FramePar
 }
 
 namespace {
-__device__ __forceinline__ float4 vertex_pos(const ShadeVert* verts, uint32_t v) {
-    return *reinterpret_cast<const float4*>(&verts[v].px);  // (px, py, pz, nx)
-}
-
 // Descriptor of an unclipped triangle's record in slot t.  When its clamped
 // tile bbox spans at most 2x2 tiles, binning lists it in every bbox tile with
 // no corner test (Core/Renderer.cpp:168-180), so the bbox alone (packed here,
@@ -361,11 +357,13 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
         const FrameParams& P = jobs[Q.num_jobs == 1 ? 0u : Q.sblock_job[t / kBlock]];
         const uint32_t* idx = P.mesh.idx + 3 * (size_t)(t - P.slot_base);
         const uint32_t vi[3] = {P.vert_base + idx[0], P.vert_base + idx[1], P.vert_base + idx[2]};
+        // The parents' shading values, read once with their positions (inside the
+        // vertex loop below they would be re-read after every vertex store).
+        const ShadeVert pa = B.verts[vi[0]], pb = B.verts[vi[1]], pc = B.verts[vi[2]];
         float4 c[3];
-        for (int k = 0; k < 3; k++) {
-            const float4 p = vertex_pos(B.verts, vi[k]);
-            c[k] = xform4(P.mvp, p.x, p.y, p.z, 1.0f);
-        }
+        c[0] = xform4(P.mvp, pa.px, pa.py, pa.pz, 1.0f);
+        c[1] = xform4(P.mvp, pb.px, pb.py, pb.pz, 1.0f);
+        c[2] = xform4(P.mvp, pc.px, pc.py, pc.pz, 1.0f);
         const uint32_t k0 = clip_code(c[0]), k1 = clip_code(c[1]), k2 = clip_code(c[2]);
         const uint32_t planeCode = (k0 ^ k1) | (k1 ^ k2) | (k2 ^ k0);
         atomicAdd(&B.counters[kCntClipped], 1u);
@@ -451,9 +449,9 @@ __global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* _
                 const uint32_t xi = vbase + (uint32_t)e++;
                 vid = Q.total_verts + xi;
                 pos = sPos[cur][j][lane];
-                const ShadeVert& a = B.verts[vi[0]];
-                const ShadeVert& b = B.verts[vi[1]];
-                const ShadeVert& cc = B.verts[vi[2]];
+                const ShadeVert& a = pa;
+                const ShadeVert& b = pb;
+                const ShadeVert& cc = pc;
                 auto comb = [&](float fa, float fb, float fc) { return (w.x * fa + w.y * fb) + w.z * fc; };
                 ShadeVert o;
                 o.px = comb(a.px, b.px, cc.px);
