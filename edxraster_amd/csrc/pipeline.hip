@@ -702,15 +702,25 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
 // wave a quarter of the group's chunks), so every step has C/64 x tiles/64
 // blocks per job (grid z = job; blocks beyond a smaller job's matrix exit).
 namespace {
+constexpr uint32_t kWaveChunks = kChunkGroup / (kBlock / kWave);  // chunks of a group per wave
+
+// The wave's counts of one tile column (all loads issued before any use).
+__device__ __forceinline__ void colgroup_load(const FrameParams& P, const DeviceBuffers& B, uint32_t tile,
+                                              uint32_t c0, uint32_t c1, uint32_t (&v)[kWaveChunks]) {
+    const bool in = tile < P.num_tiles;
+#pragma unroll
+    for (uint32_t k = 0; k < kWaveChunks; k++)
+        v[k] = in && c0 + k < c1 ? B.bin_counts[P.count_base + (size_t)(c0 + k) * P.num_tiles + tile] : 0u;
+}
+
 // Per-wave partial sums of one (64-tile, chunk-group) block, combined in LDS;
 // returns this wave's exclusive base within the group and the group total.
-__device__ __forceinline__ void colgroup_sums(const FrameParams& P, const DeviceBuffers& B, uint32_t tile,
-                                              uint32_t c0, uint32_t c1, uint32_t (*part)[kWave], uint32_t& waveBase,
-                                              uint32_t& groupTotal) {
+__device__ __forceinline__ void colgroup_sums(const uint32_t (&v)[kWaveChunks], uint32_t (*part)[kWave],
+                                              uint32_t& waveBase, uint32_t& groupTotal) {
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     uint32_t sum = 0;
-    if (tile < P.num_tiles)
-        for (uint32_t c = c0; c < c1; c++) sum += B.bin_counts[P.count_base + (size_t)c * P.num_tiles + tile];
+#pragma unroll
+    for (uint32_t k = 0; k < kWaveChunks; k++) sum += v[k];
     part[wave][lane] = sum;
     __syncthreads();
     waveBase = 0;
@@ -721,9 +731,9 @@ __device__ __forceinline__ void colgroup_sums(const FrameParams& P, const Device
     }
 }
 __device__ __forceinline__ void colgroup_range(const FrameParams& P, uint32_t& c0, uint32_t& c1) {
-    const uint32_t wave = threadIdx.x / kWave, per = kChunkGroup / (kBlock / kWave);
-    c0 = min(P.num_chunks, blockIdx.y * kChunkGroup + wave * per);
-    c1 = min(P.num_chunks, c0 + per);
+    const uint32_t wave = threadIdx.x / kWave;
+    c0 = min(P.num_chunks, blockIdx.y * kChunkGroup + wave * kWaveChunks);
+    c1 = min(P.num_chunks, c0 + kWaveChunks);
 }
 }  // namespace
 
@@ -739,9 +749,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_colsum(BatchParams Q, const Fram
     const FrameParams& P = jobs[blockIdx.z];
     if (!colgroup_in_job(P)) return;
     const uint32_t tile = blockIdx.x * kWave + __lane_id();
-    uint32_t c0, c1, waveBase, groupTotal;
+    uint32_t c0, c1, waveBase, groupTotal, v[kWaveChunks];
     colgroup_range(P, c0, c1);
-    colgroup_sums(P, B, tile, c0, c1, part, waveBase, groupTotal);
+    colgroup_load(P, B, tile, c0, c1, v);
+    colgroup_sums(v, part, waveBase, groupTotal);
     if (threadIdx.x < kWave && tile < P.num_tiles)
         B.group_sums[P.group_base + (size_t)blockIdx.y * P.num_tiles + tile] = groupTotal;
 }
@@ -753,12 +764,19 @@ __global__ __launch_bounds__(kBlock) void k_bin_groupscan(BatchParams Q, const F
     const uint32_t tile = blockIdx.x * kBlock + threadIdx.x;
     if (tile >= P.num_tiles) return;
     const uint32_t G = (P.num_chunks + kChunkGroup - 1) / kChunkGroup;
+    uint32_t* col = &B.group_sums[P.group_base + tile];
     uint32_t run = 0;
-    for (uint32_t g = 0; g < G; g++) {
-        uint32_t* p = &B.group_sums[P.group_base + (size_t)g * P.num_tiles + tile];
-        const uint32_t v = *p;
-        *p = run;
-        run += v;
+    // Batches of 16 groups: the loads of a batch are issued before its stores.
+    constexpr uint32_t kBatch = 16;
+    for (uint32_t g0 = 0; g0 < G; g0 += kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; k++) v[k] = g0 + k < G ? col[(size_t)(g0 + k) * P.num_tiles] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; k++) {
+            if (g0 + k < G) col[(size_t)(g0 + k) * P.num_tiles] = run;
+            run += v[k];
+        }
     }
     B.tile_totals[P.tile_base + tile] = run;
 }
@@ -770,16 +788,16 @@ __global__ __launch_bounds__(kBlock) void k_bin_colscan(BatchParams Q, const Fra
     const FrameParams& P = jobs[blockIdx.z];
     if (!colgroup_in_job(P)) return;
     const uint32_t tile = blockIdx.x * kWave + __lane_id();
-    uint32_t c0, c1, waveBase, groupTotal;
+    uint32_t c0, c1, waveBase, groupTotal, v[kWaveChunks];
     colgroup_range(P, c0, c1);
-    colgroup_sums(P, B, tile, c0, c1, part, waveBase, groupTotal);
+    colgroup_load(P, B, tile, c0, c1, v);
+    colgroup_sums(v, part, waveBase, groupTotal);
     if (tile >= P.num_tiles) return;
     uint32_t run = B.group_sums[P.group_base + (size_t)blockIdx.y * P.num_tiles + tile] + waveBase;
-    for (uint32_t c = c0; c < c1; c++) {
-        const size_t o = P.count_base + (size_t)c * P.num_tiles + tile;
-        const uint32_t v = B.bin_counts[o];
-        B.bin_counts[o] = run;
-        run += v;
+#pragma unroll
+    for (uint32_t k = 0; k < kWaveChunks; k++) {
+        if (c0 + k < c1) B.bin_counts[P.count_base + (size_t)(c0 + k) * P.num_tiles + tile] = run;
+        run += v[k];
     }
 }
 
