@@ -842,11 +842,26 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     const uint32_t t0 = min(nt, threadIdx.x * per), t1 = min(nt, t0 + per);
     if (threadIdx.x == 0) sMax = 0;
     __syncthreads();
+    // Up to 16 tiles per thread (16384 tiles) the totals are read once, all
+    // loads in flight together, and kept in registers for the later passes.
+    constexpr uint32_t kCache = 16;
+    const bool cached = per <= kCache;
+    uint32_t cache[kCache];
     uint32_t localRefs = 0, localMax = 0;
-    for (uint32_t t = t0; t < t1; t++) {
-        const uint32_t n = B.tile_totals[t];
-        localRefs += n;
-        localMax = max(localMax, n);
+    if (cached) {
+#pragma unroll
+        for (uint32_t k = 0; k < kCache; k++) cache[k] = t0 + k < t1 ? B.tile_totals[t0 + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kCache; k++) {
+            localRefs += cache[k];
+            localMax = max(localMax, cache[k]);
+        }
+    } else {
+        for (uint32_t t = t0; t < t1; t++) {
+            const uint32_t n = B.tile_totals[t];
+            localRefs += n;
+            localMax = max(localMax, n);
+        }
     }
     atomicMax(&sMax, localMax);
     uint32_t total;
@@ -858,12 +873,16 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     while (segLog2 < kSegLog2Max && tile_segments(maxN, segLog2) > (1u << (32 - kTileBits))) segLog2++;
     const bool segsFit = tile_segments(maxN, segLog2) <= (1u << (32 - kTileBits));
     uint32_t localSegs = 0;
-    for (uint32_t t = t0; t < t1; t++) localSegs += tile_segments(B.tile_totals[t], segLog2);
+    if (cached) {
+#pragma unroll
+        for (uint32_t k = 0; k < kCache; k++) localSegs += tile_segments(cache[k], segLog2);
+    } else {
+        for (uint32_t t = t0; t < t1; t++) localSegs += tile_segments(B.tile_totals[t], segLog2);
+    }
     uint32_t totalSegs;
     uint32_t seg = scan1024(localSegs, wsum, totalSegs);
     const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && segsFit;
-    for (uint32_t t = t0; t < t1; t++) {
-        const uint32_t n = B.tile_totals[t];
+    auto place = [&](uint32_t t, uint32_t n) {
         B.tile_start[t] = run;
         run += n;
         if (fits) {
@@ -872,6 +891,13 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
             for (uint32_t s = 0; s < ns; s++) B.work[seg + s] = t | (s << kTileBits);
             seg += ns;
         }
+    };
+    if (cached) {
+#pragma unroll
+        for (uint32_t k = 0; k < kCache; k++)
+            if (t0 + k < t1) place(t0 + k, cache[k]);
+    } else {
+        for (uint32_t t = t0; t < t1; t++) place(t, B.tile_totals[t]);
     }
     if (threadIdx.x == 0) {
         B.tile_start[nt] = total;
