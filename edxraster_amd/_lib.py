@@ -88,7 +88,12 @@ class Tuning(C.Structure):
 class HostMesh(C.Structure):
     _fields_ = [("px", F), ("py", F), ("pz", F), ("nx", F), ("ny", F), ("nz", F),
                 ("cr", F), ("cg", F), ("cb", F), ("indices", U32),
-                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32)]
+                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32),
+                ("tu", F), ("tv", F), ("tex_ids", U32)]
+
+
+class ObjMaterial(C.Structure):
+    _fields_ = [("name", C.c_char * 128), ("kd", C.c_float * 3), ("map_kd", C.c_char * 512)]
 
 
 # (name, restype, argtypes) for every symbol of include/edxraster.h.
@@ -130,6 +135,10 @@ SIGNATURES = [
     ("edxr_scene_free", None, [C.POINTER(HostMesh)]),
     ("edxr_load_obj", C.c_int, [C.c_char_p, F, F, F, C.POINTER(HostMesh)]),
     ("edxr_write_bmp", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    ("edxr_obj_materials", C.c_int64, [C.c_char_p, C.POINTER(ObjMaterial), C.c_uint32]),
+    ("edxr_load_bmp", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.POINTER(C.c_uint8))]),
+    ("edxr_free", None, [C.c_void_p]),
     ("edxr_set_frame_directory", C.c_int, [C.c_void_p, C.c_char_p]),
     ("edxr_set_textures", C.c_int, [C.c_void_p, C.POINTER(Texture), C.c_uint32]),
     ("edxr_write_frame_to_file", C.c_int, [C.c_void_p]),

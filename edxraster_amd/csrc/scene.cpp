@@ -220,6 +220,11 @@ void edxr_scene_free(edxr_host_mesh* m) {
     }
     free(m->indices);
     m->indices = nullptr;
+    free(m->tu);
+    free(m->tv);
+    free(m->tex_ids);
+    m->tu = m->tv = nullptr;
+    m->tex_ids = nullptr;
     m->num_vertices = m->num_triangles = 0;
 }
 
@@ -260,14 +265,16 @@ void edxr_raster_matrix(uint32_t width, uint32_t height, float out[16]) {
 // EDXUtil's ObjMesh::LoadFromObj, which is not available.  Declared semantics:
 //   - 'v', 'vn', 'vt', 'f' (v, v/t, v//n, v/t/n; 1-based or negative
 //     relative indices), 'mtllib' and 'usemtl'; the .mtl file supplies
-//     'newmtl' and 'Kd' (the material colour, ConstantTexture2D in the
-//     reference); other statements are skipped;
+//     'newmtl', 'Kd' (the material colour, ConstantTexture2D in the
+//     reference) and 'map_Kd' (its image, ImageTexture); other statements
+//     are skipped;
 //   - a polygon face is split into a fan (0, i, i + 1), keeping the file's
 //     vertex order (front faces of a right-handed export are clockwise on
 //     screen in the left-handed, y-down raster space);
-//   - one output vertex per distinct (position, normal, material) triple;
-//     faces without normals get area-weighted vertex normals of their
-//     position (computed in double);
+//   - one output vertex per distinct (position, texture coordinate, normal,
+//     material); faces without normals get area-weighted vertex normals of
+//     their position (computed in double); missing texture coordinates are
+//     (0, 0);
 //   - world transform p' = T(pos) Rz Ry Rx S(scl) p (rotation in degrees),
 //     normals n' = normalize(Rz Ry Rx (n / scl));
 //   - vertex colour = the face material's Kd, (0.9, 0.9, 0.9) without one
@@ -278,6 +285,7 @@ namespace {
 struct ObjMaterial {
     std::string name;
     float kd[3] = {0.9f, 0.9f, 0.9f};
+    std::string map;  // diffuse map path as written in the library
 };
 
 bool read_file(const std::string& path, std::string& out) {
@@ -314,6 +322,7 @@ void for_each_line(const std::string& text, F f) {
 }
 
 void load_mtl(const std::string& path, std::vector<ObjMaterial>& mats) {
+    const std::string dir = dir_of(path);
     std::string text;
     if (!read_file(path, text)) return;  // a missing library leaves the default colour
     for_each_line(text, [&](const std::string& l) {
@@ -322,6 +331,9 @@ void load_mtl(const std::string& path, std::vector<ObjMaterial>& mats) {
         if (!strcmp(key, "newmtl") && sscanf(l.c_str(), "%*s %511s", name) == 1) {
             mats.push_back(ObjMaterial());
             mats.back().name = name;
+        } else if (!strcmp(key, "map_Kd") && !mats.empty()) {
+            char file[512] = {0};
+            if (sscanf(l.c_str(), "%*s %511s", file) == 1) mats.back().map = file[0] == '/' ? file : dir + file;
         } else if (!strcmp(key, "Kd") && !mats.empty()) {
             float r, g, b;
             if (sscanf(l.c_str(), "%*s %f %f %f", &r, &g, &b) == 3) {
@@ -355,9 +367,10 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
     std::string text;
     if (!read_file(path, text)) return EDXR_E_IO;
     std::vector<double> P, N;  // file positions and normals, xyz each
+    std::vector<float> UV;     // file texture coordinates, uv each
     std::vector<ObjMaterial> mats;
     struct Corner {
-        uint32_t p, n, m;  // n = UINT32_MAX: none
+        uint32_t p, t, n, m;  // t, n = UINT32_MAX: none
     };
     std::vector<Corner> corners;  // triangle corners, 3 per triangle
     uint32_t cur = UINT32_MAX;    // current material (UINT32_MAX: default)
@@ -377,6 +390,14 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
             d.push_back(x);
             d.push_back(y);
             d.push_back(z);
+        } else if (!strcmp(key, "vt")) {
+            float u, v = 0.0f;
+            if (sscanf(l.c_str(), "%*s %f %f", &u, &v) < 1) {
+                bad = true;
+                return;
+            }
+            UV.push_back(u);
+            UV.push_back(v);
         } else if (!strcmp(key, "mtllib")) {
             char name[512] = {0};
             if (sscanf(l.c_str(), "%*s %511s", name) == 1) load_mtl(dir + name, mats);
@@ -399,11 +420,15 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
                     return;
                 }
                 s = e;
-                long ni = 0;
+                long ni = 0, ti = 0;
                 if (*s == '/') {
                     s++;
                     if (*s != '/') {
-                        strtol(s, &e, 10);  // texture coordinate (unused)
+                        ti = strtol(s, &e, 10);
+                        if (e == s) {
+                            bad = true;
+                            return;
+                        }
                         s = e;
                     }
                     if (*s == '/') {
@@ -416,8 +441,9 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
                         s = e;
                     }
                 }
-                Corner c{0, UINT32_MAX, cur};
-                if (!obj_index(vi, P.size() / 3, c.p) || (ni && !obj_index(ni, N.size() / 3, c.n))) {
+                Corner c{0, UINT32_MAX, UINT32_MAX, cur};
+                if (!obj_index(vi, P.size() / 3, c.p) || (ni && !obj_index(ni, N.size() / 3, c.n)) ||
+                    (ti && !obj_index(ti, UV.size() / 2, c.t))) {
                     bad = true;
                     return;
                 }
@@ -458,14 +484,15 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             const Corner &x = corners[a], &y = corners[b];
             if (x.p != y.p) return x.p < y.p;
+            if (x.t != y.t) return x.t < y.t;
             if (x.n != y.n) return x.n < y.n;
             return x.m < y.m;
         });
         std::vector<uint32_t> rep(corners.size());
         for (size_t k = 0; k < order.size(); k++) {
             const Corner& c = corners[order[k]];
-            const bool same = k && corners[order[k - 1]].p == c.p && corners[order[k - 1]].n == c.n &&
-                              corners[order[k - 1]].m == c.m;
+            const Corner& d = corners[order[k ? k - 1 : 0]];
+            const bool same = k && d.p == c.p && d.t == c.t && d.n == c.n && d.m == c.m;
             rep[order[k]] = same ? rep[order[k - 1]] : order[k];
         }
         std::vector<uint32_t> idOf(corners.size(), UINT32_MAX);
@@ -477,10 +504,20 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
             vid[i] = idOf[rep[i]];
         }
     }
-    if (!alloc_mesh(out, first.size(), corners.size() / 3)) {
+    const size_t T = corners.size() / 3;
+    if (!alloc_mesh(out, first.size(), T)) {
         edxr_scene_free(out);
         return EDXR_E_NOMEM;
     }
+    out->tu = (float*)malloc(sizeof(float) * (first.size() ? first.size() : 1));
+    out->tv = (float*)malloc(sizeof(float) * (first.size() ? first.size() : 1));
+    out->tex_ids = (uint32_t*)malloc(sizeof(uint32_t) * (T ? T : 1));
+    if (!out->tu || !out->tv || !out->tex_ids) {
+        edxr_scene_free(out);
+        return EDXR_E_NOMEM;
+    }
+    for (size_t t = 0; t < T; t++)
+        out->tex_ids[t] = corners[3 * t].m != UINT32_MAX ? corners[3 * t].m : (uint32_t)mats.size();
     const double d2r = 3.14159265358979323846 / 180.0;
     const double cx = cos(rot[0] * d2r), sx = sin(rot[0] * d2r), cy = cos(rot[1] * d2r), sy = sin(rot[1] * d2r),
                  cz = cos(rot[2] * d2r), sz = sin(rot[2] * d2r);
@@ -513,6 +550,8 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
         out->cr[v] = kd[0];
         out->cg[v] = kd[1];
         out->cb[v] = kd[2];
+        out->tu[v] = c.t != UINT32_MAX ? UV[2 * c.t] : 0.0f;
+        out->tv[v] = c.t != UINT32_MAX ? UV[2 * c.t + 1] : 0.0f;
     }
     for (size_t i = 0; i < corners.size(); i++) out->indices[i] = vid[i];
     return EDXR_OK;
@@ -559,3 +598,62 @@ int edxr_write_bmp(const char* path, const uint8_t* rgba, uint32_t width, uint32
     ok = (fclose(f) == 0) && ok;
     return ok ? EDXR_OK : EDXR_E_IO;
 }
+
+int64_t edxr_obj_materials(const char* path, edxr_obj_material* out, uint32_t cap) {
+    if (!path || (cap && !out)) return EDXR_E_INVALID;
+    std::string text;
+    if (!read_file(path, text)) return EDXR_E_IO;
+    std::vector<ObjMaterial> mats;
+    const std::string dir = dir_of(path);
+    for_each_line(text, [&](const std::string& l) {
+        char key[64] = {0}, name[512] = {0};
+        if (sscanf(l.c_str(), "%63s %511s", key, name) == 2 && !strcmp(key, "mtllib")) load_mtl(dir + name, mats);
+    });
+    for (size_t i = 0; i < mats.size() && i < cap; i++) {
+        memset(&out[i], 0, sizeof(out[i]));
+        strncpy(out[i].name, mats[i].name.c_str(), sizeof(out[i].name) - 1);
+        for (int k = 0; k < 3; k++) out[i].kd[k] = mats[i].kd[k];
+        strncpy(out[i].map_kd, mats[i].map.c_str(), sizeof(out[i].map_kd) - 1);
+    }
+    return (int64_t)mats.size();
+}
+
+// Uncompressed BMP input (BITMAPINFOHEADER or later, BI_RGB, 24 or 32 bits,
+// bottom-up rows as stored; a negative height means top-down rows, which are
+// reversed so that the result is bottom-up in every case).
+int edxr_load_bmp(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba) {
+    if (!path || !width || !height || !rgba) return EDXR_E_INVALID;
+    *rgba = nullptr;
+    std::string d;
+    if (!read_file(path, d)) return EDXR_E_IO;
+    auto u16 = [&](size_t at) { return (uint32_t)(uint8_t)d[at] | ((uint32_t)(uint8_t)d[at + 1] << 8); };
+    auto u32 = [&](size_t at) { return u16(at) | (u16(at + 2) << 16); };
+    if (d.size() < 54 || d[0] != 'B' || d[1] != 'M') return EDXR_E_FORMAT;
+    const uint32_t off = u32(10), hdr = u32(14), bpp = u16(28), comp = u32(30);
+    const int32_t w = (int32_t)u32(18), h = (int32_t)u32(22);
+    if (hdr < 40 || u16(26) != 1 || (bpp != 24 && bpp != 32) || comp != 0 || w <= 0 || h == 0 || w > 16384 ||
+        h > 16384 || h < -16384)
+        return EDXR_E_FORMAT;
+    const uint32_t H = (uint32_t)(h < 0 ? -h : h), W = (uint32_t)w;
+    const size_t row = ((size_t)W * (bpp / 8) + 3) & ~(size_t)3;
+    if ((size_t)off + row * H > d.size()) return EDXR_E_FORMAT;
+    uint8_t* img = (uint8_t*)malloc(4ull * W * H);
+    if (!img) return EDXR_E_NOMEM;
+    for (uint32_t y = 0; y < H; y++) {
+        const uint8_t* src = (const uint8_t*)d.data() + off + row * (h < 0 ? H - 1 - y : y);
+        for (uint32_t x = 0; x < W; x++) {
+            const uint8_t* p = src + (size_t)x * (bpp / 8);
+            uint8_t* q = img + 4ull * ((size_t)y * W + x);
+            q[0] = p[2];
+            q[1] = p[1];
+            q[2] = p[0];
+            q[3] = bpp == 32 ? p[3] : 255;
+        }
+    }
+    *width = W;
+    *height = H;
+    *rgba = img;
+    return EDXR_OK;
+}
+
+void edxr_free(void* p) { free(p); }

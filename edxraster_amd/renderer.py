@@ -26,6 +26,17 @@ def _mat(m):
     return a
 
 
+def load_bmp(path):
+    """(H, W, 4) uint8 RGBA of an uncompressed 24/32-bit BMP, rows bottom-up (v = 0 first)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    p = C.POINTER(C.c_uint8)()
+    _check(None, L.lib().edxr_load_bmp(os.fsencode(path), C.byref(w), C.byref(h), C.byref(p)))
+    try:
+        return np.ctypeslib.as_array(p, shape=(h.value, w.value, 4)).copy()
+    finally:
+        L.lib().edxr_free(p)
+
+
 class Mesh:
     """SoA mesh: positions, normals, RGB colours (float32, V each) and uint32
     indices (3T); optionally texture coordinates tu, tv (V each) and a texture
@@ -69,8 +80,14 @@ class Mesh:
             p = getattr(hm, name)
             planes.append(np.ctypeslib.as_array(p, shape=(V,)).copy() if V else np.zeros(0, np.float32))
         idx = np.ctypeslib.as_array(hm.indices, shape=(3 * T,)).copy() if T else np.zeros(0, np.uint32)
+        extra = {}
+        for name, n in (("tu", V), ("tv", V), ("tex_ids", T)):
+            p = getattr(hm, name)
+            if p:
+                extra[name] = np.ctypeslib.as_array(p, shape=(n,)).copy() if n else \
+                    np.zeros(0, np.uint32 if name == "tex_ids" else np.float32)
         L.lib().edxr_scene_free(C.byref(hm))
-        return Mesh(*planes, idx)
+        return Mesh(*planes, idx, **extra)
 
     # Mesh::LoadSphere (EDXRaster/Utils/Mesh.cpp:53-70)
     @staticmethod
@@ -95,6 +112,26 @@ class Mesh:
         p, s, r = (np.asarray(v, dtype=np.float32) for v in (pos, scl, rot))
         _check(None, L.lib().edxr_load_obj(os.fsencode(path), L.fptr(p), L.fptr(s), L.fptr(r), C.byref(hm)))
         return Mesh._from_host(hm)
+
+    # The material list Mesh::LoadMesh builds (Utils/Mesh.cpp:22-31): per
+    # material (name, Kd, diffuse map path or "").
+    @staticmethod
+    def obj_materials(path):
+        n = L.lib().edxr_obj_materials(os.fsencode(path), None, 0)
+        _check(None, n if n < 0 else 0)
+        arr = (L.ObjMaterial * max(1, n))()
+        L.lib().edxr_obj_materials(os.fsencode(path), arr, n)
+        return [(m.name.decode(), tuple(m.kd), m.map_kd.decode()) for m in arr[:n]]
+
+    # The texture slots of an OBJ's triangles (tex_ids): ImageTexture for a
+    # material with a .bmp diffuse map, ConstantTexture2D(Kd) otherwise, and a
+    # last 0.9-grey slot for faces without a material.
+    @staticmethod
+    def obj_texture_slots(path):
+        slots = []
+        for name, kd, mp in Mesh.obj_materials(path):
+            slots.append(load_bmp(mp) if mp.lower().endswith(".bmp") and os.path.exists(mp) else kd)
+        return slots + [(0.9, 0.9, 0.9)]
 
     @staticmethod
     def sphere_grid(grid=16, segments=64, seed=2):

@@ -307,6 +307,8 @@ typedef struct edxr_host_mesh {
     float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb;
     uint32_t* indices;
     uint32_t num_vertices, num_triangles;
+    float *tu, *tv;        /* texture coordinates (OBJ input; NULL for generated scenes) */
+    uint32_t* tex_ids;     /* material slot per triangle (OBJ input; NULL otherwise) */
 } edxr_host_mesh;
 
 /* UV sphere: (stacks+1)*(slices+1) vertices, 2*slices*stacks triangles,
@@ -324,13 +326,33 @@ void edxr_scene_free(edxr_host_mesh* m);
 /* Mesh::LoadMesh (Utils/Mesh.cpp:11-34): a Wavefront OBJ file (with its .mtl
  * colours) placed by p' = T(pos) Rz Ry Rx S(scl) p, rotation in degrees.
  * Polygons are fanned in file order; one vertex per distinct (position,
- * normal, material); missing normals are area-weighted; colour = the
- * material's Kd, 0.9 grey without one.  Free with edxr_scene_free. */
+ * texture coordinate, normal, material); missing normals are area-weighted;
+ * missing texture coordinates are (0, 0); colour = the material's Kd, 0.9
+ * grey without one.  tex_ids holds each triangle's material slot in
+ * edxr_obj_materials order, or the material count for faces without one.
+ * Free with edxr_scene_free. */
 int edxr_load_obj(const char* path, const float pos[3], const float scl[3], const float rot[3],
                   edxr_host_mesh* out);
 /* Bitmap::SaveBitmapFile: 24-bit uncompressed BMP of a bottom-up RGBA8 image
  * (rows kept in order, BGR bytes, alpha dropped). */
 int edxr_write_bmp(const char* path, const uint8_t* rgba, uint32_t width, uint32_t height);
+
+/* One material of an OBJ's libraries (Mesh::LoadMesh's material list): the
+ * diffuse colour (ConstantTexture2D) and the diffuse map path (ImageTexture),
+ * resolved against the .obj file's directory, or "" for none. */
+typedef struct edxr_obj_material {
+    char name[128];
+    float kd[3];
+    char map_kd[512];
+} edxr_obj_material;
+/* The materials of an OBJ file's mtllib statements in slot order.  Returns
+ * the material count (at most cap are written) or a negative error. */
+int64_t edxr_obj_materials(const char* path, edxr_obj_material* out, uint32_t cap);
+/* Reads an uncompressed 24- or 32-bit BMP (the image input of ImageTexture
+ * for .bmp diffuse maps) as RGBA8 with its rows in file order (bottom-up,
+ * which is v = 0 first).  *rgba is allocated here: release with edxr_free. */
+int edxr_load_bmp(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba);
+void edxr_free(void* p);
 /* Left-handed look-at view matrix and D3D-style projection (z in [0,1]),
  * raster matrix x_r = (x+1)W/2, y_r = (1-y)H/2.  Computed in double. */
 void edxr_look_at(const float eye[3], const float at[3], const float up[3], float out[16]);

@@ -80,6 +80,13 @@ class BlinnPhongPixelShader : public PixelShader {
     BlinnPhongPixelShader() : PixelShader(EDXR_SHADER_BLINN_PHONG_REF) {}
 };
 
+/* LambertianAlbedoPixelShader sampling the mesh's texture slots
+ * (Core/Shader.h:209-244), as the reference binds it to loaded meshes. */
+class TexturedLambertianPixelShader : public PixelShader {
+  public:
+    TexturedLambertianPixelShader() : PixelShader(EDXR_SHADER_TEXTURED_LAMBERT) {}
+};
+
 /* Vertex-colour Blinn-Phong: diffuse * interpolated RGB + specular. */
 class VertexColorBlinnPhongPixelShader : public PixelShader {
   public:
@@ -106,13 +113,50 @@ class Mesh {
         Release();
         check(edxr_scene_heightfield(&m_, n, amplitude, heightSeed, colorSeed));
     }
-    /* Mesh::LoadMesh (Utils/Mesh.cpp:11-34): a Wavefront OBJ (+ .mtl colours)
-     * placed by translation pos, rotation rot (degrees) and scale scl. */
+    /* Mesh::LoadMesh (Utils/Mesh.cpp:11-34): a Wavefront OBJ placed by
+     * translation pos, rotation rot (degrees) and scale scl, with its texture
+     * slots: an image texture for a material with a .bmp diffuse map, a
+     * constant colour (Kd) otherwise, and a last 0.9-grey slot for faces
+     * without a material. */
     void LoadMesh(const float pos[3], const float scl[3], const float rot[3], const char* path) {
         Release();
         check(edxr_load_obj(path, pos, scl, rot, &m_));
+        const int64_t n = edxr_obj_materials(path, nullptr, 0);
+        check(n < 0 ? (int)n : EDXR_OK);
+        std::vector<edxr_obj_material> mats((size_t)n);
+        if (n) edxr_obj_materials(path, mats.data(), (uint32_t)n);
+        for (const edxr_obj_material& mt : mats) {
+            edxr_texture t = {};
+            const std::string map = mt.map_kd;
+            const bool bmp = map.size() > 4 && (map.substr(map.size() - 4) == ".bmp" || map.substr(map.size() - 4) == ".BMP");
+            uint8_t* img = nullptr;
+            if (bmp && edxr_load_bmp(map.c_str(), &t.width, &t.height, &img) == EDXR_OK) {
+                images_.push_back(img);
+                t.rgba = img;
+            } else {
+                t.color[0] = mt.kd[0];
+                t.color[1] = mt.kd[1];
+                t.color[2] = mt.kd[2];
+            }
+            textures_.push_back(t);
+        }
+        edxr_texture grey = {};
+        grey.color[0] = grey.color[1] = grey.color[2] = 0.9f;
+        textures_.push_back(grey);
+        static uint64_t loads = 0;
+        textureSetId_ = ++loads;
     }
-    void Release() { edxr_scene_free(&m_); }
+    void Release() {
+        textureSetId_ = 0;
+        edxr_scene_free(&m_);
+        for (uint8_t* p : images_) edxr_free(p);
+        images_.clear();
+        textures_.clear();
+    }
+    /* Mesh::GetTextures: the texture slots tex_ids refer to (empty for generated meshes). */
+    const std::vector<edxr_texture>& GetTextures() const { return textures_; }
+    /* Identifies the texture set of the last LoadMesh (0: none). */
+    uint64_t TextureSetId() const { return textureSetId_; }
 
     uint32_t GetVertexCount() const { return m_.num_vertices; }
     uint32_t GetTriangleCount() const { return m_.num_triangles; }
@@ -125,6 +169,9 @@ class Mesh {
         v.num_vertices = m_.num_vertices;
         v.num_triangles = m_.num_triangles;
         v.memory = EDXR_MEM_HOST;
+        v.tu = m_.tu;
+        v.tv = m_.tv;
+        v.tex_ids = m_.tex_ids;
         return v;
     }
 
@@ -133,6 +180,9 @@ class Mesh {
         if (rc != EDXR_OK) throw std::runtime_error("edxraster: mesh generation failed (" + std::to_string(rc) + ")");
     }
     edxr_host_mesh m_;
+    std::vector<edxr_texture> textures_;
+    std::vector<uint8_t*> images_;
+    uint64_t textureSetId_ = 0;
 };
 
 /* Renderer (Core/Renderer.h:15-59) on one MI355X. */
@@ -178,11 +228,20 @@ class Renderer {
     void WriteFrameToFile() { check(edxr_write_frame_to_file(ctx_)); }
     /* Renderer::SetMSAAMode (Core/Renderer.cpp:94-98): 0 (1 sample) .. 5 (32). */
     void SetMSAAMode(int sampleCountLog2) { check(edxr_set_state(ctx_, EDXR_STATE_MSAA, sampleCountLog2)); }
+    /* Renderer::RenderMesh binds the mesh's texture slots
+     * (RenderStates::TextureSlots, Core/Renderer.cpp:105-106) and renders. */
     void RenderMesh(const Mesh& mesh) {
+        const std::vector<edxr_texture>& t = mesh.GetTextures();
+        if (!t.empty() && mesh.TextureSetId() != boundTextureSet_) {
+            check(edxr_set_textures(ctx_, t.data(), (uint32_t)t.size()));
+            boundTextureSet_ = mesh.TextureSetId();
+        }
         const edxr_mesh v = mesh.View();
         check(edxr_render_mesh(ctx_, &v));
         backBufferValid_ = false;
     }
+    /* Renderer::SetTextureFilter (RenderStates::TexFilter): EDXR_FILTER_*. */
+    void SetTextureFilter(int filter) { check(edxr_set_state(ctx_, EDXR_STATE_TEXTURE_FILTER, filter)); }
     void RenderMesh(const edxr_mesh& mesh) {
         check(edxr_render_mesh(ctx_, &mesh));
         backBufferValid_ = false;
@@ -221,6 +280,7 @@ class Renderer {
     uint32_t width_ = 0, height_ = 0;
     std::vector<uint8_t> backBuffer_;
     bool backBufferValid_ = false;
+    uint64_t boundTextureSet_ = 0;
 };
 
 }  // namespace RasterRenderer

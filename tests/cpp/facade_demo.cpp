@@ -1,7 +1,9 @@
 // Reference-style host code over the C++ facade: the flow of the reference
 // viewer's OnInit / OnRender (RealtimeViewer/Main.cpp:32-75) -- Initialize,
-// LoadSphere, SetTransform, RenderMesh, GetBackBuffer -- without the window.
-// Writes the back buffer (RGBA8, bottom-up rows) to the file given as argv[1].
+// LoadSphere or LoadMesh, SetTransform, RenderMesh, GetBackBuffer -- without
+// the window.  Writes the back buffer (RGBA8, bottom-up rows) to argv[1].
+// With an OBJ path as argv[2] it renders that mesh (320 x 240) with its
+// texture slots and the textured Lambertian shader.
 #include <cstdio>
 #include <vector>
 
@@ -15,17 +17,28 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "usage: facade_demo out.rgba\n");
         return 2;
     }
-    const uint32_t W = 800, H = 600;
+    const bool obj = argc >= 3;
+    const uint32_t W = obj ? 320 : 800, H = obj ? 240 : 600;
     try {
         Renderer renderer;
         renderer.Initialize(W, H);
         Mesh mesh;
         const float center[3] = {0.0f, 0.0f, 0.0f};
-        mesh.LoadSphere(center, 1.0f, 64, 32, 1);
-        const float eye[3] = {0.0f, 0.0f, -3.0f}, at[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
-        renderer.SetTransform(Matrix::LookAt(eye, at, up), Matrix::Perspective(60.0f, (float)W / H, 0.1f, 100.0f),
-                              Matrix::ToRaster(W, H));
-        renderer.SetPixelShader(VertexColorBlinnPhongPixelShader(32.0f));
+        const float up[3] = {0.0f, 1.0f, 0.0f};
+        if (obj) {
+            const float pos[3] = {0.0f, 0.0f, 0.0f}, scl[3] = {1.0f, 1.0f, 1.0f}, rot[3] = {0.0f, 0.0f, 0.0f};
+            mesh.LoadMesh(pos, scl, rot, argv[2]);
+            const float eye[3] = {0.0f, 1.2f, -2.2f};
+            renderer.SetTransform(Matrix::LookAt(eye, center, up),
+                                  Matrix::Perspective(60.0f, (float)W / H, 0.1f, 100.0f), Matrix::ToRaster(W, H));
+            renderer.SetPixelShader(TexturedLambertianPixelShader());
+        } else {
+            mesh.LoadSphere(center, 1.0f, 64, 32, 1);
+            const float eye[3] = {0.0f, 0.0f, -3.0f};
+            renderer.SetTransform(Matrix::LookAt(eye, center, up),
+                                  Matrix::Perspective(60.0f, (float)W / H, 0.1f, 100.0f), Matrix::ToRaster(W, H));
+            renderer.SetPixelShader(VertexColorBlinnPhongPixelShader(32.0f));
+        }
         renderer.RenderMesh(mesh);
         const uint8_t* bb = renderer.GetBackBuffer();
         FILE* f = std::fopen(argv[1], "wb");

@@ -178,3 +178,103 @@ def test_obj_mesh_renders_like_the_oracle_and_frames_are_written(built, tmp_path
     r.write_frame_to_file()  # explicit write: the counter has advanced to 2
     assert os.path.exists(tmp_path / "Frames" / "Frame00002.bmp")
     r.close()
+
+
+def test_obj_texture_coordinates_slots_and_materials(built, tmp_path):
+    write(tmp_path / "m.mtl", "newmtl img\nKd 0.5 0.5 0.5\nmap_Kd tex/checker.bmp\nnewmtl flat\nKd 0.1 0.2 0.3\n")
+    obj = write(tmp_path / "t.obj", """mtllib m.mtl
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+vt 0.5 0.5
+usemtl img
+f 1/1 2/2 3/3
+usemtl flat
+f 1/5 3/3 4/4
+f 2/2 3/3 4
+""")
+    m = Mesh.load_obj(obj)
+    assert m.num_triangles == 3
+    np.testing.assert_array_equal(m.tex_ids, [0, 1, 1])
+    uv = np.stack([m.tu, m.tv], 1)
+    np.testing.assert_array_equal(uv[m.indices[0:3]], [[0, 0], [1, 0], [1, 1]])
+    np.testing.assert_array_equal(uv[m.indices[3:6]], [[0.5, 0.5], [1, 1], [0, 1]])
+    np.testing.assert_array_equal(uv[m.indices[8]], [0, 0])  # no texture coordinate given
+    # position 1 appears with two coordinates: two vertices
+    assert len({tuple(uv[i]) for i in m.indices if tuple(m.positions()[i]) == (0.0, 0.0, 0.0)}) == 2
+    mats = Mesh.obj_materials(obj)
+    assert [x[0] for x in mats] == ["img", "flat"]
+    np.testing.assert_array_equal(mats[1][1], np.float32([0.1, 0.2, 0.3]))
+    assert mats[0][2] == str(tmp_path / "tex" / "checker.bmp") and mats[1][2] == ""
+    # faces without a material use the slot after the materials
+    m2 = Mesh.load_obj(write(tmp_path / "n.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"))
+    np.testing.assert_array_equal(m2.tex_ids, [0])
+
+
+def test_bmp_reader_round_trip_and_top_down(built, tmp_path):
+    from edxraster_amd import load_bmp
+    rng = np.random.default_rng(5)
+    rgba = rng.integers(0, 256, (7, 9, 4), dtype=np.uint8)
+    assert L.lib().edxr_write_bmp(os.fsencode(tmp_path / "a.bmp"), rgba.ctypes.data, 9, 7) == 0
+    back = load_bmp(tmp_path / "a.bmp")
+    np.testing.assert_array_equal(back[:, :, :3], rgba[:, :, :3])
+    assert (back[:, :, 3] == 255).all()
+    # a 32-bit top-down file (negative height) comes back bottom-up with alpha
+    w, h = 3, 2
+    px = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)  # file rows top-down, BGRA
+    hdr = b"BM" + struct.pack("<IHHI", 54 + 4 * w * h, 0, 0, 54) + \
+        struct.pack("<IiiHHIIiiII", 40, w, -h, 1, 32, 0, 4 * w * h, 2835, 2835, 0, 0)
+    (tmp_path / "td.bmp").write_bytes(hdr + px.tobytes())
+    got = load_bmp(tmp_path / "td.bmp")
+    np.testing.assert_array_equal(got, px[::-1][:, :, [2, 1, 0, 3]])
+    (tmp_path / "bad.bmp").write_bytes(b"BM" + b"\0" * 20)
+    with pytest.raises(RuntimeError):
+        load_bmp(tmp_path / "bad.bmp")
+
+
+def write_textured_obj(tmp_path):
+    """An OBJ heightfield with texture coordinates and a .bmp diffuse map
+    (ImageTexture) beside a Kd-only material (ConstantTexture2D)."""
+    (tmp_path / "tex").mkdir()
+    img = configs.checker_texture(128, 8, 3)
+    assert L.lib().edxr_write_bmp(os.fsencode(tmp_path / "tex" / "c.bmp"), img.ctypes.data, 128, 128) == 0
+    write(tmp_path / "m.mtl", "newmtl a\nKd 0.4 0.4 0.4\nmap_Kd tex/c.bmp\nnewmtl b\nKd 0.9 0.5 0.2\n")
+    g = Mesh.load_heightfield(48, 0.3, 7, 9)
+    lines = ["mtllib m.mtl"] + ["v %.9g %.9g %.9g" % tuple(p) for p in g.positions()]
+    lines += ["vt %.9g %.9g" % ((x + 1) * 2, (z + 1) * 2) for x, z in zip(g.px, g.pz)]
+    for k, (a, b, c) in enumerate(g.indices.reshape(-1, 3)):
+        if k % 200 == 0:
+            lines.append("usemtl %s" % ("a" if (k // 200) % 3 else "b"))
+        lines.append("f %d/%d %d/%d %d/%d" % (a + 1, a + 1, b + 1, b + 1, c + 1, c + 1))
+    return write(tmp_path / "hf.obj", "\n".join(lines) + "\n")
+
+
+TEXTURED_OBJ_CAMERA = ((0.0, 1.2, -2.2), (0.0, 0.0, 0.0))
+
+
+@pytest.mark.gpu
+def test_obj_with_bmp_diffuse_map_renders_like_the_oracle(built, tmp_path):
+    """The textured OBJ through the textured shader, its slots taken from the
+    OBJ's materials, against the oracle."""
+    from oracle.oracle import Oracle
+    from edxraster_amd import Renderer
+    from tests.test_gpu_parity import assert_frame_parity
+    path = write_textured_obj(tmp_path)
+    m = Mesh.load_obj(path)
+    slots = Mesh.obj_texture_slots(path)
+    assert isinstance(slots[0], np.ndarray) and len(slots) == 3
+    w = configs.Workload("obj_textured", 320, 240, m, *configs.camera(320, 240, *TEXTURED_OBJ_CAMERA),
+                        shader=L.SHADER_TEXTURED_LAMBERT, shininess=0.0, textures=slots)
+    r = Renderer(w.width, w.height)
+    w.apply(r)
+    r.render_mesh(w.mesh)
+    o = Oracle()
+    o.render_workload(w)
+    info = assert_frame_parity(r, o, "obj textured")
+    assert info["covered"] > 1000
+    r.close()
