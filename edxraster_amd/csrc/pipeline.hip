@@ -1036,10 +1036,8 @@ __device__ __forceinline__ void load_rtri(const RasterRec& r, RTri& t) {
 
 // Record of an unclipped triangle in slot r from its vertices' screen values
 // (the values k_setup used); corner codes are not needed below tile level.
-__device__ __forceinline__ void screen_rtri(const FrameParams& P, const DeviceBuffers& B, uint32_t r, RTri& t) {
-    const uint32_t* idx = P.mesh.idx + 3 * (size_t)(r - P.slot_base);
-    const ScreenVert s0 = B.sverts[P.vert_base + idx[0]], s1 = B.sverts[P.vert_base + idx[1]],
-                     s2 = B.sverts[P.vert_base + idx[2]];
+__device__ __forceinline__ void screen_rtri_from(const ScreenVert& s0, const ScreenVert& s1, const ScreenVert& s2,
+                                                 uint32_t r, RTri& t) {
     t.x0 = s0.x; t.y0 = s0.y; t.x1 = s1.x; t.y1 = s1.y; t.x2 = s2.x; t.y2 = s2.y;
     t.B0 = t.y0 - t.y1; t.C0 = t.x1 - t.x0;
     t.B1 = t.y1 - t.y2; t.C1 = t.x2 - t.x1;
@@ -1051,6 +1049,20 @@ __device__ __forceinline__ void screen_rtri(const FrameParams& P, const DeviceBu
     t.z0 = s0.z; t.z1 = s1.z; t.z2 = s2.z;
     t.info = 0;
     t.order = make_order(r, 0, false);
+}
+__device__ __forceinline__ void screen_rtri(const FrameParams& P, const DeviceBuffers& B, uint32_t r, RTri& t) {
+    const uint32_t* idx = P.mesh.idx + 3 * (size_t)(r - P.slot_base);
+    screen_rtri_from(B.sverts[P.vert_base + idx[0]], B.sverts[P.vert_base + idx[1]], B.sverts[P.vert_base + idx[2]],
+                     r, t);
+}
+
+// A reference's index-buffer entries when it is a compact unclipped triangle
+// (otherwise the job's first entries, a valid address whose value is unused).
+__device__ __forceinline__ uint3 ref_indices(const FrameParams& P, uint32_t total_slots, uint32_t pay) {
+    const uint32_t rid = pay & kRecordMask;
+    const bool compact = !(pay >> 31) && rid < total_slots && rid >= P.slot_base;
+    const uint32_t* idx = P.mesh.idx + (compact ? 3 * (size_t)(rid - P.slot_base) : 0);
+    return make_uint3(idx[0], idx[1], idx[2]);
 }
 
 // Ordered depth key: (depth, submission order) min = the winner of the
@@ -1482,32 +1494,43 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
         __syncthreads();
 
         // Each wave takes slices of L references (64, or a quarter of a short
-        // segment so that every wave has work); the next slice's references
-        // are loaded before this slice's work.
+        // segment so that every wave has work).  Loads run ahead of the work:
+        // references two slices ahead and index-buffer entries one slice
+        // ahead, so a slice waits only for its screen vertices.  Those are
+        // issued first and the look-ahead loads after them (all unconditional,
+        // at clamped addresses), because the vector-memory wait is in order.
         const uint32_t segLen = segEnd - segBegin;
         const uint32_t L = segLen >= kBlock ? kWave : (segLen + kBlock / kWave - 1) / (kBlock / kWave);
         const uint32_t stride = L * (kBlock / kWave);
         const uint32_t first = segBegin + wave * L + lane;
-        uint32_t payNext = lane < L && first < segEnd ? B.refs[start + first] : 0u;
+        const uint32_t* segRefs = B.refs + start;
+        uint32_t pay1 = segRefs[min(first, segEnd - 1)];
+        uint32_t pay2 = segRefs[min(first + stride, segEnd - 1)];
+        uint3 idx1 = ref_indices(P, Q.total_slots, pay1);
         for (uint32_t b = segBegin + wave * L; b < segEnd; b += stride) {
             const uint32_t p = b + lane;
             const bool valid = lane < L && p < segEnd;
-            const uint32_t pay = payNext;
-            payNext = lane < L && p + stride < segEnd ? B.refs[start + p + stride] : 0u;
+            const uint32_t pay = pay1;
+            const uint3 idx = idx1;
+            const uint32_t rid = pay & kRecordMask;
             const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
             const bool big = (pay >> 31) != 0;
+            const bool compact = !big && rid < Q.total_slots;
+            // Compact unclipped triangles rebuild their record from their
+            // vertices' screen values.
+            const ScreenVert s0 = B.sverts[P.vert_base + (compact ? idx.x : 0u)];
+            const ScreenVert s1 = B.sverts[P.vert_base + (compact ? idx.y : 0u)];
+            const ScreenVert s2 = B.sverts[P.vert_base + (compact ? idx.z : 0u)];
+            pay1 = pay2;
+            pay2 = segRefs[min(p + 2 * stride, segEnd - 1)];
+            idx1 = ref_indices(P, Q.total_slots, pay1);
             // Tile-level paths (RasterizeTile, Core/Renderer.cpp:255-268):
             // trivially accepted, or big with hierarchical rasterisation on.
             const bool large = valid && (trivial || (big && P.hierarchical));
-            // Every lane loads its own record; large ones are broadcast below.
-            // References without the big bit below total_slots are compact
-            // unclipped triangles, whose record is rebuilt from their vertices.
+            // Every lane holds its own record; large ones are broadcast below.
             RTri t;
-            if (valid) {
-                const uint32_t rid = pay & kRecordMask;
-                if (big || rid >= Q.total_slots) load_rtri(B.rrec[rid], t);
-                else screen_rtri(P, B, rid, t);
-            }
+            screen_rtri_from(s0, s1, s2, rid, t);
+            if (valid && !compact) load_rtri(B.rrec[rid], t);
             uint32_t myQ = 0;  // quads of a FineRasterize reference beyond the tiny path
             int fx = 0, fy = 0, fnx = 1;
             if (valid && !large) {
