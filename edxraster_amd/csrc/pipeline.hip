@@ -1495,35 +1495,42 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
 
         // Each wave takes slices of L references (64, or a quarter of a short
         // segment so that every wave has work).  Loads run ahead of the work:
-        // references two slices ahead and index-buffer entries one slice
-        // ahead, so a slice waits only for its screen vertices.  Those are
-        // issued first and the look-ahead loads after them (all unconditional,
-        // at clamped addresses), because the vector-memory wait is in order.
+        // references three slices ahead, index-buffer entries two and screen
+        // vertices one, so no slice waits on a chain of dependent loads.  All
+        // are unconditional, at clamped addresses, so that the in-order
+        // vector-memory wait count stays exact.
         const uint32_t segLen = segEnd - segBegin;
         const uint32_t L = segLen >= kBlock ? kWave : (segLen + kBlock / kWave - 1) / (kBlock / kWave);
         const uint32_t stride = L * (kBlock / kWave);
         const uint32_t first = segBegin + wave * L + lane;
         const uint32_t* segRefs = B.refs + start;
+        auto sv_of = [&](uint32_t pay, const uint3& idx, ScreenVert& a, ScreenVert& b2, ScreenVert& c) {
+            const uint32_t rid = pay & kRecordMask;
+            const bool compact = !(pay >> 31) && rid < Q.total_slots;
+            a = B.sverts[P.vert_base + (compact ? idx.x : 0u)];
+            b2 = B.sverts[P.vert_base + (compact ? idx.y : 0u)];
+            c = B.sverts[P.vert_base + (compact ? idx.z : 0u)];
+        };
         uint32_t pay1 = segRefs[min(first, segEnd - 1)];
         uint32_t pay2 = segRefs[min(first + stride, segEnd - 1)];
-        uint3 idx1 = ref_indices(P, Q.total_slots, pay1);
+        uint32_t pay3 = segRefs[min(first + 2 * stride, segEnd - 1)];
+        ScreenVert n0, n1, n2;
+        sv_of(pay1, ref_indices(P, Q.total_slots, pay1), n0, n1, n2);
+        uint3 idx2 = ref_indices(P, Q.total_slots, pay2);
         for (uint32_t b = segBegin + wave * L; b < segEnd; b += stride) {
             const uint32_t p = b + lane;
             const bool valid = lane < L && p < segEnd;
             const uint32_t pay = pay1;
-            const uint3 idx = idx1;
             const uint32_t rid = pay & kRecordMask;
             const bool trivial = ((pay >> kRecordBits) & 7u) == 7u;
             const bool big = (pay >> 31) != 0;
             const bool compact = !big && rid < Q.total_slots;
-            // Compact unclipped triangles rebuild their record from their
-            // vertices' screen values.
-            const ScreenVert s0 = B.sverts[P.vert_base + (compact ? idx.x : 0u)];
-            const ScreenVert s1 = B.sverts[P.vert_base + (compact ? idx.y : 0u)];
-            const ScreenVert s2 = B.sverts[P.vert_base + (compact ? idx.z : 0u)];
+            const ScreenVert s0 = n0, s1 = n1, s2 = n2;
+            sv_of(pay2, idx2, n0, n1, n2);
+            idx2 = ref_indices(P, Q.total_slots, pay3);
             pay1 = pay2;
-            pay2 = segRefs[min(p + 2 * stride, segEnd - 1)];
-            idx1 = ref_indices(P, Q.total_slots, pay1);
+            pay2 = pay3;
+            pay3 = segRefs[min(p + 3 * stride, segEnd - 1)];
             // Tile-level paths (RasterizeTile, Core/Renderer.cpp:255-268):
             // trivially accepted, or big with hierarchical rasterisation on.
             const bool large = valid && (trivial || (big && P.hierarchical));
