@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
 }
 
 // ---------------------------------------------------------------------------
-// Clipping of queued straddlers (rare path, one lane per triangle).
+// Clipping of queued straddlers (rare path).
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -335,172 +335,210 @@ __device__ __forceinline__ void plane_snap(float4& v, int p) {
 // Plane order of ClipPolygon (Core/Clipper.h:238-278): L R B T F N.
 __device__ const uint32_t kPlaneBits[6] = {1u, 2u, 4u, 8u, 32u, 16u};
 
-// Setup of fan triangle (0, k-1, k) of a projected clipped polygon.
-__device__ __forceinline__ bool clip_fan(const FrameParams& P, const float4 (*h)[64], uint32_t lane, int k,
-                                         RasterRec& r) {
-    const float4 h0 = h[0][lane], h1 = h[k - 1][lane], h2 = h[k][lane];
-    return setup_record(P, h0.x, h0.y, h0.z, h1.x, h1.y, h1.z, h2.x, h2.y, h2.z, r);
-}
-
 }  // namespace
 
-// One lane per queued triangle.  The polygons live in LDS ([buffer][vertex][lane],
-// conflict-free): Sutherland-Hodgman output positions are data-dependent, and
-// private arrays indexed that way would go to scratch memory.
-__global__ __launch_bounds__(64) void k_clip(BatchParams Q, const FrameParams* __restrict__ jobs, DeviceBuffers B) {
-    __shared__ float4 sPos[2][kMaxPoly][64];  // clip-space positions
-    __shared__ float4 sWgt[2][kMaxPoly][64];  // clip weights (Core/Clipper.h:111-119)
-    const uint32_t lane = threadIdx.x;
-    const uint32_t n = min(B.counters[kCntClipQueue], B.clip_queue_cap);
-    for (uint32_t q = blockIdx.x * 64 + lane; q < n; q += gridDim.x * 64) {
-        const uint32_t t = B.clip_queue[q];  // triangle slot; its job varies per lane
-        const FrameParams& P = jobs[Q.num_jobs == 1 ? 0u : Q.sblock_job[t / kBlock]];
-        const uint32_t* idx = P.mesh.idx + 3 * (size_t)(t - P.slot_base);
-        const uint32_t vi[3] = {P.vert_base + idx[0], P.vert_base + idx[1], P.vert_base + idx[2]};
-        // The parents' shading values, read once with their positions (inside the
-        // vertex loop below they would be re-read after every vertex store).
-        const ShadeVert pa = B.verts[vi[0]], pb = B.verts[vi[1]], pc = B.verts[vi[2]];
-        float4 c[3];
-        c[0] = xform4(P.mvp, pa.px, pa.py, pa.pz, 1.0f);
-        c[1] = xform4(P.mvp, pb.px, pb.py, pb.pz, 1.0f);
-        c[2] = xform4(P.mvp, pc.px, pc.py, pc.pz, 1.0f);
-        const uint32_t k0 = clip_code(c[0]), k1 = clip_code(c[1]), k2 = clip_code(c[2]);
-        const uint32_t planeCode = (k0 ^ k1) | (k1 ^ k2) | (k2 ^ k0);
-        atomicAdd(&B.counters[kCntClipped], 1u);
-        if ((k0 | k1 | k2) & 16u) atomicAdd(&B.counters[kCntNearClipped], 1u);
+// One wave per queued triangle, one polygon vertex per lane (kMaxPoly <= 64),
+// 16 triangles per workgroup pass.  Clipper::ClipByPlane (Core/Clipper.h:
+// 192-232) per plane: lane i takes the edge (i, i + 1) and emits 0, 1 or 2
+// vertices, and a wave prefix sum places them in the reference's output
+// order; the arithmetic per vertex is the serial algorithm's.  The passes'
+// vertex and record allocations are summed in LDS and taken with one atomic
+// each per workgroup pass (one per triangle would serialise in L2).
+namespace {
+__device__ __forceinline__ float4 shfl4(const float4& v, int src) {
+    return make_float4(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), __shfl(v.w, src));
+}
+constexpr uint32_t kClipWaves = 16;  // waves (triangles) per workgroup pass
+}  // namespace
 
-        int cnt = 3, cur = 0;
+__global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                           DeviceBuffers B) {
+    __shared__ uint32_t sNew[kClipWaves], sRec[kClipWaves], sVBase[kClipWaves], sRBase[kClipWaves];
+    __shared__ uint32_t sStat[2];
+    const int lane = (int)__lane_id();
+    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t n = min(B.counters[kCntClipQueue], B.clip_queue_cap);
+    uint32_t clipped = 0, nearClipped = 0;  // this wave's statistics (lane 0)
+    for (uint32_t base = blockIdx.x * kClipWaves; base < n; base += gridDim.x * kClipWaves) {
+        const uint32_t q = base + wave;
+        const bool live = q < n;
+        // ---- per wave: clip its triangle (no global writes yet) ----
+        uint32_t t = 0, vi[3] = {0, 0, 0};
+        ShadeVert pa, pb, pc;
+        float4 pos = make_float4(0.0f, 0.0f, 0.0f, 1.0f), wgt = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        int cnt = 0;
         bool ok = true;
-        for (int k = 0; k < 3; k++) {
-            sPos[0][k][lane] = c[k];
-            sWgt[0][k][lane] = make_float4(k == 0 ? 1.0f : 0.0f, k == 1 ? 1.0f : 0.0f, k == 2 ? 1.0f : 0.0f, 0.0f);
-        }
-        // Clipper::ClipByPlane (Core/Clipper.h:192-232).
-        for (int p = 0; p < 6 && ok; p++) {
-            if (!(planeCode & kPlaneBits[p])) continue;
-            int m = 0;
-            const int nx = cur ^ 1;
-            for (int i = 0; i < cnt; i++) {
-                int i1 = i + 1;
-                if (i1 == cnt) i1 = 0;
-                const float4 v0 = sPos[cur][i][lane], v1 = sPos[cur][i1][lane];
-                const float4 w0 = sWgt[cur][i][lane], w1 = sWgt[cur][i1][lane];
+        const FrameParams* Pp = jobs;
+        if (live) {
+            t = __builtin_amdgcn_readfirstlane(B.clip_queue[q]);  // triangle slot
+            Pp = &jobs[Q.num_jobs == 1 ? 0u : __builtin_amdgcn_readfirstlane(Q.sblock_job[t / kBlock])];
+            const FrameParams& P = *Pp;
+            const uint32_t* idx = P.mesh.idx + 3 * (size_t)(t - P.slot_base);
+            vi[0] = P.vert_base + __builtin_amdgcn_readfirstlane(idx[0]);
+            vi[1] = P.vert_base + __builtin_amdgcn_readfirstlane(idx[1]);
+            vi[2] = P.vert_base + __builtin_amdgcn_readfirstlane(idx[2]);
+            pa = B.verts[vi[0]];
+            pb = B.verts[vi[1]];
+            pc = B.verts[vi[2]];
+            const float4 c0 = xform4(P.mvp, pa.px, pa.py, pa.pz, 1.0f);
+            const float4 c1 = xform4(P.mvp, pb.px, pb.py, pb.pz, 1.0f);
+            const float4 c2 = xform4(P.mvp, pc.px, pc.py, pc.pz, 1.0f);
+            const uint32_t k0 = clip_code(c0), k1 = clip_code(c1), k2 = clip_code(c2);
+            const uint32_t planeCode = (k0 ^ k1) | (k1 ^ k2) | (k2 ^ k0);
+            clipped++;
+            if ((k0 | k1 | k2) & 16u) nearClipped++;
+            // The polygon: vertex `lane` of cnt (clip-space position, clip weights).
+            pos = lane == 0 ? c0 : lane == 1 ? c1 : c2;
+            wgt = make_float4(lane == 0 ? 1.0f : 0.0f, lane == 1 ? 1.0f : 0.0f, lane == 2 ? 1.0f : 0.0f, 0.0f);
+            cnt = 3;
+            for (int p = 0; p < 6; p++) {
+                if (!(planeCode & kPlaneBits[p])) continue;
+                const bool act = lane < cnt;
+                const int i1 = lane + 1 >= cnt ? 0 : lane + 1;
+                const float4 v0 = pos, w0 = wgt, v1 = shfl4(pos, i1), w1 = shfl4(wgt, i1);
                 const bool in0 = plane_inside(v0, p), in1 = plane_inside(v1, p);
-                if (in0 && in1) {
-                    if (m >= kMaxPoly) { ok = false; break; }
-                    sPos[nx][m][lane] = v1; sWgt[nx][m][lane] = w1; m++;
-                } else if (in0 != in1) {
+                const int emit = !act ? 0 : (in0 && in1) ? 1 : (in0 != in1) ? (in0 ? 1 : 2) : 0;
+                // Intersection of the edge with the plane.
+                float4 ip = v1, iw = w1;
+                if (act && in0 != in1) {
                     const float tt = plane_t(v0, v1, p);
                     const float s = 1 - tt;
-                    float4 pos = make_float4(v0.x * s + v1.x * tt, v0.y * s + v1.y * tt, v0.z * s + v1.z * tt,
-                                             v0.w * s + v1.w * tt);
-                    plane_snap(pos, p);
-                    const float4 w = make_float4(w0.x * s + w1.x * tt, w0.y * s + w1.y * tt, w0.z * s + w1.z * tt,
-                                                 0.0f);
-                    if (m >= kMaxPoly) { ok = false; break; }
-                    sPos[nx][m][lane] = pos; sWgt[nx][m][lane] = w; m++;
-                    if (!in0) {
-                        if (m >= kMaxPoly) { ok = false; break; }
-                        sPos[nx][m][lane] = v1; sWgt[nx][m][lane] = w1; m++;
-                    }
+                    ip = make_float4(v0.x * s + v1.x * tt, v0.y * s + v1.y * tt, v0.z * s + v1.z * tt,
+                                     v0.w * s + v1.w * tt);
+                    plane_snap(ip, p);
+                    iw = make_float4(w0.x * s + w1.x * tt, w0.y * s + w1.y * tt, w0.z * s + w1.z * tt, 0.0f);
                 }
+                // Inclusive prefix of the emitted counts over the lanes.
+                int incl = emit;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int y = __shfl_up(incl, d);
+                    if (lane >= d) incl += y;
+                }
+                const int total = __shfl(incl, 63);
+                if (total > kMaxPoly) {
+                    ok = false;
+                    break;
+                }
+                // Output vertex `lane` comes from the edge lane whose range holds
+                // it: its first item (the intersection, or v1 when both ends are
+                // inside) or its second (v1 after an entering intersection).
+                const int excl = incl - emit;
+                int src = 0;
+                for (int l = 1; l < cnt; l++) {
+                    // both shuffles run with every lane active (a lane switched
+                    // off by a condition reads as 0 to the others)
+                    const int el = __shfl(excl, l), ml = __shfl(emit, l);
+                    if (el <= lane && ml > 0) src = l;
+                }
+                const int k = lane - __shfl(excl, src);
+                const float4 a = shfl4(ip, src), aw = shfl4(iw, src), b = shfl4(v1, src), bw = shfl4(w1, src);
+                if (lane < total) {
+                    pos = k == 0 ? a : b;
+                    wgt = k == 0 ? aw : bw;
+                }
+                cnt = total;
             }
-            cnt = m;
-            cur = nx;
+            if (!ok || cnt > 9) {
+                if (lane == 0) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvPolygon);
+                cnt = 0;
+            }
+            if (__ballot(lane < cnt && pos.w <= 0.0f) != 0) cnt = 0;
+            if (cnt < 3) cnt = 0;
         }
-        if (!ok || cnt > 9) {
-            atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvPolygon);
-            B.desc[t] = 0;
-            continue;
+        // Vertex ids: weight == 1 reuses a parent vertex (Core/Clipper.h:121-153);
+        // new vertices are numbered in polygon order.
+        const int which = (wgt.x == 1.0f) ? 0 : (wgt.y == 1.0f) ? 1 : (wgt.z == 1.0f) ? 2 : -1;
+        const uint64_t newMask = __ballot(lane < cnt && which < 0);
+        // Projected vertices (x/w, y/w, z/w, z * (1/w)) and the fans
+        // (Core/Clipper.h:155-170): fan k = lane + 2 uses vertices (0, k - 1, k).
+        const float4 h = make_float4(pos.x / pos.w, pos.y / pos.w, pos.z / pos.w, pos.z * (1.0f / pos.w));
+        const int fk = lane + 2;
+        const float4 h0 = shfl4(h, 0), h1 = shfl4(h, fk - 1 < 64 ? fk - 1 : 0), h2 = shfl4(h, fk < 64 ? fk : 0);
+        RasterRec r;
+        const bool fan = fk < cnt && setup_record(*Pp, h0.x, h0.y, h0.z, h1.x, h1.y, h1.z, h2.x, h2.y, h2.z, r);
+        const uint64_t fanMask = __ballot(fan);
+        const uint32_t nNew = (uint32_t)__popcll(newMask);
+        const uint32_t nrec = (uint32_t)__popcll(fanMask);
+        // ---- per workgroup: one allocation of vertices and records ----
+        if (lane == 0) {
+            sNew[wave] = nNew;
+            sRec[wave] = nrec;
         }
-        for (int i = 0; i < cnt; i++)
-            if (sPos[cur][i][lane].w <= 0.0f) { cnt = 0; break; }
-        if (cnt < 3) {
-            B.desc[t] = 0;
-            continue;
+        if (threadIdx.x == 0) sStat[0] = sStat[1] = 0;
+        __syncthreads();
+        if (lane == 0 && (clipped | nearClipped)) {
+            atomicAdd(&sStat[0], clipped);
+            atomicAdd(&sStat[1], nearClipped);
+            clipped = nearClipped = 0;
         }
-        // Vertex ids: weight == 1 reuses a parent vertex (Core/Clipper.h:121-153).
-        auto parent = [](const float4& w) { return (w.x == 1.0f) ? 0 : (w.y == 1.0f) ? 1 : (w.z == 1.0f) ? 2 : -1; };
-        int nNew = 0;
-        for (int j = 0; j < cnt; j++)
-            if (parent(sWgt[cur][j][lane]) < 0) nNew++;
-        uint32_t vbase = 0;
-        if (nNew) {
-            vbase = atomicAdd(&B.counters[kCntExtraVertices], (uint32_t)nNew);
-            if (vbase + (uint32_t)nNew > B.extra_vert_cap) {
-                atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvExtraVertices);
-                B.desc[t] = 0;
-                continue;
+        if (threadIdx.x == 0) {
+            uint32_t tv = 0, tr = 0;
+            for (uint32_t w = 0; w < kClipWaves; w++) {
+                sVBase[w] = tv;
+                sRBase[w] = tr;
+                tv += sNew[w];
+                tr += sRec[w];
+            }
+            const uint32_t vb = tv ? atomicAdd(&B.counters[kCntExtraVertices], tv) : 0u;
+            const uint32_t rb = tr ? atomicAdd(&B.counters[kCntExtraRecords], tr) : 0u;
+            for (uint32_t w = 0; w < kClipWaves; w++) {
+                sVBase[w] += vb;
+                sRBase[w] += rb;
             }
         }
-        // Projected vertices (x/w, y/w, z/w, z * (1/w)) and vertex ids go to the
-        // other buffer.
-        const int out = cur ^ 1;
-        int e = 0;
-        for (int j = 0; j < cnt; j++) {
-            const float4 w = sWgt[cur][j][lane];
-            const int which = parent(w);
-            float4 pos;
-            uint32_t vid;
-            if (which >= 0) {
-                vid = which == 0 ? vi[0] : which == 1 ? vi[1] : vi[2];
-                pos = which == 0 ? c[0] : which == 1 ? c[1] : c[2];
-            } else {
-                const uint32_t xi = vbase + (uint32_t)e++;
-                vid = Q.total_verts + xi;
-                pos = sPos[cur][j][lane];
-                const ShadeVert& a = pa;
-                const ShadeVert& b = pb;
-                const ShadeVert& cc = pc;
-                auto comb = [&](float fa, float fb, float fc) { return (w.x * fa + w.y * fb) + w.z * fc; };
+        __syncthreads();
+        if (threadIdx.x == 0 && (sStat[0] | sStat[1])) {
+            atomicAdd(&B.counters[kCntClipped], sStat[0]);
+            if (sStat[1]) atomicAdd(&B.counters[kCntNearClipped], sStat[1]);
+        }
+        // ---- per wave: write the new vertices, the records and the descriptor ----
+        if (live) {
+            const FrameParams& P = *Pp;
+            const uint32_t vbase = sVBase[wave], rbase = sRBase[wave];
+            // As the serial clipper: the polygon's new vertices are created
+            // whenever it survives, its records when it has fans.
+            const bool vfit = vbase + nNew <= B.extra_vert_cap;
+            if (!vfit && lane == 0) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvExtraVertices);
+            bool keep = vfit && nrec > 0;
+            if (keep && rbase + nrec > B.extra_rec_cap) {
+                if (lane == 0) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvExtraRecords);
+                keep = false;
+            }
+            uint32_t vid = which == 0 ? vi[0] : which == 1 ? vi[1] : vi[2];
+            if (vfit && lane < cnt && which < 0) {
+                vid = Q.total_verts + vbase + (uint32_t)__popcll(newMask & lanemask_lt());
+                auto comb = [&](float fa, float fb, float fc) { return (wgt.x * fa + wgt.y * fb) + wgt.z * fc; };
                 ShadeVert o;
-                o.px = comb(a.px, b.px, cc.px);
-                o.py = comb(a.py, b.py, cc.py);
-                o.pz = comb(a.pz, b.pz, cc.pz);
-                o.nx = comb(a.nx, b.nx, cc.nx);
-                o.ny = comb(a.ny, b.ny, cc.ny);
-                o.nz = comb(a.nz, b.nz, cc.nz);
-                o.cr = comb(a.cr, b.cr, cc.cr);
-                o.cg = comb(a.cg, b.cg, cc.cg);
-                o.cb = comb(a.cb, b.cb, cc.cb);
+                o.px = comb(pa.px, pb.px, pc.px);
+                o.py = comb(pa.py, pb.py, pc.py);
+                o.pz = comb(pa.pz, pb.pz, pc.pz);
+                o.nx = comb(pa.nx, pb.nx, pc.nx);
+                o.ny = comb(pa.ny, pb.ny, pc.ny);
+                o.nz = comb(pa.nz, pb.nz, pc.nz);
+                o.cr = comb(pa.cr, pb.cr, pc.cr);
+                o.cg = comb(pa.cg, pb.cg, pc.cg);
+                o.cb = comb(pa.cb, pb.cb, pc.cb);
                 o.inv_w = 1.0f / pos.w;
-                o.u = comb(a.u, b.u, cc.u);
-                o.v = comb(a.v, b.v, cc.v);
+                o.u = comb(pa.u, pb.u, pc.u);
+                o.v = comb(pa.v, pb.v, pc.v);
                 B.verts[vid] = o;
             }
-            sPos[out][j][lane] = make_float4(pos.x / pos.w, pos.y / pos.w, pos.z / pos.w, pos.z * (1.0f / pos.w));
-            sWgt[out][j][lane] = make_float4(__uint_as_float(vid), 0.0f, 0.0f, 0.0f);
+            const uint32_t vid0 = __shfl(vid, 0), vid1 = __shfl(vid, fk - 1 < 64 ? fk - 1 : 0),
+                           vid2 = __shfl(vid, fk < 64 ? fk : 0);
+            const uint32_t first = Q.total_slots + rbase;
+            if (keep && fan) {
+                const uint32_t sidx = (uint32_t)__popcll(fanMask & lanemask_lt());
+                r.z0 = h0.w;
+                r.z1 = h1.w;
+                r.z2 = h2.w;
+                r.info |= (uint32_t)(fk - 2) << 12;
+                r.order = make_order(t, sidx, true);
+                B.rrec[first + sidx] = r;
+                B.srec[first + sidx] = ShadeRec{vid0, vid1, vid2, t};
+            }
+            if (lane == 0) B.desc[t] = keep ? (first << 3) | nrec : 0u;
         }
-        // Fan triangulation (Core/Clipper.h:155-170): count survivors, reserve, write.
-        uint32_t nrec = 0;
-        RasterRec r;
-        for (int k = 2; k < cnt; k++)
-            if (clip_fan(P, sPos[out], lane, k, r)) nrec++;
-        if (nrec == 0) {
-            B.desc[t] = 0;
-            continue;
-        }
-        const uint32_t rbase = atomicAdd(&B.counters[kCntExtraRecords], nrec);
-        if (rbase + nrec > B.extra_rec_cap) {
-            atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvExtraRecords);
-            B.desc[t] = 0;
-            continue;
-        }
-        const uint32_t first = Q.total_slots + rbase;
-        uint32_t s = 0;
-        for (int k = 2; k < cnt; k++) {
-            if (!clip_fan(P, sPos[out], lane, k, r)) continue;
-            r.z0 = sPos[out][0][lane].w;
-            r.z1 = sPos[out][k - 1][lane].w;
-            r.z2 = sPos[out][k][lane].w;
-            r.info |= (uint32_t)(k - 2) << 12;
-            r.order = make_order(t, s, true);
-            B.rrec[first + s] = r;
-            B.srec[first + s] = ShadeRec{__float_as_uint(sWgt[out][0][lane].x), __float_as_uint(sWgt[out][k - 1][lane].x),
-                                         __float_as_uint(sWgt[out][k][lane].x), t};
-            s++;
-        }
-        B.desc[t] = (first << 3) | nrec;
+        __syncthreads();  // the LDS tables are rewritten by the next pass
     }
 }
 
@@ -1845,8 +1883,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     if (Q.slot_blocks) {
         hipLaunchKernelGGL(k_setup, dim3((Q.slot_blocks + kSetupSlots - 1) / kSetupSlots), dim3(kBlock), 0, s, Q,
                            jobs, B);
-        const uint32_t clipBlocks = max(1u, min(512u, (B.clip_queue_cap + 63) / 64));
-        hipLaunchKernelGGL(k_clip, dim3(clipBlocks), dim3(64), 0, s, Q, jobs, B);
+        const uint32_t clipBlocks = max(1u, min(1024u, (B.clip_queue_cap + kClipWaves - 1) / kClipWaves));
+        hipLaunchKernelGGL(k_clip, dim3(clipBlocks), dim3(kClipWaves * 64), 0, s, Q, jobs, B);
     }
     mark(1);
     if (Q.total_chunks) hipLaunchKernelGGL(k_bin_count, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
