@@ -232,6 +232,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
         }
     }
     uint32_t nrej = 0, nbad = 0, nmain = 0;
+    bool queuedU[kSetupSlots];
+    uint64_t qmask[kSetupSlots];
 #pragma unroll
     for (uint32_t u = 0; u < kSetupSlots; u++) {
         bool rejected = false, queued = false, main = false;
@@ -249,14 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
                 main = true;
             }
         }
-        // Wave-aggregated clip-queue append.
-        const uint64_t qmask = __ballot(queued);
-        uint32_t qbase = 0;
-        if (qmask) {
-            const int leader = __ffsll((unsigned long long)qmask) - 1;
-            if ((int)__lane_id() == leader) qbase = atomicAdd(&B.counters[kCntClipQueue], (uint32_t)__popcll(qmask));
-            qbase = __shfl(qbase, leader);
-        }
+        queuedU[u] = queued;
+        qmask[u] = __ballot(queued);
         if (valid[u]) {
             uint32_t d = 0;
             if (main) {
@@ -267,10 +263,6 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
                     rec.order = make_order(s[u], 0, false);
                     B.rrec[s[u]] = rec;
                 }
-            } else if (queued) {
-                const uint32_t slot = qbase + (uint32_t)__popcll(qmask & lanemask_lt());
-                if (slot < B.clip_queue_cap) B.clip_queue[slot] = s[u];
-                else atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvClipQueue);
             }
             B.desc[s[u]] = d;
         }
@@ -278,10 +270,27 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
         nbad += wave_count(bad[u]);
         nmain += wave_count(main);
     }
-    // Per-workgroup totals, one striped atomic per counter per workgroup.
+    // Per-workgroup totals, one striped atomic per counter per workgroup, and
+    // one clip-queue allocation per workgroup (same-address atomics from every
+    // wave would serialise in L2).
     __shared__ uint32_t wgCount[3];
+    __shared__ uint32_t sQueued[kBlock / kWave], sQueueBase[kBlock / kWave];
+    const uint32_t wave = threadIdx.x / kWave;
+    uint32_t nq = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kSetupSlots; u++) nq += (uint32_t)__popcll(qmask[u]);
+    if (__lane_id() == 0) sQueued[wave] = nq;
     if (threadIdx.x < 3) wgCount[threadIdx.x] = 0;
     __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tq = 0;
+        for (uint32_t w = 0; w < kBlock / kWave; w++) {
+            sQueueBase[w] = tq;
+            tq += sQueued[w];
+        }
+        const uint32_t qb = tq ? atomicAdd(&B.counters[kCntClipQueue], tq) : 0u;
+        for (uint32_t w = 0; w < kBlock / kWave; w++) sQueueBase[w] += qb;
+    }
     if (__lane_id() == 0) {
         if (nrej) atomicAdd(&wgCount[0], nrej);
         if (nbad) atomicAdd(&wgCount[1], nbad);
@@ -293,6 +302,18 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
         if (wgCount[0]) atomicAdd(&B.counters[kCntStriped + kStRejected * kStripes + stripe], wgCount[0]);
         if (wgCount[1]) atomicAdd(&B.counters[kCntBadIndex], wgCount[1]);
         if (wgCount[2]) atomicAdd(&B.counters[kCntStriped + kStMainRecords * kStripes + stripe], wgCount[2]);
+    }
+    if (nq) {
+        uint32_t slot = sQueueBase[wave];
+#pragma unroll
+        for (uint32_t u = 0; u < kSetupSlots; u++) {
+            if (queuedU[u]) {
+                const uint32_t at = slot + (uint32_t)__popcll(qmask[u] & lanemask_lt());
+                if (at < B.clip_queue_cap) B.clip_queue[at] = s[u];
+                else atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvClipQueue);
+            }
+            slot += (uint32_t)__popcll(qmask[u]);
+        }
     }
 }
 
