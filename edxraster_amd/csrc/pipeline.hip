@@ -1724,8 +1724,10 @@ __global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FramePar
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const uint32_t nItems = B.counters[kCntWorkItems], segLog2 = B.counters[kCntSegLog2];
-    for (;;) {
-        if (threadIdx.x == 0) sItem = atomicAdd(&B.counters[kCntWorkNext], 1u);
+    // The first item is the workgroup's index (no burst of same-address
+    // atomics at launch); later ones come from the shared counter.
+    for (uint32_t round = 0;; round++) {
+        if (threadIdx.x == 0) sItem = round ? gridDim.x + atomicAdd(&B.counters[kCntWorkNext], 1u) : blockIdx.x;
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= nItems) break;
