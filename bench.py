@@ -66,8 +66,8 @@ def cpu_baseline(w, budget_s):
 # float4 copy kernel of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
 # profiles/pmc_raster_shade_*.md.  None where not measured.
 RASTER_SHADE_TRAFFIC = {
-    "sphere_grid_16x16x64_1920x1080": 59797 * 1024,   # (5575 + 6801) KiB x 2 + (10746 + 24300) KiB
-    "heightfield_2236_3840x2160": 792871 * 1024,     # (135038 + 189906) KiB x 2 + (45778 + 97205) KiB
+    "sphere_grid_16x16x64_1920x1080": 61485 * 1024,   # (6439 + 6796) KiB x 2 + (10714 + 24300) KiB
+    "heightfield_2236_3840x2160": 794299 * 1024,     # (135576 + 190098) KiB x 2 + (45746 + 97204) KiB
 }
 
 
