@@ -1127,9 +1127,8 @@ __device__ __forceinline__ uint3 ref_indices(const FrameParams& P, uint32_t tota
 // Ordered depth key: (depth, submission order) min = the winner of the
 // sequential d <= D test (Core/FrameBuffer.cpp:54-68).  -0 and +0 compare equal.
 __device__ __forceinline__ uint32_t depth_bits(float d) {
-    uint32_t b = __float_as_uint(d);
-    if ((b & 0x7FFFFFFFu) == 0) b = 0;
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    const uint32_t b = __float_as_uint(d + 0.0f);  // -0 + 0 = +0; any other d is unchanged
+    return b ^ ((uint32_t)((int)b >> 31) | 0x80000000u);  // negative: ~b, otherwise b | sign
 }
 __device__ __forceinline__ uint64_t bg_key() { return ((uint64_t)0xBF800000u << 32) | 0xFFFFFFFFull; }
 
@@ -1148,13 +1147,18 @@ struct SamplePass {
 };
 
 // Depth test of one pixel from its edge values e0..e2 (coverage when
-// useEdges, Core/Rasterizer.h:162) and barycentric numerators n0, n1
-// (Core/RasterTriangle.h:324-337), then the (depth, order) key update.  With
-// multisampling each sample of the pass is tested at centre + offset
-// (Core/Rasterizer.h:243-266): e + ox B + oy C, wrapping as the reference.
+// useEdges, Core/Rasterizer.h:162), then the (depth, order) key update; klo is
+// the key's low word, 0xFFFFFFFE - order.  The barycentric numerators
+// n0 = B1 (x - x2) + C1 (y - y2) and n1 = B2 (x - x2) + C2 (y - y2)
+// (Core/RasterTriangle.h:324-337) are e1 and e2 without their top-left bias:
+// e1 - pb1 - n0 = B1 (x2 - x1) + C1 (y2 - y1) = 0 as a polynomial identity, so
+// it holds in wrapping int32 as well.  With multisampling each sample of the
+// pass is tested at centre + offset (Core/Rasterizer.h:243-266):
+// e + ox B + oy C, wrapping as the reference.
 template <bool kMS>
-__device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2, int n0, int n1, bool useEdges,
+__device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2, uint32_t klo, bool useEdges,
                                           uint64_t* slot, const SamplePass& sp) {
+    const int n0 = e1 - t.pb1, n1 = e2 - t.pb2;
     if (!kMS) {
         if (useEdges && (e0 | e1 | e2) < 0) return;
         const float l0 = (float)n0 * t.inv_det;
@@ -1162,7 +1166,7 @@ __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2,
         const float l2 = 1.0f - l0 - l1;
         const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
         if (!(d <= 1.0f)) return;   // never passes d <= D with D <= 1 (NaN included)
-        const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - t.order);
+        const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)klo;
         if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
         return;
     }
@@ -1179,7 +1183,7 @@ __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2,
         const float l2 = 1.0f - l0 - l1;
         const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
         if (!(d <= 1.0f)) continue;
-        const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)(0xFFFFFFFEu - t.order);
+        const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)klo;
         uint64_t* sl = slot + k * (kTileSize * kTileSize);
         if (key < *sl) atomicMin((unsigned long long*)sl, (unsigned long long)key);
     }
@@ -1187,8 +1191,8 @@ __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2,
 
 // Pixels (qx + dx, qy + dy), dx, dy in {0, 1}, that lie within (maxX, maxY):
 // the 2x2 quad at (qx, qy) (one SSE lane per pixel in the reference).  Edge
-// functions and barycentric numerators are evaluated at (qx, qy)'s centre
-// (16x+8, 16y+8) and stepped by 16 B and 16 C to the other pixels, which is
+// functions are evaluated at (qx, qy)'s centre (16x+8, 16y+8) and stepped by
+// 16 B and 16 C to the other pixels, which is
 // exact in wrapping int32 arithmetic (the reference steps the same way,
 // Core/Rasterizer.h:190-198).
 template <bool kMS>
@@ -1198,18 +1202,16 @@ __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool u
     const int e0 = t.B0 * (px - t.x0) + t.C0 * (py - t.y0) + t.pb0;
     const int e1 = t.B1 * (px - t.x1) + t.C1 * (py - t.y1) + t.pb1;
     const int e2 = t.B2 * (px - t.x2) + t.C2 * (py - t.y2) + t.pb2;
-    const int n0 = t.B1 * (px - t.x2) + t.C1 * (py - t.y2);
-    const int n1 = t.B2 * (px - t.x2) + t.C2 * (py - t.y2);
     const int sB0 = t.B0 * 16, sB1 = t.B1 * 16, sB2 = t.B2 * 16;
     const int sC0 = t.C0 * 16, sC1 = t.C1 * 16, sC2 = t.C2 * 16;
+    const uint32_t klo = 0xFFFFFFFEu - t.order;
     uint64_t* row = &keys[(qy - tminY) * kTileSize + (qx - tminX)];
     const bool right = qx + 1 <= maxX, down = qy + 1 <= maxY;
-    pixel_key<kMS>(t, e0, e1, e2, n0, n1, useEdges, row, sp);
-    if (right) pixel_key<kMS>(t, e0 + sB0, e1 + sB1, e2 + sB2, n0 + sB1, n1 + sB2, useEdges, row + 1, sp);
-    if (down) pixel_key<kMS>(t, e0 + sC0, e1 + sC1, e2 + sC2, n0 + sC1, n1 + sC2, useEdges, row + kTileSize, sp);
+    pixel_key<kMS>(t, e0, e1, e2, klo, useEdges, row, sp);
+    if (right) pixel_key<kMS>(t, e0 + sB0, e1 + sB1, e2 + sB2, klo, useEdges, row + 1, sp);
+    if (down) pixel_key<kMS>(t, e0 + sC0, e1 + sC1, e2 + sC2, klo, useEdges, row + kTileSize, sp);
     if (right && down)
-        pixel_key<kMS>(t, e0 + sB0 + sC0, e1 + sB1 + sC1, e2 + sB2 + sC2, n0 + sB1 + sC1, n1 + sB2 + sC2, useEdges,
-                       row + kTileSize + 1, sp);
+        pixel_key<kMS>(t, e0 + sB0 + sC0, e1 + sB1 + sC1, e2 + sB2 + sC2, klo, useEdges, row + kTileSize + 1, sp);
 }
 template <bool kMS>
 __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
@@ -1715,9 +1717,11 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
 }  // namespace
 
 // Persistent raster kernel: workgroups pull (tile, segment) items from the
-// list built by k_bin_tilescan until it is exhausted.
+// list built by k_bin_tilescan until it is exhausted.  The single-sample
+// kernel is held to 96 VGPRs so that five workgroups per CU, as many as its LDS
+// allows, are resident.
 template <bool kMS>
-__global__ __launch_bounds__(kBlock) void k_raster(BatchParams Q, const FrameParams* __restrict__ jobs,
+__global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                    DeviceBuffers B) {
     __shared__ uint64_t keys[(kMS ? kPassSamples : 1) * kTileSize * kTileSize];
     __shared__ uint16_t sOwner[kBlock / kWave][kOwnerCap];
