@@ -500,6 +500,9 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.tile_item = c->tileItem.p;
         B.partials = c->partials.p;
         B.raster_blocks = rasterBlocks;
+        // Workgroups beyond the resident capacity would start only when a
+        // resident one exits, i.e. after the work queue has drained.
+        B.raster_grid = std::min(rasterBlocks, c->numCUs * raster_resident_per_cu(Q.max_samples > 1));
         B.counters = c->counters.p;
         B.color = color;
         B.depth = depth;

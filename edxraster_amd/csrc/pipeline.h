@@ -221,9 +221,10 @@ struct DeviceBuffers {
     uint32_t ref_cap;
     uint32_t* work;        // raster work items: global tile | segment << kTileBits
     uint32_t work_cap;
-    uint32_t* tile_item;   // per global tile: index of its first work item
+    uint32_t* tile_item;   // per global tile: index of its first key image (one per segment)
     uint64_t* partials;    // work_cap * samples * 1024 (depth, order) keys, one tile image per item and sample
-    uint32_t raster_blocks;
+    uint32_t raster_blocks;  // workgroups the segment size is chosen for
+    uint32_t raster_grid;    // launched raster workgroups: raster_blocks, at most the resident capacity
     uint32_t* counters;
     uint32_t* color;       // every job's pixels at its pix_base (resolved)
     float* depth;          // samples planes of the job's pixels (single frame), sample-major
@@ -256,7 +257,9 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
 hipError_t launch_adjoint(const BatchParams& Q, const FrameParams* jobs, const GradPtrs* grads,
                           const ShadeVert* verts, const uint32_t* ids, const float* d_rgb, double* acc,
                           hipStream_t s);
-// Kernel attributes (dynamic LDS limits); once per process before launches.
+// Kernel attributes (dynamic LDS limits) and occupancy; once per process before launches.
 hipError_t init_kernels();
+// Workgroups of the persistent raster kernel that one CU holds at once.
+uint32_t raster_resident_per_cu(bool multisample);
 
 }  // namespace edxr

@@ -881,6 +881,39 @@ __device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* wsum, uint32_
     return base + x - v;
 }
 
+// Exclusive scans of kN values at once over the 1024 threads of a block:
+// one pair of barriers for all of them.
+template <int kN>
+__device__ __forceinline__ void scan1024_n(uint32_t (&v)[kN], uint32_t (*wsum)[16], uint32_t (&total)[kN]) {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
+    uint32_t x[kN];
+#pragma unroll
+    for (int i = 0; i < kN; i++) x[i] = v[i];
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+        for (int i = 0; i < kN; i++) {
+            const uint32_t y = __shfl_up(x[i], d);
+            if (lane >= (uint32_t)d) x[i] += y;
+        }
+    }
+    if (lane == kWave - 1) {
+#pragma unroll
+        for (int i = 0; i < kN; i++) wsum[i][wave] = x[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kN; i++) {
+        uint32_t base = 0, tot = 0;
+        for (uint32_t w = 0; w < 16; w++) {
+            if (w < wave) base += wsum[i][w];
+            tot += wsum[i][w];
+        }
+        total[i] = tot;
+        v[i] = base + x[i] - v[i];
+    }
+    __syncthreads();
+}
+
 // Raster work items of a tile with n references (an empty tile has none).
 __device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) {
     return (n + (1u << segLog2) - 1) >> segLog2;
@@ -893,8 +926,14 @@ __device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) 
 // 2048 that gives no more items than the raster grid has workgroups (beyond
 // one per tile), so a frame with few references is still spread over the
 // whole grid instead of being bound by one workgroup walking a deep tile.
+// Items are listed longest first, so that the last ones claimed are short and
+// the pass does not end on one workgroup finishing a full segment: every full
+// segment, then the tiles' last (partial) segments in four size classes,
+// largest first.  A segment's key image is at the tile's image base
+// (tile_item) plus its segment index, whatever its place in the list.
 __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuffers B) {
     __shared__ uint32_t wsum[16];
+    __shared__ uint32_t sScan5[5][16];
     __shared__ uint32_t sMax;
     const uint32_t nt = Q.total_tiles;
     const uint32_t per = (nt + 1023) / 1024;
@@ -931,24 +970,44 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     // a tile's segment index must fit above the tile bits of a work item
     while (segLog2 < kSegLog2Max && tile_segments(maxN, segLog2) > (1u << (32 - kTileBits))) segLog2++;
     const bool segsFit = tile_segments(maxN, segLog2) <= (1u << (32 - kTileBits));
-    uint32_t localSegs = 0;
+    // Per thread: full segments (cnt[0]) and partial last segments by size
+    // class (cnt[1 + c], class c = remainder >> (segLog2 - 2)).
+    const uint32_t segMask = (1u << segLog2) - 1;
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};
+    auto count = [&](uint32_t n) {
+        cnt[0] += n >> segLog2;
+        const uint32_t r = n & segMask;
+        if (r) cnt[1 + (r >> (segLog2 - 2))]++;
+    };
     if (cached) {
 #pragma unroll
-        for (uint32_t k = 0; k < kCache; k++) localSegs += tile_segments(cache[k], segLog2);
+        for (uint32_t k = 0; k < kCache; k++) count(cache[k]);
     } else {
-        for (uint32_t t = t0; t < t1; t++) localSegs += tile_segments(B.tile_totals[t], segLog2);
+        for (uint32_t t = t0; t < t1; t++) count(B.tile_totals[t]);
     }
-    uint32_t totalSegs;
-    uint32_t seg = scan1024(localSegs, wsum, totalSegs);
+    uint32_t pos[5] = {cnt[0], cnt[1], cnt[2], cnt[3], cnt[4]}, tot[5];
+    scan1024_n<5>(pos, sScan5, tot);
+    const uint32_t totalSegs = tot[0] + tot[1] + tot[2] + tot[3] + tot[4];
+    // the thread's image base: segments of all earlier threads' tiles
+    uint32_t seg = pos[0] + pos[1] + pos[2] + pos[3] + pos[4];
+    // list order: full segments, then classes 3, 2, 1, 0
+    pos[4] += tot[0];
+    pos[3] += tot[0] + tot[4];
+    pos[2] += tot[0] + tot[4] + tot[3];
+    pos[1] += tot[0] + tot[4] + tot[3] + tot[2];
     const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && segsFit;
     auto place = [&](uint32_t t, uint32_t n) {
         B.tile_start[t] = run;
         run += n;
         if (fits) {
-            const uint32_t ns = tile_segments(n, segLog2);
+            const uint32_t nf = n >> segLog2, r = n & segMask;
             B.tile_item[t] = seg;
-            for (uint32_t s = 0; s < ns; s++) B.work[seg + s] = t | (s << kTileBits);
-            seg += ns;
+            for (uint32_t s = 0; s < nf; s++) B.work[pos[0]++] = t | (s << kTileBits);
+            if (r) {
+                const uint32_t c = 1 + (r >> (segLog2 - 2));
+                B.work[pos[c]++] = t | (nf << kTileBits);
+            }
+            seg += nf + (r ? 1u : 0u);
         }
     };
     if (cached) {
@@ -1708,7 +1767,7 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
             }
         }
         __syncthreads();
-        uint64_t* g = B.partials + ((size_t)workIndex * S + sp.base) * (kTileSize * kTileSize);
+        uint64_t* g = B.partials + ((size_t)(B.tile_item[gtile] + seg) * S + sp.base) * (kTileSize * kTileSize);
         for (uint32_t i = threadIdx.x; i < sp.n * kTileSize * kTileSize; i += kBlock) g[i] = keys[i];
         __syncthreads();
     }
@@ -1887,13 +1946,27 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+namespace {
+int g_rasterResident[2] = {0, 0};  // resident k_raster workgroups per CU: single-sample, multisample
+}
+
 hipError_t init_kernels() {
     // Bin cursors for up to kMaxTiles tiles live in LDS (160 KB per CU on gfx950).
     const int maxDyn = (int)(sizeof(uint32_t) * kMaxTiles);
     hipError_t e = hipFuncSetAttribute((const void*)k_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&g_rasterResident[0],
+                                                         reinterpret_cast<const void*>(&k_raster<false>), kBlock, 0);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&g_rasterResident[1],
+                                                         reinterpret_cast<const void*>(&k_raster<true>), kBlock, 0);
     return e;
+}
+
+uint32_t raster_resident_per_cu(bool multisample) {
+    return (uint32_t)std::max(1, g_rasterResident[multisample ? 1 : 0]);
 }
 
 hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FrameParams* job0, const DeviceBuffers& B,
@@ -1927,8 +2000,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     mark(3);
     if (Q.total_chunks) hipLaunchKernelGGL(k_bin_scatter, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
     mark(4);
-    if (Q.max_samples > 1) hipLaunchKernelGGL((k_raster<true>), dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
-    else hipLaunchKernelGGL((k_raster<false>), dim3(B.raster_blocks), dim3(kBlock), 0, s, Q, jobs, B);
+    if (Q.max_samples > 1) hipLaunchKernelGGL((k_raster<true>), dim3(B.raster_grid), dim3(kBlock), 0, s, Q, jobs, B);
+    else hipLaunchKernelGGL((k_raster<false>), dim3(B.raster_grid), dim3(kBlock), 0, s, Q, jobs, B);
     if (Q.any_textured) hipLaunchKernelGGL((k_shade<true>), dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
     else hipLaunchKernelGGL((k_shade<false>), dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
     mark(5);
