@@ -861,26 +861,6 @@ __global__ __launch_bounds__(kBlock) void k_bin_colscan(BatchParams Q, const Fra
 }
 
 namespace {
-// Exclusive scan over the 1024 threads of a block (16 waves).
-__device__ __forceinline__ uint32_t scan1024(uint32_t v, uint32_t* wsum, uint32_t& total) {
-    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
-    uint32_t x = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    if (lane == kWave - 1) wsum[wave] = x;
-    __syncthreads();
-    uint32_t base = 0;
-    total = 0;
-    for (uint32_t w = 0; w < 16; w++) {
-        if (w < wave) base += wsum[w];
-        total += wsum[w];
-    }
-    __syncthreads();
-    return base + x - v;
-}
-
 // Exclusive scans of kN values at once over the 1024 threads of a block:
 // one pair of barriers for all of them.
 template <int kN>
@@ -932,13 +912,12 @@ __device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) 
 // largest first.  A segment's key image is at the tile's image base
 // (tile_item) plus its segment index, whatever its place in the list.
 __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuffers B) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t sScan5[5][16];
-    __shared__ uint32_t sMax;
+    __shared__ uint32_t sScan6[6][16];
+    __shared__ uint32_t sMax, sTotal;
     const uint32_t nt = Q.total_tiles;
     const uint32_t per = (nt + 1023) / 1024;
     const uint32_t t0 = min(nt, threadIdx.x * per), t1 = min(nt, t0 + per);
-    if (threadIdx.x == 0) sMax = 0;
+    if (threadIdx.x == 0) sMax = sTotal = 0;
     __syncthreads();
     // Up to 16 tiles per thread (16384 tiles) the totals are read once, all
     // loads in flight together, and kept in registers for the later passes.
@@ -961,10 +940,18 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
             localMax = max(localMax, n);
         }
     }
-    atomicMax(&sMax, localMax);
-    uint32_t total;
-    uint32_t run = scan1024(localRefs, wsum, total);  // ends with a barrier: sMax is final
-    const uint32_t maxN = sMax;
+    // Total and deepest tile: wave reductions, then one LDS atomic per wave.
+    uint32_t waveRefs = localRefs, waveMax = localMax;
+    for (int d = 1; d < 64; d <<= 1) {
+        waveRefs += __shfl_xor(waveRefs, d);
+        waveMax = max(waveMax, __shfl_xor(waveMax, d));
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&sTotal, waveRefs);
+        atomicMax(&sMax, waveMax);
+    }
+    __syncthreads();
+    const uint32_t total = sTotal, maxN = sMax;
     uint32_t segLog2 = kSegLog2Min;
     while (segLog2 < kSegLog2Max && (total >> segLog2) > B.raster_blocks) segLog2++;
     // a tile's segment index must fit above the tile bits of a work item
@@ -985,8 +972,10 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     } else {
         for (uint32_t t = t0; t < t1; t++) count(B.tile_totals[t]);
     }
-    uint32_t pos[5] = {cnt[0], cnt[1], cnt[2], cnt[3], cnt[4]}, tot[5];
-    scan1024_n<5>(pos, sScan5, tot);
+    // the tile-reference prefix rides along as a sixth value
+    uint32_t pos[6] = {cnt[0], cnt[1], cnt[2], cnt[3], cnt[4], localRefs}, tot[6];
+    scan1024_n<6>(pos, sScan6, tot);
+    uint32_t run = pos[5];
     const uint32_t totalSegs = tot[0] + tot[1] + tot[2] + tot[3] + tot[4];
     // the thread's image base: segments of all earlier threads' tiles
     uint32_t seg = pos[0] + pos[1] + pos[2] + pos[3] + pos[4];
