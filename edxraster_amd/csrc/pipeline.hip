@@ -1762,6 +1762,24 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
     }
 }
 
+// Cleared pixels of an empty tile (as k_shade writes uncovered ones).
+__device__ __forceinline__ void clear_tile(const BatchParams& Q, const FrameParams* __restrict__ jobs,
+                                           const DeviceBuffers& B, uint32_t gtile) {
+    const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
+    const uint32_t tile = gtile - P.tile_base;
+    const int tminX = (int)(tile % P.tiles_x) * kTileSize, tminY = (int)(tile / P.tiles_x) * kTileSize;
+    for (uint32_t i = threadIdx.x; i < kTileSize * kTileSize; i += kBlock) {
+        const int x = tminX + (int)(i % kTileSize), y = tminY + (int)(i / kTileSize);
+        if (x < (int)P.width && y < (int)P.height) {
+            const size_t o = P.pix_base + (size_t)(P.height - 1 - y) * P.width + x;
+            B.color[o] = 0u;
+            B.depth[o] = 1.0f;
+            B.ids[o] = 0xFFFFFFFFu;
+            if (B.color_f32) B.color_f32[3 * o] = B.color_f32[3 * o + 1] = B.color_f32[3 * o + 2] = 0.0f;
+        }
+    }
+}
+
 }  // namespace
 
 // Persistent raster kernel: workgroups pull (tile, segment) items from the
@@ -1776,6 +1794,22 @@ __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, c
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const uint32_t nItems = B.counters[kCntWorkItems], segLog2 = B.counters[kCntSegLog2];
+    // Single-sample batches: the empty tiles are cleared here, a share per
+    // workgroup, before any item is taken (k_shade skips them).  Lane l looks
+    // at tile blockIdx.x + l * gridDim.x of each pass; every wave computes the
+    // same mask.
+    if (!kMS) {
+        for (uint32_t base = blockIdx.x; base < Q.total_tiles; base += gridDim.x * kWave) {
+            const uint32_t t = base + __lane_id() * gridDim.x;
+            const bool empty = t < Q.total_tiles && B.tile_start[t + 1] == B.tile_start[t];
+            uint64_t m = __ballot(empty);
+            while (m) {
+                const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                clear_tile(Q, jobs, B, base + l * gridDim.x);
+            }
+        }
+    }
     // The first item is the workgroup's index (no burst of same-address
     // atomics at launch); later ones come from the shared counter.
     for (uint32_t round = 0;; round++) {
@@ -1792,7 +1826,8 @@ __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, c
 // winner is the minimum (depth, order) key over the tile's segments (the
 // sequential d <= D test is an associative minimum), then perspective-correct
 // interpolation, the pixel shader and one write of colour, depth and
-// triangle ID (Core/Renderer.cpp:272-350).
+// triangle ID (Core/Renderer.cpp:272-350).  Empty tiles of single-sample
+// batches are cleared by k_raster.
 namespace {
 // Record and shading-vertex slots of the winner with submission order `order`:
 // clipped pieces from their stored record, unclipped triangles rebuilt from
@@ -1847,13 +1882,14 @@ __global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams
     constexpr uint32_t kPix = kTileSize * kTileSize;
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const uint32_t gtile = blockIdx.x;
+    const uint32_t start = B.tile_start[gtile];
+    const uint32_t n = B.tile_start[gtile + 1] - start;
+    if (n == 0 && Q.max_samples == 1) return;  // an empty tile, cleared by k_raster
     const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
     const uint32_t tile = gtile - P.tile_base;
     const int x = (int)(tile % P.tiles_x) * kTileSize + (int)(threadIdx.x & 31u);
     const int y = (int)(tile / P.tiles_x) * kTileSize + (int)(threadIdx.x >> 5);
     if (x >= (int)P.width || y >= (int)P.height) return;
-    const uint32_t start = B.tile_start[gtile];
-    const uint32_t n = B.tile_start[gtile + 1] - start;
     const uint32_t nseg = tile_segments(n, B.counters[kCntSegLog2]), S = P.samples;
     const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * S * kPix + threadIdx.x;
     const size_t o = P.pix_base + (size_t)(P.height - 1 - y) * P.width + x;
