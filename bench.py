@@ -66,8 +66,8 @@ def cpu_baseline(w, budget_s):
 # float4 copy kernel of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
 # profiles/pmc_raster_shade_*.md.  None where not measured.
 RASTER_SHADE_TRAFFIC = {
-    "sphere_grid_16x16x64_1920x1080": 61485 * 1024,   # (6439 + 6796) KiB x 2 + (10714 + 24300) KiB
-    "heightfield_2236_3840x2160": 794299 * 1024,     # (135576 + 190098) KiB x 2 + (45746 + 97204) KiB
+    "sphere_grid_16x16x64_1920x1080": 70842 * 1024,   # (6533 + 9068) KiB x 2 + (28900 + 10740) KiB
+    "heightfield_2236_3840x2160": 799829 * 1024,     # (137462 + 190977) KiB x 2 + (105110 + 37842) KiB
 }
 
 
