@@ -458,7 +458,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
     const uint32_t refWant = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * Q.total_tiles + 1024, 0xFFFFFFF0ull);
     c->refCap = std::max(c->refCap, refWant);
 
-    // Persistent raster grid: enough resident workgroups per CU to hide latency.
+    // Raster work items per frame the segment size aims for (tuning).
     const uint32_t rasterBlocks = c->numCUs * (c->rasterPerCU ? c->rasterPerCU : 8);
     c->stats = edxr_stats{};
     for (uint32_t attempt = 0; attempt < 8; attempt++) {
@@ -500,9 +500,11 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.tile_item = c->tileItem.p;
         B.partials = c->partials.p;
         B.raster_blocks = rasterBlocks;
-        // Workgroups beyond the resident capacity would start only when a
-        // resident one exits, i.e. after the work queue has drained.
-        B.raster_grid = std::min(rasterBlocks, c->numCUs * raster_resident_per_cu(Q.max_samples > 1));
+        // The persistent raster grid is the resident capacity: workgroups
+        // beyond it would start only when a resident one exits, i.e. after the
+        // work queue has drained.  The tuning's workgroups per CU set the
+        // segment size only (raster_blocks).
+        B.raster_grid = c->numCUs * raster_resident_per_cu(Q.max_samples > 1);
         B.counters = c->counters.p;
         B.color = color;
         B.depth = depth;

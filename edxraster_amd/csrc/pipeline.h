@@ -223,8 +223,8 @@ struct DeviceBuffers {
     uint32_t work_cap;
     uint32_t* tile_item;   // per global tile: index of its first key image (one per segment)
     uint64_t* partials;    // work_cap * samples * 1024 (depth, order) keys, one tile image per item and sample
-    uint32_t raster_blocks;  // workgroups the segment size is chosen for
-    uint32_t raster_grid;    // launched raster workgroups: raster_blocks, at most the resident capacity
+    uint32_t raster_blocks;  // work items the segment size is chosen for
+    uint32_t raster_grid;    // launched raster workgroups: the resident capacity
     uint32_t* counters;
     uint32_t* color;       // every job's pixels at its pix_base (resolved)
     float* depth;          // samples planes of the job's pixels (single frame), sample-major
