@@ -86,7 +86,7 @@ def frame_leg(w, steps, warmup, brief=False, tuning=None, msaa=0):
     r.set_msaa_mode(msaa)
     w.apply(r)
     dm, keep = device_mesh(w.mesh)
-    # Launch tuning (triangles per binning chunk, raster workgroups per CU):
+    # Launch tuning (triangles per binning chunk, raster work items per CU):
     # given, or picked by the library's autotuner on this frame before timing.
     if tuning:
         r.set_tuning(*tuning)

@@ -89,7 +89,9 @@ enum edxr_state_key {
     EDXR_STATE_GRAPH = 4,
     /* Triangles per binning chunk (0 = automatic, 1024). */
     EDXR_STATE_TRIS_PER_CHUNK = 5,
-    /* Resident raster workgroups per compute unit (0 = automatic, 8). */
+    /* Raster work items per compute unit that the tile-segment size aims for
+     * (0 = automatic, 8); the raster grid itself is always as many
+     * workgroups as the GPU holds at once. */
     EDXR_STATE_RASTER_BLOCKS_PER_CU = 6,
     /* 1 = also write the clamped float RGB image (3 floats per pixel, the
      * function edxr_backward differentiates); default 0. */
@@ -254,7 +256,7 @@ int edxr_batch_read(edxr_ctx* ctx, uint32_t job, uint8_t* rgba, float* depth, ui
 /* Device addresses of the last batch's colour, depth and ID arrays and, when
  * pixel_offsets is non-NULL, the first pixel of every job (num_jobs + 1). */
 int edxr_batch_device_buffers(edxr_ctx* ctx, void** color, void** depth, void** ids, uint64_t* pixel_offsets);
-/* Times candidate (triangles per chunk, raster workgroups per CU) pairs on the
+/* Times candidate (triangles per chunk, raster work items per CU) pairs on the
  * given batch (or single job, num_jobs = 1), keeps the fastest in the
  * context's state and returns it in *out (may be NULL). */
 int edxr_autotune(edxr_ctx* ctx, const edxr_job* jobs, uint32_t num_jobs, edxr_tuning* out);
