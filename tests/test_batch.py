@@ -82,6 +82,29 @@ def test_batch_graph_replays_follow_camera_changes(built):
 
 
 @pytest.mark.gpu
+def test_single_job_graph_replays_follow_camera_changes(built):
+    """A single frame replayed from a captured graph takes its job (camera,
+    light) from host memory at run time: each replay equals the eager frame."""
+    w = configs.sphere_grid_1080p(320, 240, grid=4, segments=24)
+    eager = Renderer(w.width, w.height)
+    graph = Renderer(w.width, w.height)
+    graph.set_graph(True)
+    for step in range(4):
+        ang = 0.4 * step
+        w.light = (float(np.cos(ang)), 1.0, float(np.sin(ang)))
+        eye = (float(2.5 * np.sin(ang)), 1.5, float(-6.0 + 0.5 * step))
+        w.mv, w.proj, w.raster = configs.camera(w.width, w.height, eye, (0.0, 0.0, 0.0))
+        for r in (eager, graph):
+            w.apply(r)
+            r.render_mesh(w.mesh)
+        assert np.array_equal(eager.get_back_buffer(), graph.get_back_buffer()), "step %d: colour" % step
+        assert np.array_equal(eager.read_depth(), graph.read_depth()), "step %d: depth" % step
+        assert np.array_equal(eager.read_triangle_ids(), graph.read_triangle_ids()), "step %d: IDs" % step
+    eager.close()
+    graph.close()
+
+
+@pytest.mark.gpu
 def test_autotune_picks_a_candidate_and_keeps_results(built):
     jobs = small_batch()
     r = Renderer(64, 64)

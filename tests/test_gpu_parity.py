@@ -125,3 +125,24 @@ def test_clipping_stats_and_parity(built):
     assert st["refs"] == ost["refs"]
     assert st["clipped"] > 0
     r.close()
+
+
+def test_empty_frame_after_a_full_one(built):
+    """A frame without triangles is cleared tile by tile by the raster pass,
+    over the previous frame's pixels: colour, depth, IDs and float colour."""
+    import ctypes as C
+    from edxraster_amd import Mesh
+    w = configs.small_sphere(96, 64)
+    r = Renderer(w.width, w.height)
+    w.apply(r)
+    assert L.lib().edxr_set_state(r.ctx, L.STATE_FLOAT_COLOR, 1) == 0
+    r.render_mesh(w.mesh)
+    assert (r.read_triangle_ids() != 0xFFFFFFFF).sum() > 1000
+    r.render_mesh(Mesh(*w.mesh.planes(), w.mesh.indices[:0]))
+    assert (r.read_triangle_ids() == 0xFFFFFFFF).all()
+    assert (r.read_depth() == np.float32(1.0)).all()
+    assert (r.get_back_buffer() == 0).all()
+    f = np.full((w.height, w.width, 3), 7.0, np.float32)
+    assert L.lib().edxr_copy_float_color(r.ctx, C.c_void_p(f.ctypes.data), f.nbytes) == 0
+    assert (f == 0.0).all()
+    r.close()
