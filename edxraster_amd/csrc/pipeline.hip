@@ -1190,11 +1190,15 @@ __device__ __forceinline__ void bary_depth(const RTri& t, int px, int py, float&
 
 // LDS key image rows are one key longer than a tile row, so that vertically
 // adjacent pixels fall in different LDS banks.
-constexpr int kKeyStride = kTileSize + 1;
-constexpr uint32_t kKeyPlane = kTileSize * kKeyStride;
+// (Single-sample only: with four padded planes the multisample kernel ran with
+// one resident workgroup per CU fewer, and slower.)
+template <bool kMS>
+constexpr int kKeyStride = kMS ? kTileSize : kTileSize + 1;
+template <bool kMS>
+constexpr uint32_t kKeyPlane = kTileSize * kKeyStride<kMS>;
 
 // Samples of one raster pass: MSAA level, first sample and count; sample
-// k's keys live in plane k (kKeyPlane keys apart).
+// k's keys live in plane k (kKeyPlane<true> keys apart).
 struct SamplePass {
     uint32_t level, base, n;
 };
@@ -1237,7 +1241,7 @@ __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2,
         const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
         if (!(d <= 1.0f)) continue;
         const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)klo;
-        uint64_t* sl = slot + k * kKeyPlane;
+        uint64_t* sl = slot + k * kKeyPlane<kMS>;
         if (key < *sl) atomicMin((unsigned long long*)sl, (unsigned long long)key);
     }
 }
@@ -1258,13 +1262,13 @@ __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool u
     const int sB0 = t.B0 * 16, sB1 = t.B1 * 16, sB2 = t.B2 * 16;
     const int sC0 = t.C0 * 16, sC1 = t.C1 * 16, sC2 = t.C2 * 16;
     const uint32_t klo = 0xFFFFFFFEu - t.order;
-    uint64_t* row = &keys[(qy - tminY) * kKeyStride + (qx - tminX)];
+    uint64_t* row = &keys[(qy - tminY) * kKeyStride<kMS> + (qx - tminX)];
     const bool right = qx + 1 <= maxX, down = qy + 1 <= maxY;
     pixel_key<kMS>(t, e0, e1, e2, klo, useEdges, row, sp);
     if (right) pixel_key<kMS>(t, e0 + sB0, e1 + sB1, e2 + sB2, klo, useEdges, row + 1, sp);
-    if (down) pixel_key<kMS>(t, e0 + sC0, e1 + sC1, e2 + sC2, klo, useEdges, row + kKeyStride, sp);
+    if (down) pixel_key<kMS>(t, e0 + sC0, e1 + sC1, e2 + sC2, klo, useEdges, row + kKeyStride<kMS>, sp);
     if (right && down)
-        pixel_key<kMS>(t, e0 + sB0 + sC0, e1 + sB1 + sC1, e2 + sB2 + sC2, klo, useEdges, row + kKeyStride + 1, sp);
+        pixel_key<kMS>(t, e0 + sB0 + sC0, e1 + sB1 + sC1, e2 + sB2 + sC2, klo, useEdges, row + kKeyStride<kMS> + 1, sp);
 }
 template <bool kMS>
 __device__ __forceinline__ void quad_depth(const RTri& t, int qx, int qy, bool useEdges, int tminX, int tminY,
@@ -1604,7 +1608,7 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
     const uint32_t S = kMS ? P.samples : 1u;
     for (uint32_t pass = 0; pass * kPassSamples < S; pass++) {
         const SamplePass sp{P.sample_level, pass * kPassSamples, min(S - pass * kPassSamples, kPassSamples)};
-        for (uint32_t i = threadIdx.x; i < sp.n * kKeyPlane; i += kBlock) keys[i] = bg_key();
+        for (uint32_t i = threadIdx.x; i < sp.n * kKeyPlane<kMS>; i += kBlock) keys[i] = bg_key();
         __syncthreads();
 
         // Each wave takes slices of L references (64, or a quarter of a short
@@ -1764,7 +1768,7 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
         uint64_t* g = B.partials + ((size_t)(B.tile_item[gtile] + seg) * S + sp.base) * (kTileSize * kTileSize);
         for (uint32_t i = threadIdx.x; i < sp.n * kTileSize * kTileSize; i += kBlock) {
             const uint32_t plane = i / (kTileSize * kTileSize), p = i % (kTileSize * kTileSize);
-            g[i] = keys[plane * kKeyPlane + (p / kTileSize) * kKeyStride + p % kTileSize];
+            g[i] = keys[plane * kKeyPlane<kMS> + (p / kTileSize) * kKeyStride<kMS> + p % kTileSize];
         }
         __syncthreads();
     }
@@ -1797,7 +1801,7 @@ __device__ __forceinline__ void clear_tile(const BatchParams& Q, const FramePara
 template <bool kMS>
 __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                    DeviceBuffers B) {
-    __shared__ uint64_t keys[(kMS ? kPassSamples : 1) * kKeyPlane];
+    __shared__ uint64_t keys[(kMS ? kPassSamples : 1) * kKeyPlane<kMS>];
     __shared__ uint16_t sOwner[kBlock / kWave][kOwnerCap];
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
