@@ -1834,7 +1834,9 @@ __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, c
     }
 }
 
-// Deferred shading, one pixel per thread and one workgroup per tile: the
+// Deferred shading, one pixel per thread and two 512-thread workgroups per
+// tile (16 rows each; smaller workgroups than one per tile schedule better and
+// cost less where the tile is empty): the
 // winner is the minimum (depth, order) key over the tile's segments (the
 // sequential d <= D test is an associative minimum), then perspective-correct
 // interpolation, the pixel shader and one write of colour, depth and
@@ -1889,21 +1891,22 @@ __device__ __forceinline__ uint64_t tile_key(const uint64_t* g, uint32_t nseg, u
 }  // namespace
 
 template <bool kTex>
-__global__ __launch_bounds__(1024) void k_shade(BatchParams Q, const FrameParams* __restrict__ jobs,
+__global__ __launch_bounds__(512) void k_shade(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                  DeviceBuffers B) {
     constexpr uint32_t kPix = kTileSize * kTileSize;
     if (B.counters[kCntOverflow] & kOvRefs) return;
-    const uint32_t gtile = blockIdx.x;
+    const uint32_t gtile = blockIdx.x >> 1;
+    const uint32_t pix = threadIdx.x + (blockIdx.x & 1u) * 512u;  // pixel of the tile
     const uint32_t start = B.tile_start[gtile];
     const uint32_t n = B.tile_start[gtile + 1] - start;
     if (n == 0 && Q.max_samples == 1) return;  // an empty tile, cleared by k_raster
     const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
     const uint32_t tile = gtile - P.tile_base;
-    const int x = (int)(tile % P.tiles_x) * kTileSize + (int)(threadIdx.x & 31u);
-    const int y = (int)(tile / P.tiles_x) * kTileSize + (int)(threadIdx.x >> 5);
+    const int x = (int)(tile % P.tiles_x) * kTileSize + (int)(pix & 31u);
+    const int y = (int)(tile / P.tiles_x) * kTileSize + (int)(pix >> 5);
     if (x >= (int)P.width || y >= (int)P.height) return;
     const uint32_t nseg = tile_segments(n, B.counters[kCntSegLog2]), S = P.samples;
-    const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * S * kPix + threadIdx.x;
+    const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * S * kPix + pix;
     const size_t o = P.pix_base + (size_t)(P.height - 1 - y) * P.width + x;
     if (S == 1) {
         const uint64_t key = n ? tile_key(g, nseg, 1, 0) : bg_key();
@@ -2039,8 +2042,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     mark(4);
     if (Q.max_samples > 1) hipLaunchKernelGGL((k_raster<true>), dim3(B.raster_grid), dim3(kBlock), 0, s, Q, jobs, B);
     else hipLaunchKernelGGL((k_raster<false>), dim3(B.raster_grid), dim3(kBlock), 0, s, Q, jobs, B);
-    if (Q.any_textured) hipLaunchKernelGGL((k_shade<true>), dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
-    else hipLaunchKernelGGL((k_shade<false>), dim3(Q.total_tiles), dim3(1024), 0, s, Q, jobs, B);
+    if (Q.any_textured) hipLaunchKernelGGL((k_shade<true>), dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
+    else hipLaunchKernelGGL((k_shade<false>), dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
     mark(5);
     return hipGetLastError();
 }
