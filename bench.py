@@ -168,11 +168,22 @@ def frame_leg(w, steps, warmup, brief=False, tuning=None, msaa=0):
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": RASTER_SHADE_TRAFFIC.get(w.name) if S == 1 else None,
             "kernel": "k_raster+k_shade (raster/shade pass)", "algorithmic_bytes": b_rs, "kernel_ms": raster_ms}
-    out = {"workload": w.name, "ms": ms, "eager_ms": eager_ms, "stage_ms": stage_avg, "roofline": roof,
-           "stats": st, "v_ref": v_ref, "tuning": tuning}
+    # Compulsory bytes per stage (achieved GB/s = bytes / stage time):
+    # setup: 36 B in + 64 B out (shading and screen vertex) per vertex, 12 B
+    # indices + 4 B descriptor per triangle; bin count: 4 B descriptor per
+    # triangle + the chunk x tile count matrix written (4 B per entry); scans:
+    # the matrix read twice and written once; scatter: 6 B (descriptor and
+    # count) per triangle, the matrix read, 4 B per reference written.
+    V, C = w.mesh.num_vertices, int(st["bin_chunks"])
+    tiles = ((w.width + 31) // 32) * ((w.height + 31) // 32)
+    stage_bytes = {"setup": 100 * V + 16 * T, "bin_count": 4 * T + 4 * C * tiles, "bin_scan": 12 * C * tiles,
+                   "bin_scatter": 6 * T + 4 * C * tiles + 4 * n_ref, "raster": b_rs}
+    stage_gbs = {k: stage_bytes[k] / (stage_avg[k] * 1e-3) / 1e9 for k in stage_bytes if stage_avg.get(k)}
+    out = {"workload": w.name, "ms": ms, "eager_ms": eager_ms, "stage_ms": stage_avg, "stage_gbs": stage_gbs,
+           "roofline": roof, "stats": st, "v_ref": v_ref, "tuning": tuning}
     if brief:
         out = {"workload": w.name, "samples": S, "triangles": T, "frame_ms": ms, "mtris_per_s": T / (ms * 1e-3) / 1e6,
-               "stage_ms": stage_avg, "roofline": roof, "tuning": tuning}
+               "stage_ms": stage_avg, "stage_gbs": stage_gbs, "roofline": roof, "tuning": tuning}
     return out
 
 
@@ -329,6 +340,7 @@ def main():
         "eager_frame_ms": fr["eager_ms"],
         "tuning": fr["tuning"],
         "stage_ms": stage_avg,
+        "stage_gbs": fr["stage_gbs"],
         "dominant_stage": max(stage_avg, key=stage_avg.get),
         "roofline": fr["roofline"],
         "frame_roofline_frac": b_frame / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
