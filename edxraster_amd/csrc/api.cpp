@@ -190,6 +190,7 @@ struct edxr_ctx {
     DevBuf<float> depth;
     DevBuf<uint64_t> partials;
     DevBuf<uint16_t> slotRefs;
+    DevBuf<uint32_t> chunkBox;
     DevBuf<uint32_t> groupSums;
     DevBuf<ShadeVert> verts;  // shading vertices + extraVertCap clip-created ones
     DevBuf<ScreenVert> sverts;
@@ -467,6 +468,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
             (e = c->srec.ensure((size_t)T + c->extraRecCap)) != hipSuccess ||
             (e = c->desc.ensure(std::max<uint32_t>(T, 1))) != hipSuccess ||
             (e = c->slotRefs.ensure(std::max<uint32_t>(T, 1))) != hipSuccess ||
+            (e = c->chunkBox.ensure(2 * std::max<uint32_t>(Q.total_chunks, 1))) != hipSuccess ||
             (e = c->groupSums.ensure(std::max<uint64_t>(pl.groupEntries, 1))) != hipSuccess ||
             (e = c->clipQueue.ensure(c->clipQueueCap)) != hipSuccess ||
             (e = c->binCounts.ensure(std::max<uint64_t>(pl.countEntries, 1))) != hipSuccess ||
@@ -483,6 +485,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.srec = c->srec.p;
         B.desc = c->desc.p;
         B.slot_refs = c->slotRefs.p;
+        B.chunk_box = c->chunkBox.p;
         B.group_sums = c->groupSums.p;
         B.clip_queue = c->clipQueue.p;
         B.clip_queue_cap = c->clipQueueCap;
@@ -716,6 +719,7 @@ void edxr_destroy(edxr_ctx* c) {
     c->adjAcc.release();
     c->partials.release();
     c->slotRefs.release();
+    c->chunkBox.release();
     c->verts.release();
     c->sverts.release();
     c->jobsDev.release();

@@ -213,6 +213,8 @@ struct DeviceBuffers {
     ScreenVert* sverts;    // V screen vertices (input vertices only)
     uint32_t extra_vert_cap;
     uint16_t* slot_refs;   // per input triangle: tile references (saturating)
+    uint32_t* chunk_box;   // per chunk of the batch: tile box of its references,
+                           // (min x | min y << 16, max x | max y << 16); empty: min > max
     uint32_t* bin_counts;  // num_chunks * num_tiles, scanned in place
     uint32_t* group_sums;  // (num_chunks / kChunkGroup) * num_tiles
     uint32_t* tile_totals;

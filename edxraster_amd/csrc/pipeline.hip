@@ -715,10 +715,16 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* w
 __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                       DeviceBuffers B) {
     extern __shared__ uint32_t hist[];
+    __shared__ uint32_t sBox[4];  // min x, min y, max x, max y of the chunk's tiles
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;  // chunk of the job
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) hist[i] = 0;
+    if (threadIdx.x == 0) {
+        sBox[0] = sBox[1] = 0xFFFFu;
+        sBox[2] = sBox[3] = 0u;
+    }
     __syncthreads();
+    uint32_t bx0 = 0xFFFFu, by0 = 0xFFFFu, bx1 = 0u, by1 = 0u;  // this thread's references
     const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
     const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
     // Four slots per thread and step, their descriptors loaded together so
@@ -738,6 +744,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
                 // at most 2x2 tiles, no corner tests
                 R = compact_refs(d[u]);
                 for (uint32_t k = 0; k < R; k++) atomicAdd(&hist[compact_tile(d[u], k, tilesX)], 1u);
+                const uint32_t tx = d[u] & 0x3FFFu, ty = (d[u] >> 14) & 0x3FFFu;
+                bx0 = min(bx0, tx);
+                by0 = min(by0, ty);
+                bx1 = max(bx1, tx + ((d[u] >> 28) & 1u));
+                by1 = max(by1, ty + ((d[u] >> 29) & 1u));
             } else {
                 RefEnum e;
                 e.init(d[u], s);
@@ -745,13 +756,35 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
                 while (e.next(P, B.rrec, tile, payload)) {
                     atomicAdd(&hist[tile], 1u);
                     R++;
+                    const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
+                    bx0 = min(bx0, tx);
+                    by0 = min(by0, ty);
+                    bx1 = max(bx1, tx);
+                    by1 = max(by1, ty);
                 }
             }
             // Saturated counts are recomputed by the scatter.
             B.slot_refs[s] = (uint16_t)min(R, (uint32_t)kSlotRefsSaturated);
         }
     }
+    // The chunk's tile box (k_bin_scatter sets up cursors only inside it).
+    for (int o = 1; o < kWave; o <<= 1) {
+        bx0 = min(bx0, (uint32_t)__shfl_xor(bx0, o));
+        by0 = min(by0, (uint32_t)__shfl_xor(by0, o));
+        bx1 = max(bx1, (uint32_t)__shfl_xor(bx1, o));
+        by1 = max(by1, (uint32_t)__shfl_xor(by1, o));
+    }
+    if (__lane_id() == 0) {
+        atomicMin(&sBox[0], bx0);
+        atomicMin(&sBox[1], by0);
+        atomicMax(&sBox[2], bx1);
+        atomicMax(&sBox[3], by1);
+    }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        B.chunk_box[2 * blockIdx.x] = sBox[0] | (sBox[1] << 16);
+        B.chunk_box[2 * blockIdx.x + 1] = sBox[2] | (sBox[3] << 16);
+    }
     uint32_t* row = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) row[i] = hist[i];
 }
@@ -1027,7 +1060,19 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;
     const uint32_t* offs = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
-    for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) cursor[i] = B.tile_start[P.tile_base + i] + offs[i];
+    {
+        // Cursors of the tiles inside the chunk's tile box: every reference of
+        // the chunk falls in it (k_bin_count).
+        const uint32_t b0 = B.chunk_box[2 * blockIdx.x], b1 = B.chunk_box[2 * blockIdx.x + 1];
+        const uint32_t x0 = b0 & 0xFFFFu, y0 = b0 >> 16, x1 = b1 & 0xFFFFu, y1 = b1 >> 16;
+        if (x0 <= x1 && y0 <= y1) {
+            const uint32_t bw = x1 - x0 + 1, nb = bw * (y1 - y0 + 1);
+            for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
+                const uint32_t t = (y0 + i / bw) * P.tiles_x + x0 + i % bw;
+                cursor[t] = B.tile_start[P.tile_base + t] + offs[t];
+            }
+        }
+    }
     __syncthreads();
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
