@@ -105,8 +105,8 @@ def copy_bandwidth_gbs(nbytes=1 << 30, reps=10):
 # float4 copy kernel of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
 # profiles/pmc_raster_shade_*.md.  None where not measured.
 RASTER_SHADE_TRAFFIC = {
-    "sphere_grid_16x16x64_1920x1080": 61775 * 1024,   # (raster + shade FETCH) KiB x 2 + WRITE KiB, tuning 512,4
-    "heightfield_2236_3840x2160": 839340 * 1024,     # the same at tuning 4096,8
+    "sphere_grid_16x16x64_1920x1080": 61792 * 1024,   # (raster + shade FETCH) KiB x 2 + WRITE KiB, tuning 512,4
+    "heightfield_2236_3840x2160": 840512 * 1024,     # the same at tuning 4096,8
 }
 
 
