@@ -38,6 +38,7 @@ STATE_FLOAT_COLOR = 7
 STATE_MSAA = 8
 STATE_WRITE_FRAMES = 9
 STATE_TEXTURE_FILTER = 10
+STATE_DEBUG_CAPACITY = 11
 STAGES = ("setup", "bin_count", "bin_scan", "bin_scatter", "raster")
 
 MEM_HOST = 0
@@ -117,6 +118,7 @@ SIGNATURES = [
     ("edxr_profile_read", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("edxr_profile_reset", C.c_int, [C.c_void_p]),
     ("edxr_stream", C.c_void_p, [C.c_void_p]),
+    ("edxr_render_generation", C.c_uint64, [C.c_void_p]),
     ("edxr_last_error", C.c_char_p, [C.c_void_p]),
     ("edxr_render_batch", C.c_int, [C.c_void_p, C.POINTER(Job), C.c_uint32]),
     ("edxr_batch_read", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -149,6 +149,66 @@ def textured_scene(width=640, height=480, n=96, tex_filter=L.FILTER_TRILINEAR, s
                     shader=L.SHADER_TEXTURED_LAMBERT, shininess=0.0, textures=tex, tex_filter=tex_filter)
 
 
+def _front_facing(v, mvp):
+    """Orders each triangle of v (T, 3, 3) clockwise on screen (the winding
+    that survives the det > 0 cull) when all its vertices are in front of the
+    eye; other triangles are left as given."""
+    P = np.concatenate([v, np.ones(v.shape[:2] + (1,))], 2) @ np.asarray(mvp, np.float64).T
+    w = P[..., 3]
+    sx, sy = P[..., 0] / w, -P[..., 1] / w  # y down on screen
+    det = (sx[:, 1] - sx[:, 0]) * (sy[:, 2] - sy[:, 0]) - (sy[:, 1] - sy[:, 0]) * (sx[:, 2] - sx[:, 0])
+    flip = (det < 0) & (w > 0).all(1)
+    v[flip] = v[flip][:, [0, 2, 1]]
+    return v
+
+
+def clip_scene(kind, width=160, height=120, seed=0):
+    """Scenes whose triangles straddle the clip planes (Core/Clipper.h:71-288).
+
+    kind "lateral": large triangles at depth 1.5..4.5 in front of an eye at
+    (0, 0, -3), spreading beyond the left, right, bottom and top planes, no
+    vertex near the near or far plane.  "far": the same camera with the far
+    plane at view depth 6 and triangles from depth 1.5 to 9, so they cross it.
+    "all": four triangles around the eye that cross the near, far and side
+    planes at once (clip polygons of seven and eight vertices), in both
+    windings, with the far plane at view depth 30.  Vertex normals and colours are seeded per vertex, and
+    front faces are made clockwise where a triangle lies in front of the eye.
+    Returns a Workload."""
+    rng = np.random.Generator(np.random.PCG64(9100 + seed + {"lateral": 0, "far": 10, "all": 20}[kind]))
+    eye, at = (0.0, 0.0, -3.0), (0.0, 0.0, 0.0)
+    far = {"lateral": FAR, "far": 6.0, "all": 30.0}[kind]
+    mv = look_at(eye, at)
+    p = perspective(FOV, width / height, NEAR, far)
+    r = raster_matrix(width, height)
+    if kind == "lateral":
+        n = 24
+        c = np.stack([rng.uniform(-2.5, 2.5, n), rng.uniform(-2.0, 2.0, n), rng.uniform(-1.3, 1.3, n)], 1)
+        v = c[:, None, :] + rng.normal(0.0, 1.6, (n, 3, 3)) * np.array([1.0, 1.0, 0.25])
+        v[..., 2] = np.clip(v[..., 2], -1.5, 1.5)
+    elif kind == "far":
+        n = 24
+        c = np.stack([rng.uniform(-1.2, 1.2, n), rng.uniform(-0.9, 0.9, n), rng.uniform(1.0, 4.0, n)], 1)
+        v = c[:, None, :] + rng.normal(0.0, 1.0, (n, 3, 3)) * np.array([1.0, 1.0, 2.5])
+        v[..., 2] = np.clip(v[..., 2], -1.5, 6.0)
+    else:
+        # Each crosses all six planes (clip polygons of 7-8 vertices); both
+        # windings are included, so one of each pair survives the cull.
+        v = np.array([[[9.074, 0.241, 9.2], [-1.0, 0.63, -4.97], [-3.408, -20.731, 33.659]],
+                      [[-2.068, 0.011, -4.996], [19.598, -34.443, 39.561], [21.456, 15.514, 24.759]],
+                      [[-20.488, -12.724, 29.12], [0.161, -0.044, -3.223], [2.515, 16.773, 18.458]],
+                      [[-24.562, -8.192, 31.236], [16.651, -12.3, 26.239], [0.108, 0.888, -5.33]]])
+        v = np.concatenate([v, v[:, [0, 2, 1]]])
+    mvp = np.asarray(p, np.float64) @ np.asarray(mv, np.float64)
+    v = _front_facing(v, mvp).reshape(-1, 3).astype(np.float32)
+    nrm = rng.normal(0.0, 1.0, v.shape).astype(np.float32)
+    nrm[:, 2] = -np.abs(nrm[:, 2]) - 0.5  # mostly toward the eye
+    col = rng.uniform(0.0, 1.0, v.shape).astype(np.float32)
+    mesh = Mesh(v[:, 0], v[:, 1], v[:, 2], nrm[:, 0], nrm[:, 1], nrm[:, 2], col[:, 0], col[:, 1], col[:, 2],
+                np.arange(v.shape[0], dtype=np.uint32))
+    return Workload("clip_%s_%dx%d" % (kind, width, height), width, height, mesh, mv, p, r,
+                    info={"far": far})
+
+
 def _unit(rng):
     v = rng.normal(size=3)
     return v / np.linalg.norm(v)

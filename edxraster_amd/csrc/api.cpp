@@ -214,6 +214,8 @@ struct edxr_ctx {
 
     uint32_t numCUs = 256;
     uint32_t clipQueueCap = 0, extraRecCap = 0, extraVertCap = 0, refCap = 0;
+    uint32_t debugCapacity = 0;     // EDXR_STATE_DEBUG_CAPACITY
+    uint64_t renderGeneration = 0;  // advanced by every render_mesh / render_batch
     uint32_t* hostCounters = nullptr;
 
     // captured frame
@@ -344,7 +346,7 @@ int build_plan(edxr_ctx* c, const std::vector<JobInput>& in, Plan& pl) {
     uint64_t vert = 0, slot = 0, tile = 0, chunk = 0;
     pl.countEntries = pl.groupEntries = pl.pixels = 0;
     BatchParams& Q = pl.Q;
-    Q = BatchParams{};
+    memset(&Q, 0, sizeof(Q));  // padding included: part of the graph key
     for (const JobInput& j : in) {
         uint32_t tx, ty;
         if (!tile_grid(j.W, j.H, tx, ty)) return c->fail(EDXR_E_LIMIT, "job resolution out of range");
@@ -453,11 +455,17 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         Q.tile_job = Q.chunk_job + Q.total_chunks;
     }
     const uint32_t T = Q.total_slots, V = Q.total_verts;
-    if (!c->clipQueueCap) c->clipQueueCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16));
-    if (!c->extraRecCap) c->extraRecCap = 4096;
-    if (!c->extraVertCap) c->extraVertCap = 8192;
-    const uint32_t refWant = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * Q.total_tiles + 1024, 0xFFFFFFF0ull);
-    c->refCap = std::max(c->refCap, refWant);
+    if (c->debugCapacity) {
+        // Every frame starts from tiny capacities, so the overflow flags and
+        // the grow-and-rerun loop below run (EDXR_STATE_DEBUG_CAPACITY).
+        c->clipQueueCap = c->extraRecCap = c->extraVertCap = c->refCap = c->debugCapacity;
+    } else {
+        if (!c->clipQueueCap) c->clipQueueCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16));
+        if (!c->extraRecCap) c->extraRecCap = 4096;
+        if (!c->extraVertCap) c->extraVertCap = 8192;
+        const uint32_t refWant = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * Q.total_tiles + 1024, 0xFFFFFFF0ull);
+        c->refCap = std::max(c->refCap, refWant);
+    }
 
     // Raster work items per frame the segment size aims for (tuning).
     const uint32_t rasterBlocks = c->numCUs * (c->rasterPerCU ? c->rasterPerCU : 8);
@@ -480,7 +488,10 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
             (e = c->sverts.ensure(std::max<uint32_t>(V, 1))) != hipSuccess)
             return c->hip_fail(e, "hipMalloc(pipeline)");
 
+        // Zeroed as a whole: the structure's bytes, padding included, are
+        // the graph key (launch_key).
         DeviceBuffers B;
+        memset(&B, 0, sizeof(B));
         B.rrec = c->rrec.p;
         B.srec = c->srec.p;
         B.desc = c->desc.p;
@@ -801,6 +812,11 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
             c->msaaLog2 = value;
             return set_size(c, c->W, c->H);
         }
+        case EDXR_STATE_DEBUG_CAPACITY:
+            if (value < 0 || value > 65536) return c->fail(EDXR_E_INVALID, "debug capacity out of range (0..65536)");
+            c->debugCapacity = (uint32_t)value;
+            if (!value) c->clipQueueCap = c->extraRecCap = c->extraVertCap = 0;  // back to the defaults
+            return EDXR_OK;
         case EDXR_STATE_RASTER_BLOCKS_PER_CU:
             if (value < 0 || value > 32) return c->fail(EDXR_E_INVALID, "raster blocks per CU out of range (0..32)");
             c->rasterPerCU = (uint32_t)value;
@@ -870,6 +886,7 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     in.texFilter = c->texFilter;
     Plan pl;
     if ((r = build_plan(c, std::vector<JobInput>{in}, pl)) != EDXR_OK) return r;
+    c->renderGeneration++;
     c->rendered = false;
     c->lastPlanValid = false;
     if (c->floatColor && (e = c->colorF.ensure(3ull * c->W * c->H)) != hipSuccess)
@@ -1026,6 +1043,7 @@ int edxr_render_batch(edxr_ctx* c, const edxr_job* jobs, uint32_t n) {
         return c->hip_fail(e, "hipMalloc(batch frame buffers)");
     if (c->floatColor && (e = c->bcolorF.ensure(3 * pl.pixels)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(batch float colour)");
+    c->renderGeneration++;
     c->batchRendered = false;
     c->rendered = false;
     c->lastPlanValid = false;
@@ -1242,6 +1260,8 @@ int edxr_profile_reset(edxr_ctx* c) {
 }
 
 void* edxr_stream(edxr_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+uint64_t edxr_render_generation(const edxr_ctx* c) { return c ? c->renderGeneration : 0; }
 
 const char* edxr_last_error(const edxr_ctx* c) { return c ? c->err.c_str() : "null context"; }
 

@@ -23,9 +23,33 @@ def _ptr(t):
     return C.cast(C.c_void_p(t.data_ptr()), L.F)
 
 
+PLANE_NAMES = ("px", "py", "pz", "nx", "ny", "nz", "cr", "cg", "cb")
+
+
+def check_inputs(indices, planes):
+    """Raises TypeError / ValueError unless the mesh tensors are what the
+    kernels read through raw pointers: int32 indices (3 per triangle) and
+    nine float32 planes of equal length, all on one CUDA device."""
+    if indices.dtype != torch.int32:
+        raise TypeError("indices must be int32 (got %s)" % indices.dtype)
+    if indices.dim() != 1 or indices.numel() % 3:
+        raise ValueError("indices must be a flat tensor of 3 entries per triangle")
+    V = planes[0].numel()
+    for name, t in zip(PLANE_NAMES, planes):
+        if t.dtype != torch.float32:
+            raise TypeError("%s must be float32 (got %s)" % (name, t.dtype))
+        if t.dim() != 1 or t.numel() != V:
+            raise ValueError("%s must be a flat tensor of %d values (got shape %s)" % (name, V, tuple(t.shape)))
+    dev = planes[0].device
+    for name, t in zip(("indices",) + PLANE_NAMES, (indices,) + tuple(planes)):
+        if t.device.type != "cuda" or t.device != dev:
+            raise ValueError("%s must be on the CUDA device of px (got %s)" % (name, t.device))
+
+
 class _RenderViews(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rend, views, shader, indices, px, py, pz, nx, ny, nz, cr, cg, cb, light, material):
+        check_inputs(indices, (px, py, pz, nx, ny, nz, cr, cg, cb))
         planes = [t.contiguous() for t in (px, py, pz, nx, ny, nz, cr, cg, cb)]
         idx = indices.contiguous()
         V, T = planes[0].numel(), idx.numel() // 3
@@ -44,6 +68,9 @@ class _RenderViews(torch.autograd.Function):
         out = torch.empty((len(views), H, W, 3), dtype=torch.float32, device=px.device)
         _check(rend.ctx, L.lib().edxr_copy_float_color(rend.ctx, C.c_void_p(out.data_ptr()), out.numel() * 4))
         ctx.rend = rend
+        # edxr_backward differentiates the context's last render: remember
+        # which one this image is.
+        ctx.generation = L.lib().edxr_render_generation(rend.ctx)
         ctx.njobs = len(views)
         ctx.V = V
         ctx.light_device = light.device
@@ -53,6 +80,10 @@ class _RenderViews(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         (px,) = ctx.saved_tensors
+        if L.lib().edxr_render_generation(ctx.rend.ctx) != ctx.generation:
+            raise RuntimeError("the renderer has rendered again since this image was made; the backward pass "
+                               "differentiates only the renderer's last render (use one Renderer per image "
+                               "kept for backward)")
         dev = px.device
         g = grad_out.contiguous().float()
         gv = torch.zeros((9, ctx.V), dtype=torch.float32, device=dev)

@@ -106,7 +106,13 @@ enum edxr_state_key {
      * counter (Core/Renderer.cpp:114-117, 352-358).  Default off. */
     EDXR_STATE_WRITE_FRAMES = 9,
     /* Renderer::SetTextureFilter: EDXR_FILTER_*; default trilinear. */
-    EDXR_STATE_TEXTURE_FILTER = 10
+    EDXR_STATE_TEXTURE_FILTER = 10,
+    /* Testing aid (no counterpart in the reference): n > 0 starts every frame
+     * with capacity for only n clip-queue entries, clip-created records and
+     * vertices, and tile references, so that the capacity-overflow path runs
+     * (flags raised on the device, buffers grown, frame re-run; edxr_stats
+     * .retries counts the re-runs).  0 (default) restores automatic sizes. */
+    EDXR_STATE_DEBUG_CAPACITY = 11
 };
 
 /* Pipeline stages timed when EDXR_STATE_PROFILE is on. */
@@ -242,6 +248,11 @@ int edxr_profile_read(edxr_ctx* ctx, double* stage_ms, uint64_t* frames);
 int edxr_profile_reset(edxr_ctx* ctx);
 /* The HIP stream (hipStream_t) the context launches on. */
 void* edxr_stream(edxr_ctx* ctx);
+/* Number of renders (edxr_render_mesh and edxr_render_batch calls that got
+ * as far as launching) made on the context; no counterpart in the reference.
+ * edxr_backward differentiates the last render, so a caller that keeps a
+ * frame for a later backward records this value and checks it is unchanged. */
+uint64_t edxr_render_generation(const edxr_ctx* ctx);
 const char* edxr_last_error(const edxr_ctx* ctx);
 
 /* ---- batched rendering (no counterpart in the reference) ---- */

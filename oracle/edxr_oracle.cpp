@@ -580,7 +580,7 @@ struct Polygon {
     }
 };
 
-/* Planted-mistake switch for the near-plane clip (pinning tests only). */
+/* Planted-mistake switch for the clipper (pinning tests only). */
 static thread_local int g_mutation = 0;
 
 /* Core/Clipper.h:37-68. */
@@ -646,7 +646,11 @@ static void ClipPolygon(Polygon*& in, Polygon*& buf, uint32_t planeCode) {
                     [](Vec4& v) { v.y = v.w; });
     if (planeCode & FAR_BIT)
         ClipByPlane(in, buf, [](const Vec4& v) { return v.z <= v.w; },
-                    [](const Vec4& a, const Vec4& b) { return (-a.w + a.z) / ((a.z - a.w) - (b.z - b.w)); },
+                    [](const Vec4& a, const Vec4& b) {
+                        /* planted when 8: wrong far-plane t (numerator sign) */
+                        return g_mutation == 8 ? (a.w - a.z) / ((a.z - a.w) - (b.z - b.w))
+                                               : (-a.w + a.z) / ((a.z - a.w) - (b.z - b.w));
+                    },
                     [](Vec4& v) { v.z = v.w; });
     if (planeCode & NEAR_BIT)
         ClipByPlane(in, buf, [](const Vec4& v) { return v.z >= 0.0f; },
@@ -813,10 +817,10 @@ void Renderer::Clipping(const oracle_mesh& m) {
                     ClipPolygon(cur, buf, (c0 ^ c1) | (c1 ^ c2) | (c2 ^ c0));
                     if (cur->vertices.empty()) cs.dropped++;
                     for (int j = 0; j < (int)cur->vertices.size(); j++) {
-                        const Vec3 w = cur->vertices[j].clipWeights;
-                        if (w.x == 1.0f) clipVertIds[j] = idx0;
-                        else if (w.y == 1.0f) clipVertIds[j] = idx1;
-                        else if (w.z == 1.0f) clipVertIds[j] = idx2;
+                        const Vec3 wt = cur->vertices[j].clipWeights;
+                        if (wt.x == 1.0f) clipVertIds[j] = idx0;
+                        else if (wt.y == 1.0f) clipVertIds[j] = idx1;
+                        else if (wt.z == 1.0f) clipVertIds[j] = idx2;
                         else {
                             clipVertIds[j] = (uint32_t)vb.size();
                             const ProjectedVertex& a = vb[idx0];
@@ -824,6 +828,8 @@ void Renderer::Clipping(const oracle_mesh& m) {
                             const ProjectedVertex& c = vb[idx2];
                             ProjectedVertex t;
                             t.projectedPos = cur->vertices[j].pos;
+                            /* planted when 7: the first two clip weights swapped */
+                            const Vec3 w = g_mutation == 7 ? Vec3{wt.y, wt.x, wt.z} : wt;
                             auto comb = [&](const Vec3& pa, const Vec3& pb, const Vec3& pc) {
                                 return Vec3{(w.x * pa.x + w.y * pb.x) + w.z * pc.x,
                                             (w.x * pa.y + w.y * pb.y) + w.z * pc.y,

@@ -146,3 +146,28 @@ def test_empty_frame_after_a_full_one(built):
     assert L.lib().edxr_copy_float_color(r.ctx, C.c_void_p(f.ctypes.data), f.nbytes) == 0
     assert (f == 0.0).all()
     r.close()
+
+
+CLIP = ["lateral", "far", "all"]
+
+
+@pytest.mark.parametrize("kind", CLIP + ["near_floor"])
+def test_clip_scene_parity(built, kind):
+    """Scenes straddling the side planes, the far plane (view depth 6 against
+    geometry from 1.5 to 9), and all six planes at once (clip polygons of 7-8
+    vertices): frame, setup records (clipped pieces included) and clip counts
+    equal the oracle's."""
+    import test_oracle_clip as TC
+    w = dict(TC.WORKLOADS)[kind]
+    r, o, ost = render_both(w)
+    info = assert_frame_parity(r, o, "clip_" + kind)
+    assert info["covered"] > 200
+    st = r.stats()
+    assert st["clipped"] == ost["clipped"] > 0
+    assert st["near_clipped"] == ost["near_clipped"]
+    assert st["setup_records"] == ost["setup_records"]
+    assert st["clip_vertices"] == ost["clip_vertices"]
+    gr, orr = r.debug_records(), o.records_array()
+    assert gr.shape == orr.shape
+    np.testing.assert_array_equal(gr[:, :13], orr[:, :13])
+    r.close()

@@ -1,0 +1,202 @@
+"""Pins the oracle's clipping path and pixel shaders with FP64 closed forms.
+
+Clipping (Core/Clipper.h:71-288) is linear in clip space, so every pixel of
+a clipped triangle must carry the depth and perspective-correct attributes of
+the plane through its clipped piece.  ``fp64_ref`` restates clipping in float64
+(textbook Sutherland-Hodgman, plane order L R B T F N, clip weights carried
+along); the tests check, on scenes that straddle the side planes, the far
+plane, the near plane and all planes at once:
+
+- the oracle's setup records of clipped triangles: the same fan pieces,
+  28.4 vertices within one sub-pixel step, z/w and 1/w;
+- per covered pixel: depth within 2e-6 and 8-bit colour within 1 LSB of the
+  FP64 values, with clip-created vertices' attributes from the FP64 weights.
+
+Planted mistakes show that the checks bite: 5 (near-plane t sign),
+7 (swapped clip weights in the new vertices' attributes) and 8 (wrong
+far-plane t).  The reference pixel shaders (Lambertian with a constant
+albedo, Blinn-Phong with shininess 200) are checked in FP64 with N.L > 0 and
+specular highlights in view.
+"""
+import numpy as np
+import pytest
+
+from edxraster_amd import configs, Mesh
+from oracle import oracle as O
+import fp64_ref as R
+
+F = np.float32
+
+CLIP_KINDS = ["lateral", "far", "all"]
+# Depth tolerance: fp32 clipping of the "all" scene's clip coordinates (up to
+# about 40) rounds z/w at clip-created vertices by up to ~6e-6.
+Z_TOL = {"all": 1e-5}
+
+
+def floor_workload(w=96, h=72):
+    """Two-triangle floor from z = -3 to 30, colours 0 near and 1 far, eye
+    0.03 above it: the near plane cuts the floor (from test_oracle's floor)."""
+    v = np.array([[-3, 0, -3.0], [3, 0, -3.0], [-3, 0, 30], [3, 0, 30]], F)
+    col = np.array([[0, 0, 0], [0.2, 0.5, 0.9], [1, 1, 1], [0.9, 0.4, 0.1]], F)
+    n = np.tile(np.array([0, 1, 0], F), (4, 1))
+    mesh = Mesh(v[:, 0], v[:, 1], v[:, 2], n[:, 0], n[:, 1], n[:, 2], col[:, 0], col[:, 1], col[:, 2],
+                np.array([0, 2, 1, 1, 2, 3], np.uint32))
+    mv, p, r = configs.camera(w, h, (0.0, 0.03, -1.0), (0.0, 0.02, 10.0))
+    return configs.Workload("near_floor_%dx%d" % (w, h), w, h, mesh, mv, p, r)
+
+
+def workloads():
+    out = [(k, configs.clip_scene(k)) for k in CLIP_KINDS]
+    out.append(("near_floor", floor_workload()))
+    return out
+
+
+WORKLOADS = workloads()
+
+
+def render(w, shader=2, shininess=32.0, **kw):
+    o = O.Oracle()
+    st = o.render(w.mesh, w.width, w.height, w.mv, w.proj, w.raster, light=w.light, shader=shader,
+                  shininess=shininess, **kw)
+    return o, st
+
+
+def mvp_of(w):
+    return O.mat_mul(w.proj, w.mv)
+
+
+def check_frame(o, w, shader=2, shininess=32.0, **shade_kw):
+    """Max |depth - FP64| and max colour error (LSB) over covered pixels, and
+    the count of clipped-winner pixels."""
+    mvp = mvp_of(w)
+    pieces = R.clipped_pieces(w.mesh, mvp, w.raster)
+    ids = o.ids()[::-1]
+    ref = R.frame_reference(ids, w.mesh, mvp, w.raster, o.records(), pieces)
+    d = o.depth()[::-1][ref["ys"], ref["xs"]].astype(np.float64)
+    c = R.colour(ref, R.eye_position(w.mv), w.light, shader, shininess, **shade_kw)
+    got = o.color()[::-1][ref["ys"], ref["xs"], :3].astype(np.float64)
+    clipped = np.isin(ref["tri"], list(pieces.keys()))
+    return np.abs(d - ref["depth"]), np.abs(got - c).max(1), clipped
+
+
+@pytest.mark.parametrize("kind,w", WORKLOADS, ids=[k for k, _ in WORKLOADS])
+def test_clip_scene_exercises_its_planes(built, kind, w):
+    mvp = mvp_of(w)
+    cs = R.clip_space(w.mesh, mvp)
+    idx = w.mesh.indices.reshape(-1, 3)
+    planes = 0
+    for t in range(idx.shape[0]):
+        codes = [R.outcode(v) for v in cs[idx[t]]]
+        if (codes[0] | codes[1] | codes[2]) and not (codes[0] & codes[1] & codes[2]):
+            planes |= (codes[0] ^ codes[1]) | (codes[1] ^ codes[2]) | (codes[2] ^ codes[0])
+    want = {"lateral": R.LEFT | R.RIGHT | R.BOTTOM | R.TOP, "far": R.FAR, "all": 63, "near_floor": R.NEAR}[kind]
+    assert planes & want == want, "planes straddled: %s" % bin(planes)
+    if kind == "lateral":
+        assert not planes & (R.NEAR | R.FAR)
+    o, st = render(w)
+    assert st["clipped"] > 0
+    assert (o.ids() != 0xFFFFFFFF).sum() > 200
+
+
+@pytest.mark.parametrize("kind,w", WORKLOADS, ids=[k for k, _ in WORKLOADS])
+def test_clip_records_match_fp64_clip(built, kind, w):
+    o, st = render(w)
+    pieces = R.clipped_pieces(w.mesh, mvp_of(w), w.raster)
+    n = R.check_records(o.records(), pieces, z_tol=Z_TOL.get(kind, 2e-6))
+    assert n > 0 and n == sum(1 for r in o.records() if r.tri_id in pieces)
+    # The "all" scene's polygons have 7 and 8 vertices (5 and 6 fan pieces).
+    if kind == "all":
+        assert max(len(p) for p in pieces.values()) == 6
+
+
+@pytest.mark.parametrize("kind,w", WORKLOADS, ids=[k for k, _ in WORKLOADS])
+def test_clipped_depth_and_colour_fp64(built, kind, w):
+    o, st = render(w)
+    dd, dc, clipped = check_frame(o, w)
+    assert clipped.sum() > 100, "too few pixels won by clipped triangles"
+    assert dd.max() <= Z_TOL.get(kind, 2e-6), "depth off by %g" % dd.max()
+    assert dc.max() <= 1, "colour off by %d LSB at %d pixels" % (dc.max(), (dc > 1).sum())
+
+
+@pytest.mark.parametrize("kind", ["lateral", "far", "near_floor"])
+def test_swapped_clip_weights_are_caught(built, kind):
+    """Mutation 7 swaps the first two clip weights of every clip-created vertex
+    (positions, normals, colours, texture coordinates)."""
+    w = dict(WORKLOADS)[kind]
+    o, _ = render(w, mutation=7)
+    dd, dc, clipped = check_frame(o, w)
+    assert dc[clipped].max() > 8 and (dc[clipped] > 1).sum() > 20
+
+
+def test_wrong_far_plane_t_is_caught(built):
+    w = dict(WORKLOADS)["far"]
+    o, _ = render(w, mutation=8)
+    pieces = R.clipped_pieces(w.mesh, mvp_of(w), w.raster)
+    with pytest.raises(AssertionError):
+        R.check_records(o.records(), pieces)
+
+
+def test_wrong_near_plane_t_is_caught(built):
+    w = dict(WORKLOADS)["near_floor"]
+    o, _ = render(w, mutation=5)
+    pieces = R.clipped_pieces(w.mesh, mvp_of(w), w.raster)
+    with pytest.raises(AssertionError):
+        R.check_records(o.records(), pieces)
+
+
+def test_fp64_clip_known_answers():
+    """The FP64 clip itself: a triangle half across the left plane keeps the
+    inside part, with the crossing points on x = -w at the exact fractions,
+    and a triangle with a vertex behind the eye (w <= 0 after clipping) is
+    dropped only when that vertex survives every straddled plane."""
+    c = np.array([[-3.0, 0.0, 0.5, 1.0], [1.0, 0.0, 0.5, 1.0], [1.0, 0.5, 0.5, 1.0]])
+    poly = R.clip_triangle(c)
+    assert len(poly) == 4
+    xs = sorted(round(v[0], 12) for v, _ in poly)
+    assert xs == [-1.0, -1.0, 1.0, 1.0]
+    for v, wt in poly:
+        np.testing.assert_allclose(wt @ c, v, atol=1e-12)   # weights reproduce the position
+        assert abs(wt.sum() - 1.0) < 1e-12
+    # crossing of edge (v0, v1) at t = 0.5 (x = -1): weights (0.5, 0.5, 0)
+    assert any(np.allclose(wt, [0.5, 0.5, 0.0]) for _, wt in poly)
+    # a fully inside triangle is returned unchanged
+    inside = np.array([[0.0, 0.0, 0.5, 1.0], [0.5, 0.0, 0.5, 1.0], [0.0, 0.5, 0.5, 1.0]])
+    out = R.clip_triangle(inside)
+    assert len(out) == 3 and all(np.array_equal(v, inside[k]) for k, (v, _) in enumerate(out))
+
+
+# ---------------------------------------------------------------- shaders
+def shader_scene(w=160, h=120):
+    mesh = Mesh.load_sphere(48, 24, 1.0, (0.0, 0.0, 0.0), color_seed=3)
+    mv, p, r = configs.camera(w, h, (0.4, 0.8, -2.6), (0.0, 0.0, 0.0))
+    return configs.Workload("sphere_48x24_%dx%d" % (w, h), w, h, mesh, mv, p, r)
+
+
+@pytest.mark.parametrize("shader,shininess", [(0, 0.0), (1, 200.0)], ids=["lambert_albedo", "blinn_phong_s200"])
+def test_reference_shaders_fp64(built, shader, shininess):
+    """LambertianAlbedoPixelShader with a constant albedo (0.9) and the
+    reference BlinnPhongPixelShader (Core/Shader.h:209-282) against FP64 on a
+    lit sphere: N.L > 0 on most pixels, and for Blinn-Phong a specular
+    highlight (N.H close to 1) in view."""
+    w = shader_scene()
+    o, _ = render(w, shader=shader, shininess=shininess)
+    mvp = mvp_of(w)
+    ref = R.frame_reference(o.ids()[::-1], w.mesh, mvp, w.raster)
+    N = ref["N"] / np.linalg.norm(ref["N"], axis=1, keepdims=True)
+    L = np.array([1.0, 1.0, -1.0]) / np.sqrt(3.0)
+    assert (N @ L > 0.05).sum() > 0.4 * len(N)
+    c = R.colour(ref, R.eye_position(w.mv), w.light, shader, shininess)
+    got = o.color()[::-1][ref["ys"], ref["xs"], :3].astype(np.float64)
+    assert np.abs(got - c).max() <= 1
+    if shader == 1:
+        V = R.eye_position(w.mv)[None] - ref["P"]
+        V /= np.linalg.norm(V, axis=1, keepdims=True)
+        Hh = (L[None] + V) / np.linalg.norm(L[None] + V, axis=1, keepdims=True)
+        spec = (N * Hh).sum(1) ** 200 * 3
+        assert (spec > 0.05).sum() > 20 and (spec < 1.0).sum() > 20   # a highlight with a falloff
+    else:
+        assert len(np.unique(got[:, 0])) > 50   # a shaded gradient, not a flat colour
+    # A wrong shininess (32 for 200) or albedo (1 for 0.9) is seen.
+    other = R.colour(ref, R.eye_position(w.mv), w.light, shader, 32.0 if shader == 1 else 0.0,
+                     albedo=(1.0, 1.0, 1.0))
+    assert np.abs(got - other).max() > 5
