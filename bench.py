@@ -54,10 +54,13 @@ def cpu_model():
 def cpu_baseline(w, budget_s):
     from oracle.oracle import Oracle
     try:
-        cores = len(os.sched_getaffinity(0))
+        allowed = len(os.sched_getaffinity(0))
     except Exception:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        allowed = os.cpu_count() or 1
+    nproc = os.cpu_count() or 1
+    # At most 16 threads: a one-GPU share of the host's cores (the count and
+    # the host's total are both reported).
+    cores = max(1, min(allowed, 16))
     o = Oracle()
 
     def median_frame(threads, budget, max_frames):
@@ -75,7 +78,8 @@ def cpu_baseline(w, budget_s):
     med12, n12 = median_frame(12, budget_s / 2, 10)
     return {"value": w.mesh.num_triangles / med / 1e6, "unit": "Mtris/s", "cores": cores, "kind": "port",
             "sample": "%d full frames of %s (median %.1f ms/frame), C++/SSE restatement of the reference "
-                      "CPU pipeline, %d threads on %s" % (n, w.name, med * 1e3, cores, cpu_model()),
+                      "CPU pipeline built -O3 -msse4.1, %d threads on %s" % (n, w.name, med * 1e3, cores, cpu_model()),
+            "nproc": nproc, "cpus_allowed": allowed, "cpu_model": cpu_model(),
             "value_12_threads": w.mesh.num_triangles / med12 / 1e6,
             "sample_12_threads": "%d frames, median %.1f ms/frame" % (n12, med12 * 1e3)}
 
@@ -282,12 +286,44 @@ def diff_leg(steps, warmup):
         t2 = time.perf_counter()
         fwd += t1 - t0
         bwd += t2 - t1
-    r.close()
     fwd, bwd = fwd / steps * 1e3, bwd / steps * 1e3
+    check = diff_checks(r, views, idx, planes, light, material, target)
+    r.close()
     return {"metric": "fwd+bwd ms/view", "views": len(views), "res": views[0].width,
             "triangles": mesh.num_triangles, "fwd_ms": fwd, "bwd_ms": bwd,
-            "ms_per_view": (fwd + bwd) / len(views), "bwd_over_fwd": bwd / fwd,
+            "ms_per_view": (fwd + bwd) / len(views), "bwd_over_fwd": bwd / fwd, "checks": check,
             "note": "fwd includes the L2 loss; bwd the loss gradient + edxr_backward (adjoint kernels)"}
+
+
+def diff_checks(r, views, idx, planes, light, material, target):
+    """Correctness of the timed differentiable pass, from the library alone:
+    the float image quantises to the byte image of every view, and the
+    adjoint's d loss / d light_x agrees with a central difference of the
+    float64-summed loss (h = 1e-2; visibility does not depend on the light)."""
+    import torch
+    from edxraster_amd import diff
+    for p in planes:
+        p.grad = None
+    light.grad = material.grad = None
+    img = diff.render_views(r, views, idx, *planes, light, material)
+    ((img.double() - target.double()) ** 2).sum().backward()
+    analytic = float(light.grad[0])
+    quantised = all(np.array_equal((img[k].detach().cpu().numpy() * np.float32(255.0) + np.float32(0.5))
+                                   .astype(np.int32), r.batch_read(k)[0][..., :3].astype(np.int32))
+                    for k in range(len(views)))
+    h = 1e-2
+    with torch.no_grad():
+        def loss_at(dx):
+            l2 = light.detach().clone()
+            l2[0] += dx
+            im = diff.render_views(r, views, idx, *[p.detach() for p in planes], l2, material.detach())
+            return float(((im.double() - target.double()) ** 2).sum())
+        fd = (loss_at(h) - loss_at(-h)) / (2 * h)
+    rel = abs(analytic - fd) / max(abs(fd), 1e-30)
+    assert quantised, "float image does not quantise to the byte image"
+    assert rel <= 1e-2, "adjoint d loss / d light_x %g vs central difference %g" % (analytic, fd)
+    return {"float_image_quantises_to_bytes": quantised, "param": "light[0]", "adjoint": analytic,
+            "central_difference": fd, "rel_err": rel}
 
 
 def main():

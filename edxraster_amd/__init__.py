@@ -8,7 +8,7 @@ depth test) runs as HIP kernels in libedxraster.so behind a C ABI
 Renderer / Mesh API over that ABI.
 """
 from . import _lib
-from .renderer import Mesh, Renderer, look_at, perspective, raster_matrix, default_material, load_bmp
+from .renderer import Mesh, Renderer, look_at, perspective, raster_matrix, default_material, load_bmp, load_image
 from ._lib import (SHADER_LAMBERT_ALBEDO, SHADER_BLINN_PHONG_REF, SHADER_VC_BLINN_PHONG, SHADER_TEXTURED_LAMBERT,
                    FILTER_NEAREST, FILTER_LINEAR, FILTER_TRILINEAR, EdxrError)
 

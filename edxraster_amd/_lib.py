@@ -27,6 +27,9 @@ SHADER_TEXTURED_LAMBERT = 3
 FILTER_NEAREST = 0
 FILTER_LINEAR = 1
 FILTER_TRILINEAR = 2
+FILTER_ANISO_4X = 3
+FILTER_ANISO_8X = 4
+FILTER_ANISO_16X = 5
 
 STATE_HIERARCHICAL = 1
 STATE_BIN_CHUNKS = 2
@@ -134,12 +137,20 @@ SIGNATURES = [
     ("edxr_scene_heightfield", C.c_int, [C.POINTER(HostMesh), C.c_uint32, C.c_float, C.c_uint64, C.c_uint64]),
     ("edxr_scene_sphere_grid", C.c_int, [C.POINTER(HostMesh), C.c_uint32, C.c_uint32, C.c_uint64]),
     ("edxr_scene_merge", C.c_int, [C.POINTER(HostMesh), C.POINTER(HostMesh)]),
+    ("edxr_load_sphere", C.c_int, [F, F, F, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(HostMesh)]),
+    ("edxr_load_plane", C.c_int, [F, F, F, C.c_float, C.POINTER(HostMesh)]),
     ("edxr_scene_free", None, [C.POINTER(HostMesh)]),
     ("edxr_load_obj", C.c_int, [C.c_char_p, F, F, F, C.POINTER(HostMesh)]),
     ("edxr_write_bmp", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     ("edxr_obj_materials", C.c_int64, [C.c_char_p, C.POINTER(ObjMaterial), C.c_uint32]),
     ("edxr_load_bmp", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                 C.POINTER(C.POINTER(C.c_uint8))]),
+    ("edxr_load_tga", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.POINTER(C.c_uint8))]),
+    ("edxr_load_ppm", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.POINTER(C.c_uint8))]),
+    ("edxr_load_image", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.POINTER(C.c_uint8))]),
     ("edxr_free", None, [C.c_void_p]),
     ("edxr_set_frame_directory", C.c_int, [C.c_void_p, C.c_char_p]),
     ("edxr_set_textures", C.c_int, [C.c_void_p, C.POINTER(Texture), C.c_uint32]),

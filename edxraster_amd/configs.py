@@ -58,7 +58,7 @@ def camera(width, height, eye, at, up=(0.0, 1.0, 0.0)):
 
 def small_sphere(width=800, height=600, slices=64, stacks=32):
     """UV sphere 64x32 (V=2,145, T=4,096), r=1 at the origin, eye (0,0,-3)."""
-    mesh = Mesh.load_sphere(slices, stacks, 1.0, (0, 0, 0), color_seed=1)
+    mesh = Mesh.sphere(slices, stacks, 1.0, (0, 0, 0), color_seed=1)
     mv, p, r = camera(width, height, (0.0, 0.0, -3.0), (0.0, 0.0, 0.0))
     return Workload("sphere_64x32_800x600", width, height, mesh, mv, p, r)
 
@@ -136,7 +136,7 @@ def textured_scene(width=640, height=480, n=96, tex_filter=L.FILTER_TRILINEAR, s
     hf.tv = ((hf.pz + 1.0) * 1.5).astype(np.float32)
     T = hf.num_triangles
     hf.tex_ids = ((np.arange(T) // 2) % 2).astype(np.uint32)
-    sp = Mesh.load_sphere(sphere[0], sphere[1], 0.35, (0.1, 0.45, 0.2), color_seed=8)
+    sp = Mesh.sphere(sphere[0], sphere[1], 0.35, (0.1, 0.45, 0.2), color_seed=8)
     d = np.stack([sp.px - 0.1, sp.py - 0.45, sp.pz - 0.2], 1) / 0.35
     sp.tu = (np.arctan2(d[:, 2], d[:, 0]) / (2 * np.pi) + 0.5).astype(np.float32)
     sp.tv = (np.arccos(np.clip(d[:, 1], -1, 1)) / np.pi).astype(np.float32)
@@ -231,7 +231,7 @@ def batch_jobs(n_jobs=256, seed_base=4000, tmin=1e3, tmax=5e5, resolutions=(256,
         light = tuple(float(x) for x in _unit(rng))
         if sphere:
             s = max(2, int(round(np.sqrt(target / 4.0))))
-            mesh = Mesh.load_sphere(2 * s, s, 1.0, (0.0, 0.0, 0.0), color_seed=seed_base + j)
+            mesh = Mesh.sphere(2 * s, s, 1.0, (0.0, 0.0, 0.0), color_seed=seed_base + j)
         else:
             n = max(3, int(round(1.0 + np.sqrt(target / 2.0))))
             mesh = Mesh.load_heightfield(n, 0.2, height_seed=seed_base + j, color_seed=seed_base + 1000 + j)
@@ -249,7 +249,7 @@ def diff_scene(n_views=64, res=512, hf_n=71, sphere_segments=(80, 64), shininess
     merged into one mesh; view k looks at (0, 0.15, 0) from distance 2.4 at
     elevation 30 degrees and azimuth 5.625 k degrees, res x res pixels."""
     hf = Mesh.load_heightfield(hf_n, 0.2, height_seed=5, color_seed=55)
-    sph = Mesh.load_sphere(sphere_segments[0], sphere_segments[1], 0.35, (0.0, 0.55, 0.0), color_seed=56)
+    sph = Mesh.sphere(sphere_segments[0], sphere_segments[1], 0.35, (0.0, 0.55, 0.0), color_seed=56)
     mesh = Mesh.merge(hf, sph)
     views = []
     el = np.radians(30.0)

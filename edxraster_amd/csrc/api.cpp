@@ -186,7 +186,8 @@ struct edxr_ctx {
     // pipeline buffers
     DevBuf<RasterRec> rrec;
     DevBuf<ShadeRec> srec;
-    DevBuf<uint32_t> desc, clipQueue, binCounts, tileTotals, tileStart, refs, counters, color, ids, work, tileItem;
+    DevBuf<uint32_t> desc, clipQueue, binCounts, tileTotals, tileStart, refs, counters, color, ids, work, tileItem,
+        tileBig;
     DevBuf<float> depth;
     DevBuf<uint64_t> partials;
     DevBuf<uint16_t> slotRefs;
@@ -207,7 +208,7 @@ struct edxr_ctx {
     bool lastPlanValid = false, lastFloat = false;
     DevBuf<GradPtrs> gradsDev;
     HostBuf<GradPtrs> gradsHost;
-    DevBuf<double> adjAcc;
+    DevBuf<double> adjAcc, adjVertAcc;
     std::vector<uint64_t> batchPix;
     std::vector<uint32_t> batchW, batchH;
     bool batchRendered = false;
@@ -267,12 +268,15 @@ int set_size(edxr_ctx* c, uint32_t w, uint32_t h) {
 
 // Binning chunks of a job: about trisPerChunk (1024) triangles each so the
 // count/scatter grids fill the GPU, capped so the job's chunk x tile count
-// matrix stays within 64 MB.
-uint32_t choose_chunks(const edxr_ctx* c, uint32_t T, uint32_t nTiles, bool single) {
+// matrix (allocated in full, touched only inside each chunk's tile box) stays
+// within matrixBytes.
+uint32_t choose_chunks(const edxr_ctx* c, uint32_t T, uint32_t nTiles, bool single,
+                       uint64_t matrixBytes = 256ull << 20) {
     if (T == 0) return 0;
-    if (single && c->binChunksSetting) return std::max(1u, std::min(std::min(c->binChunksSetting, kMaxChunks), T));
+    const uint32_t byMatrix = (uint32_t)std::max<uint64_t>(1, matrixBytes / (4ull * nTiles));
+    if (single && c->binChunksSetting)
+        return std::max(1u, std::min(std::min(std::min(c->binChunksSetting, kMaxChunks), T), byMatrix));
     const uint32_t tpc = c->trisPerChunk ? c->trisPerChunk : 1024;
-    const uint32_t byMatrix = (uint32_t)std::max<uint64_t>(1, (64ull << 20) / (4ull * nTiles));
     const uint32_t C = (T + tpc - 1) / tpc;
     return std::max(1u, std::min(std::min(C, byMatrix), kMaxChunks));
 }
@@ -432,7 +436,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
     if ((e = c->jobsDev.ensure(Q.num_jobs)) != hipSuccess ||
         (e = c->tileTotals.ensure(Q.total_tiles)) != hipSuccess ||
         (e = c->tileStart.ensure((size_t)Q.total_tiles + 1)) != hipSuccess ||
-        (e = c->tileItem.ensure(Q.total_tiles)) != hipSuccess)
+        (e = c->tileItem.ensure(Q.total_tiles)) != hipSuccess || (e = c->tileBig.ensure(Q.total_tiles)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(job tables)");
     if (!single || graph) {
         if ((e = c->jobsHost.ensure(Q.num_jobs)) != hipSuccess) return c->hip_fail(e, "hipHostMalloc(jobs)");
@@ -512,6 +516,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.work = c->work.p;
         B.work_cap = (uint32_t)c->work.cap;
         B.tile_item = c->tileItem.p;
+        B.tile_big = c->tileBig.p;
         B.partials = c->partials.p;
         B.raster_blocks = rasterBlocks;
         // The persistent raster grid is the resident capacity: workgroups
@@ -599,6 +604,30 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         c->stats.retries = attempt + 1;
     }
     return c->fail(EDXR_E_LIMIT, "capacity retries exhausted");
+}
+
+// The job of a single-frame render: the mesh with the context's render target,
+// transform, light, shader, states and texture slots.
+JobInput context_job(const edxr_ctx* c, const MeshPtrs& M, uint32_t V, uint32_t T) {
+    JobInput in;
+    in.mesh = M;
+    in.V = V;
+    in.T = T;
+    in.W = c->W;
+    in.H = c->H;
+    memcpy(in.mvp, c->mvp, 64);
+    memcpy(in.raster, c->raster, 64);
+    memcpy(in.eye, c->eye, 12);
+    memcpy(in.light, c->lightDir, 12);
+    in.mat = c->mat;
+    in.shader = c->shader;
+    in.hierarchical = c->hierarchical;
+    in.sampleLevel = (uint32_t)c->msaaLog2;
+    in.tex = c->texDesc.p;
+    in.texels = c->texels.p;
+    in.numTex = c->numTex;
+    in.texFilter = c->texFilter;
+    return in;
 }
 
 // Uploads the host-memory meshes of a batch into one device buffer and
@@ -718,7 +747,7 @@ void edxr_destroy(edxr_ctx* c) {
     c->rrec.release();
     c->srec.release();
     for (DevBuf<uint32_t>* b : {&c->desc, &c->clipQueue, &c->binCounts, &c->tileTotals, &c->tileStart, &c->refs,
-                                &c->counters, &c->color, &c->ids, &c->work, &c->tileItem, &c->groupSums,
+                                &c->counters, &c->color, &c->ids, &c->work, &c->tileItem, &c->tileBig, &c->groupSums,
                                 &c->jobTables, &c->bcolor, &c->bids})
         b->release();
     c->depth.release();
@@ -728,6 +757,7 @@ void edxr_destroy(edxr_ctx* c) {
     c->gradsDev.release();
     c->gradsHost.release();
     c->adjAcc.release();
+    c->adjVertAcc.release();
     c->partials.release();
     c->slotRefs.release();
     c->chunkBox.release();
@@ -801,7 +831,7 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
         case EDXR_STATE_FLOAT_COLOR: c->floatColor = value ? 1 : 0; return EDXR_OK;
         case EDXR_STATE_WRITE_FRAMES: c->writeFrames = value ? 1 : 0; return EDXR_OK;
         case EDXR_STATE_TEXTURE_FILTER:
-            if (value < EDXR_FILTER_NEAREST || value > EDXR_FILTER_TRILINEAR)
+            if (value < EDXR_FILTER_NEAREST || value > EDXR_FILTER_ANISO_16X)
                 return c->fail(EDXR_E_INVALID, "unknown texture filter");
             c->texFilter = value;
             return EDXR_OK;
@@ -866,24 +896,7 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
         M = mesh_ptrs(dst, c->meshIdx.p, dt);
     }
 
-    JobInput in;
-    in.mesh = M;
-    in.V = V;
-    in.T = T;
-    in.W = c->W;
-    in.H = c->H;
-    memcpy(in.mvp, c->mvp, 64);
-    memcpy(in.raster, c->raster, 64);
-    memcpy(in.eye, c->eye, 12);
-    memcpy(in.light, c->lightDir, 12);
-    in.mat = c->mat;
-    in.shader = c->shader;
-    in.hierarchical = c->hierarchical;
-    in.sampleLevel = (uint32_t)c->msaaLog2;
-    in.tex = c->texDesc.p;
-    in.texels = c->texels.p;
-    in.numTex = c->numTex;
-    in.texFilter = c->texFilter;
+    const JobInput in = context_job(c, M, V, T);
     Plan pl;
     if ((r = build_plan(c, std::vector<JobInput>{in}, pl)) != EDXR_OK) return r;
     c->renderGeneration++;
@@ -1129,15 +1142,45 @@ int edxr_backward(edxr_ctx* c, const float* d_rgb, const edxr_grads* g, uint32_t
     if ((e = c->gradsDev.ensure(n)) != hipSuccess || (e = c->gradsHost.ensure(n)) != hipSuccess ||
         (e = c->adjAcc.ensure(8ull * n)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(adjoint)");
-    for (uint32_t k = 0; k < n; k++)
+    // Jobs whose nine vertex outputs are the same arrays share one double
+    // accumulator (as many vertices as the largest of their meshes).
+    std::vector<uint32_t> groupOf(n), groupV, groupFirst;
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t gi = 0;
+        for (; gi < groupFirst.size(); gi++) {
+            const edxr_grads& f = g[groupFirst[gi]];
+            if (f.d_px == g[k].d_px && f.d_py == g[k].d_py && f.d_pz == g[k].d_pz && f.d_nx == g[k].d_nx &&
+                f.d_ny == g[k].d_ny && f.d_nz == g[k].d_nz && f.d_cr == g[k].d_cr && f.d_cg == g[k].d_cg &&
+                f.d_cb == g[k].d_cb)
+                break;
+        }
+        if (gi == groupFirst.size()) {
+            groupFirst.push_back(k);
+            groupV.push_back(0);
+        }
+        groupOf[k] = gi;
+        groupV[gi] = std::max(groupV[gi], pl.jobs[k].num_vertices);
+    }
+    std::vector<uint64_t> groupBase(groupFirst.size());
+    uint64_t vaccSize = 0;
+    for (size_t gi = 0; gi < groupFirst.size(); gi++) {
+        groupBase[gi] = vaccSize;
+        vaccSize += 9ull * groupV[gi];
+    }
+    if ((e = c->adjVertAcc.ensure(std::max<uint64_t>(vaccSize, 1))) != hipSuccess)
+        return c->hip_fail(e, "hipMalloc(adjoint accumulators)");
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t gi = groupOf[k];
         c->gradsHost.p[k] = GradPtrs{g[k].d_px, g[k].d_py, g[k].d_pz, g[k].d_nx, g[k].d_ny, g[k].d_nz,
-                                     g[k].d_cr, g[k].d_cg, g[k].d_cb, g[k].d_light, g[k].d_material};
+                                     g[k].d_cr, g[k].d_cg, g[k].d_cb, g[k].d_light, g[k].d_material,
+                                     groupBase[gi], groupV[gi], groupFirst[gi] == k ? 1u : 0u};
+    }
     if ((e = hipMemcpyAsync(c->gradsDev.p, c->gradsHost.p, sizeof(GradPtrs) * n, hipMemcpyHostToDevice,
                             c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(grads)");
     const uint32_t* ids = c->rendered ? c->ids.p : c->bids.p;
-    if ((e = launch_adjoint(pl.Q, c->jobsDev.p, c->gradsDev.p, c->verts.p, ids, d_rgb, c->adjAcc.p, c->stream)) !=
-        hipSuccess)
+    if ((e = launch_adjoint(pl.Q, c->jobsDev.p, c->gradsDev.p, c->verts.p, ids, d_rgb, c->adjAcc.p,
+                            c->adjVertAcc.p, vaccSize, c->stream)) != hipSuccess)
         return c->hip_fail(e, "adjoint launch");
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
     return EDXR_OK;

@@ -11,6 +11,8 @@ constexpr int kTileLog2 = 5;                 // Tile::SIZE_LOG_2 (EDXRaster/Core
 constexpr int kTileSize = 1 << kTileLog2;    // 32 x 32 pixels
 constexpr uint32_t kMaxTiles = 16384;        // LDS-resident bin cursors (per job)
 constexpr uint32_t kTileBits = 18;           // global tile index bits in a work item
+constexpr uint32_t kSegIndexBits = 31 - kTileBits;  // segment index bits in a work item, above the tile
+constexpr uint32_t kSingleSegment = 1u << 31;       // work item flag: the tile has one segment
 constexpr uint32_t kMaxBatchTiles = 1u << kTileBits;
 constexpr uint32_t kMaxDim = 8192;
 constexpr uint32_t kRecordBits = 28;         // record index bits in a tile reference
@@ -21,6 +23,15 @@ constexpr uint32_t kPassSamples = 4;         // samples whose keys one raster pa
 constexpr uint32_t kSegLog2Min = 8;          // raster work items hold 256 .. 2048 tile references,
 constexpr uint32_t kSegLog2Max = 11;         // chosen per frame from the total (k_bin_tilescan)
 constexpr uint32_t kSegRefs = 1u << kSegLog2Max;
+// Tiles whose references are mostly tile-level triangles (each takes a whole
+// wave in the raster) are cut into segments of 2^kSegLog2Heavy references when
+// at least kHeavyBigMin of their references, and a quarter of them, are such.
+constexpr uint32_t kSegLog2Heavy = 5;
+constexpr uint32_t kHeavyBigMin = 32;
+// tile_item: first key image of the tile (low kImageBits) | the tile's
+// segment log2 << kImageBits.
+constexpr uint32_t kImageBits = 26;
+constexpr uint32_t kImageMask = (1u << kImageBits) - 1;
 constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
 constexpr uint32_t kJobAlign = 1024;         // job slot alignment: vertex/setup blocks never span jobs
@@ -30,9 +41,20 @@ constexpr uint32_t kMaxChunks = 4096;
 // t; tile bbox minTX | minTY << 14 | dx << 28 | dy << 29, dx, dy in {0, 1});
 // otherwise (first record << 3) | record count (0..7).
 constexpr uint32_t kDescCompact = 1u << 31;
+// Compact descriptor flag: the triangle's pixel bounding box has kHeavyPixels
+// or more pixels (it takes a whole wave in the raster; counted in tile_big).
+constexpr uint32_t kDescHeavy = 1u << 30;
+constexpr uint32_t kHeavyPixels = 256;
 
 enum ShaderKind { kLambertAlbedo = 0, kBlinnPhongRef = 1, kVcBlinnPhong = 2, kTexturedLambert = 3 };
-enum TextureFilter { kFilterNearest = 0, kFilterLinear = 1, kFilterTrilinear = 2 };
+enum TextureFilter {
+    kFilterNearest = 0,
+    kFilterLinear = 1,
+    kFilterTrilinear = 2,
+    kFilterAniso4 = 3,  // N:1 anisotropic, N = 4 << (filter - kFilterAniso4)
+    kFilterAniso8 = 4,
+    kFilterAniso16 = 5
+};
 
 struct MeshPtrs {
     const float *px, *py, *pz, *nx, *ny, *nz, *cr, *cg, *cb;
@@ -221,9 +243,10 @@ struct DeviceBuffers {
     uint32_t* tile_start;  // num_tiles + 1
     uint32_t* refs;
     uint32_t ref_cap;
-    uint32_t* work;        // raster work items: global tile | segment << kTileBits
+    uint32_t* work;        // raster work items: global tile | segment << kTileBits | kSingleSegment
     uint32_t work_cap;
-    uint32_t* tile_item;   // per global tile: index of its first key image (one per segment)
+    uint32_t* tile_item;   // per global tile: first key image (one per segment) | segment log2 << kImageBits
+    uint32_t* tile_big;    // per global tile: heavy references (big, or compact with kDescHeavy)
     uint64_t* partials;    // work_cap * samples * 1024 (depth, order) keys, one tile image per item and sample
     uint32_t raster_blocks;  // work items the segment size is chosen for
     uint32_t raster_grid;    // launched raster workgroups: the resident capacity
@@ -238,10 +261,17 @@ struct DeviceBuffers {
 // be null).  Positions, normals and colours are per input vertex of the job's
 // mesh; light is w.r.t. the light direction as given, material is (ambient,
 // kd, ks, shininess).
+// The per-vertex sums are accumulated in double first: jobs that share their
+// vertex outputs share one accumulator of 9 x acc_v doubles (value-major) at
+// acc_base, which the first of them (group_first) adds to the outputs at the
+// end.
 struct GradPtrs {
     float *d_px, *d_py, *d_pz, *d_nx, *d_ny, *d_nz, *d_cr, *d_cg, *d_cb;
     float* d_light;
     float* d_material;
+    uint64_t acc_base;
+    uint32_t acc_v;
+    uint32_t group_first;
 };
 
 // Stage boundaries recorded when profiling: events[0] before the counter reset,
@@ -255,10 +285,11 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
                         hipStream_t s, hipEvent_t* events);
 // Adjoint of the shading of a rendered batch with visibility held fixed (the
 // triangle-ID buffer): d_rgb holds dLoss/d(clamped float RGB) per pixel in the
-// layout of color_f32; acc is 8 doubles of scratch per job.
+// layout of color_f32; acc is 8 doubles of scratch per job, vacc the vertex
+// accumulators (vacc_size doubles, the GradPtrs' acc_base ranges).
 hipError_t launch_adjoint(const BatchParams& Q, const FrameParams* jobs, const GradPtrs* grads,
                           const ShadeVert* verts, const uint32_t* ids, const float* d_rgb, double* acc,
-                          hipStream_t s);
+                          double* vacc, uint64_t vacc_size, hipStream_t s);
 // Kernel attributes (dynamic LDS limits) and occupancy; once per process before launches.
 hipError_t init_kernels();
 // Workgroups of the persistent raster kernel that one CU holds at once.

@@ -191,7 +191,12 @@ __device__ __forceinline__ uint32_t compact_desc(const FrameParams& P, const Ras
     const int maxTY = min((int)P.tiles_y - 1, max(r.y0, max(r.y1, r.y2)) >> Shift);
     const int dx = maxTX - minTX, dy = maxTY - minTY;
     if (dx < 0 || dy < 0 || dx > 1 || dy > 1) return (t << 3) | 1u;
-    return kDescCompact | (uint32_t)minTX | ((uint32_t)minTY << 14) | ((uint32_t)dx << 28) | ((uint32_t)dy << 29);
+    // a pixel bounding box of kHeavyPixels or more: a whole wave's work in the raster
+    const int bw = (max(r.x0, max(r.x1, r.x2)) - min(r.x0, min(r.x1, r.x2))) >> 4;
+    const int bh = (max(r.y0, max(r.y1, r.y2)) - min(r.y0, min(r.y1, r.y2))) >> 4;
+    const uint32_t heavy = bw * bh >= (int)kHeavyPixels ? kDescHeavy : 0u;
+    return kDescCompact | heavy | (uint32_t)minTX | ((uint32_t)minTY << 14) | ((uint32_t)dx << 28) |
+           ((uint32_t)dy << 29);
 }
 }  // namespace
 
@@ -205,6 +210,9 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
                                                   DeviceBuffers B) {
     const FrameParams& P = jobs[job_of(Q, Q.sblock_job, blockIdx.x * kSetupSlots)];
     const MeshPtrs& M = P.mesh;
+    // k_bin_count adds the big references of every tile to zeroed counts.
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < Q.total_tiles; t += gridDim.x * kBlock)
+        B.tile_big[t] = 0u;
     uint32_t s[kSetupSlots], i0[kSetupSlots], i1[kSetupSlots], i2[kSetupSlots];
     bool valid[kSetupSlots];
 #pragma unroll
@@ -743,7 +751,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
             if (d[u] & kDescCompact) {
                 // at most 2x2 tiles, no corner tests
                 R = compact_refs(d[u]);
-                for (uint32_t k = 0; k < R; k++) atomicAdd(&hist[compact_tile(d[u], k, tilesX)], 1u);
+                for (uint32_t k = 0; k < R; k++) {
+                    const uint32_t tile = compact_tile(d[u], k, tilesX);
+                    atomicAdd(&hist[tile], 1u);
+                    if (d[u] & kDescHeavy) atomicAdd(&B.tile_big[P.tile_base + tile], 1u);
+                }
                 const uint32_t tx = d[u] & 0x3FFFu, ty = (d[u] >> 14) & 0x3FFFu;
                 bx0 = min(bx0, tx);
                 by0 = min(by0, ty);
@@ -755,6 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
                 uint32_t tile, payload;
                 while (e.next(P, B.rrec, tile, payload)) {
                     atomicAdd(&hist[tile], 1u);
+                    if (payload >> 31) atomicAdd(&B.tile_big[P.tile_base + tile], 1u);
                     R++;
                     const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
                     bx0 = min(bx0, tx);
@@ -781,12 +794,21 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
         atomicMax(&sBox[3], by1);
     }
     __syncthreads();
+    const uint32_t x0 = sBox[0], y0 = sBox[1], x1 = sBox[2], y1 = sBox[3];
     if (threadIdx.x == 0) {
-        B.chunk_box[2 * blockIdx.x] = sBox[0] | (sBox[1] << 16);
-        B.chunk_box[2 * blockIdx.x + 1] = sBox[2] | (sBox[3] << 16);
+        B.chunk_box[2 * blockIdx.x] = x0 | (y0 << 16);
+        B.chunk_box[2 * blockIdx.x + 1] = x1 | (y1 << 16);
     }
-    uint32_t* row = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
-    for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) row[i] = hist[i];
+    // Only the box part of the chunk's row is written: the column scans and
+    // the scatter read an entry only when its tile lies in the chunk's box.
+    if (x0 <= x1 && y0 <= y1) {
+        uint32_t* row = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
+        const uint32_t bw = x1 - x0 + 1, nb = bw * (y1 - y0 + 1);
+        for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
+            const uint32_t t = (y0 + i / bw) * tilesX + x0 + i % bw;
+            row[t] = hist[t];
+        }
+    }
 }
 
 // Each job's chunk x tile count matrix is scanned down each tile's column in
@@ -797,12 +819,28 @@ namespace {
 constexpr uint32_t kWaveChunks = kChunkGroup / (kBlock / kWave);  // chunks of a group per wave
 
 // The wave's counts of one tile column (all loads issued before any use).
-__device__ __forceinline__ void colgroup_load(const FrameParams& P, const DeviceBuffers& B, uint32_t tile,
-                                              uint32_t c0, uint32_t c1, uint32_t (&v)[kWaveChunks]) {
+// Entries whose tile lies outside the chunk's tile box are zero and are not
+// stored: they are neither read nor written.  Bit k of the returned mask is
+// set when chunk c0 + k's entry is stored.
+__device__ __forceinline__ uint32_t colgroup_load(const FrameParams& P, const DeviceBuffers& B, uint32_t tile,
+                                                  uint32_t c0, uint32_t c1, uint32_t (&v)[kWaveChunks]) {
     const bool in = tile < P.num_tiles;
+    const uint32_t ty = in ? tile / P.tiles_x : 0u, tx = in ? tile - ty * P.tiles_x : 0u;
+    const uint32_t* box = B.chunk_box + 2 * (size_t)(P.chunk_base + __builtin_amdgcn_readfirstlane(c0));
+    uint32_t mask = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kWaveChunks; k++) {
+        bool stored = false;
+        if (c0 + k < c1) {
+            const uint32_t b0 = box[2 * k], b1 = box[2 * k + 1];
+            stored = in && tx >= (b0 & 0xFFFFu) && tx <= (b1 & 0xFFFFu) && ty >= (b0 >> 16) && ty <= (b1 >> 16);
+        }
+        mask |= stored ? 1u << k : 0u;
+    }
 #pragma unroll
     for (uint32_t k = 0; k < kWaveChunks; k++)
-        v[k] = in && c0 + k < c1 ? B.bin_counts[P.count_base + (size_t)(c0 + k) * P.num_tiles + tile] : 0u;
+        v[k] = (mask >> k) & 1u ? B.bin_counts[P.count_base + (size_t)(c0 + k) * P.num_tiles + tile] : 0u;
+    return mask;
 }
 
 // Per-wave partial sums of one (64-tile, chunk-group) block, combined in LDS;
@@ -882,13 +920,13 @@ __global__ __launch_bounds__(kBlock) void k_bin_colscan(BatchParams Q, const Fra
     const uint32_t tile = blockIdx.x * kWave + __lane_id();
     uint32_t c0, c1, waveBase, groupTotal, v[kWaveChunks];
     colgroup_range(P, c0, c1);
-    colgroup_load(P, B, tile, c0, c1, v);
+    const uint32_t stored = colgroup_load(P, B, tile, c0, c1, v);
     colgroup_sums(v, part, waveBase, groupTotal);
     if (tile >= P.num_tiles) return;
     uint32_t run = B.group_sums[P.group_base + (size_t)blockIdx.y * P.num_tiles + tile] + waveBase;
 #pragma unroll
     for (uint32_t k = 0; k < kWaveChunks; k++) {
-        if (c0 + k < c1) B.bin_counts[P.count_base + (size_t)(c0 + k) * P.num_tiles + tile] = run;
+        if ((stored >> k) & 1u) B.bin_counts[P.count_base + (size_t)(c0 + k) * P.num_tiles + tile] = run;
         run += v[k];
     }
 }
@@ -946,11 +984,11 @@ __device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) 
 // (tile_item) plus its segment index, whatever its place in the list.
 __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuffers B) {
     __shared__ uint32_t sScan6[6][16];
-    __shared__ uint32_t sMax, sTotal;
+    __shared__ uint32_t sMax, sTotal, sSegs, sExtra;
     const uint32_t nt = Q.total_tiles;
     const uint32_t per = (nt + 1023) / 1024;
     const uint32_t t0 = min(nt, threadIdx.x * per), t1 = min(nt, t0 + per);
-    if (threadIdx.x == 0) sMax = sTotal = 0;
+    if (threadIdx.x == 0) sMax = sTotal = sSegs = sExtra = 0;
     __syncthreads();
     // Up to 16 tiles per thread (16384 tiles) the totals are read once, all
     // loads in flight together, and kept in registers for the later passes.
@@ -988,22 +1026,63 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     uint32_t segLog2 = kSegLog2Min;
     while (segLog2 < kSegLog2Max && (total >> segLog2) > B.raster_blocks) segLog2++;
     // a tile's segment index must fit above the tile bits of a work item
-    while (segLog2 < kSegLog2Max && tile_segments(maxN, segLog2) > (1u << (32 - kTileBits))) segLog2++;
-    const bool segsFit = tile_segments(maxN, segLog2) <= (1u << (32 - kTileBits));
+    while (segLog2 < kSegLog2Max && tile_segments(maxN, segLog2) > (1u << kSegIndexBits)) segLog2++;
+    const bool segsFit = tile_segments(maxN, segLog2) <= (1u << kSegIndexBits);
+    // A tile's segment log2: the frame's, or kSegLog2Heavy for a tile whose
+    // references are mostly tile-level triangles (a whole wave each), so that
+    // such a tile is spread over several workgroups.  Finer segments never
+    // make a tile's segment index overflow its bits.
+    auto tile_log2 = [&](uint32_t n, uint32_t big, bool split) {
+        if (!split || big < kHeavyBigMin || 4 * big < n) return segLog2;
+        uint32_t s = kSegLog2Heavy;
+        while (s < segLog2 && tile_segments(n, s) > (1u << kSegIndexBits)) s++;
+        return s;
+    };
+    uint32_t cacheBig[kCache];
+    if (cached) {
+#pragma unroll
+        for (uint32_t k = 0; k < kCache; k++) cacheBig[k] = t0 + k < t1 ? B.tile_big[t0 + k] : 0u;
+    }
+    // The finer segments of heavy tiles are taken only while the work list
+    // has room for them: segments with and without them, summed over the block.
+    {
+        uint32_t segs = 0, extra = 0;
+        auto add = [&](uint32_t n, uint32_t big) {
+            const uint32_t plain = tile_segments(n, segLog2);
+            segs += plain;
+            extra += tile_segments(n, tile_log2(n, big, true)) - plain;
+        };
+        if (cached) {
+#pragma unroll
+            for (uint32_t k = 0; k < kCache; k++) add(cache[k], cacheBig[k]);
+        } else {
+            for (uint32_t t = t0; t < t1; t++) add(B.tile_totals[t], B.tile_big[t]);
+        }
+        for (int d = 1; d < 64; d <<= 1) {
+            segs += __shfl_xor(segs, d);
+            extra += __shfl_xor(extra, d);
+        }
+        if (__lane_id() == 0) {
+            atomicAdd(&sSegs, segs);
+            atomicAdd(&sExtra, extra);
+        }
+        __syncthreads();
+    }
+    const bool split = (uint64_t)sSegs + sExtra <= B.work_cap;
     // Per thread: full segments (cnt[0]) and partial last segments by size
-    // class (cnt[1 + c], class c = remainder >> (segLog2 - 2)).
-    const uint32_t segMask = (1u << segLog2) - 1;
+    // class (cnt[1 + c], class c = remainder >> (the tile's segment log2 - 2)).
     uint32_t cnt[5] = {0, 0, 0, 0, 0};
-    auto count = [&](uint32_t n) {
-        cnt[0] += n >> segLog2;
-        const uint32_t r = n & segMask;
-        if (r) cnt[1 + (r >> (segLog2 - 2))]++;
+    auto count = [&](uint32_t n, uint32_t big) {
+        const uint32_t s = tile_log2(n, big, split);
+        cnt[0] += n >> s;
+        const uint32_t r = n & ((1u << s) - 1);
+        if (r) cnt[1 + (r >> (s - 2))]++;
     };
     if (cached) {
 #pragma unroll
-        for (uint32_t k = 0; k < kCache; k++) count(cache[k]);
+        for (uint32_t k = 0; k < kCache; k++) count(cache[k], cacheBig[k]);
     } else {
-        for (uint32_t t = t0; t < t1; t++) count(B.tile_totals[t]);
+        for (uint32_t t = t0; t < t1; t++) count(B.tile_totals[t], B.tile_big[t]);
     }
     // the tile-reference prefix rides along as a sixth value
     uint32_t pos[6] = {cnt[0], cnt[1], cnt[2], cnt[3], cnt[4], localRefs}, tot[6];
@@ -1017,17 +1096,21 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     pos[3] += tot[0] + tot[4];
     pos[2] += tot[0] + tot[4] + tot[3];
     pos[1] += tot[0] + tot[4] + tot[3] + tot[2];
-    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && segsFit;
-    auto place = [&](uint32_t t, uint32_t n) {
+    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && totalSegs <= kImageMask && segsFit;
+    auto place = [&](uint32_t t, uint32_t n, uint32_t big) {
         B.tile_start[t] = run;
         run += n;
         if (fits) {
-            const uint32_t nf = n >> segLog2, r = n & segMask;
-            B.tile_item[t] = seg;
-            for (uint32_t s = 0; s < nf; s++) B.work[pos[0]++] = t | (s << kTileBits);
+            const uint32_t s = tile_log2(n, big, split);
+            const uint32_t nf = n >> s, r = n & ((1u << s) - 1);
+            B.tile_item[t] = seg | (s << kImageBits);
+            // a tile of one segment is marked (kSingleSegment): its image is
+            // the 32-bit winner orders
+            const uint32_t single = nf + (r ? 1u : 0u) == 1 ? kSingleSegment : 0u;
+            for (uint32_t i = 0; i < nf; i++) B.work[pos[0]++] = t | (i << kTileBits) | single;
             if (r) {
-                const uint32_t c = 1 + (r >> (segLog2 - 2));
-                B.work[pos[c]++] = t | (nf << kTileBits);
+                const uint32_t c = 1 + (r >> (s - 2));
+                B.work[pos[c]++] = t | (nf << kTileBits) | single;
             }
             seg += nf + (r ? 1u : 0u);
         }
@@ -1035,9 +1118,9 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     if (cached) {
 #pragma unroll
         for (uint32_t k = 0; k < kCache; k++)
-            if (t0 + k < t1) place(t0 + k, cache[k]);
+            if (t0 + k < t1) place(t0 + k, cache[k], cacheBig[k]);
     } else {
-        for (uint32_t t = t0; t < t1; t++) place(t, B.tile_totals[t]);
+        for (uint32_t t = t0; t < t1; t++) place(t, B.tile_totals[t], B.tile_big[t]);
     }
     if (threadIdx.x == 0) {
         B.tile_start[nt] = total;
@@ -1429,8 +1512,24 @@ __device__ __forceinline__ uint32_t tex_level(const TexDesc& d, uint32_t l, int&
     }
     return off;
 }
-// Sample(uv, differentials): nearest / bilinear on level 0, or trilinear with
-// lod = log2 of the longer of the two texel-space differentials.
+// The trilinear blend of the two mip levels around lod (clamped to the chain).
+__device__ __forceinline__ float3 tex_trilinear_at(const FrameParams& P, const TexDesc& d, float lod, float u,
+                                                   float v) {
+    lod = fminf(fmaxf(lod, 0.0f), (float)(d.levels - 1));
+    const float fl = floorf(lod);
+    const uint32_t l0 = (uint32_t)fl, l1 = min(l0 + 1, d.levels - 1);
+    const float t = lod - fl;
+    int w0, h0, w1, h1;
+    const uint32_t o0 = tex_level(d, l0, w0, h0), o1 = tex_level(d, l1, w1, h1);
+    const float3 a = tex_bilinear(P, o0, w0, h0, u, v), b = tex_bilinear(P, o1, w1, h1, u, v);
+    const float s = 1.0f - t;
+    return make_float3(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z);
+}
+// Sample(uv, differentials): nearest / bilinear on level 0; trilinear with
+// lod = log2 of the longer of the two texel-space differentials; or N:1
+// anisotropic (N = 4, 8, 16): N trilinear taps spread evenly along the longer
+// differential (tap k at offset (2k + 1 - N) / 2N of it) at the lod of the
+// shorter one, raised to at least 1/N of the longer, averaged in tap order.
 __device__ __forceinline__ float3 tex_sample(const FrameParams& P, const TexDesc& d, float u, float v, float dux,
                                              float dvx, float duy, float dvy) {
     if (d.levels == 0) return make_float3(d.color[0], d.color[1], d.color[2]);
@@ -1441,17 +1540,28 @@ __device__ __forceinline__ float3 tex_sample(const FrameParams& P, const TexDesc
     }
     if (P.tex_filter == kFilterLinear) return tex_bilinear(P, d.offset, w, h, u, v);
     const float ax = dux * (float)w, ay = dvx * (float)h, bx = duy * (float)w, by = dvy * (float)h;
-    const float rho2 = fmaxf(ax * ax + ay * ay, bx * bx + by * by);
-    float lod = rho2 > 0.0f ? 0.5f * log2f(rho2) : 0.0f;
-    lod = fminf(fmaxf(lod, 0.0f), (float)(d.levels - 1));
-    const float fl = floorf(lod);
-    const uint32_t l0 = (uint32_t)fl, l1 = min(l0 + 1, d.levels - 1);
-    const float t = lod - fl;
-    int w0, h0, w1, h1;
-    const uint32_t o0 = tex_level(d, l0, w0, h0), o1 = tex_level(d, l1, w1, h1);
-    const float3 a = tex_bilinear(P, o0, w0, h0, u, v), b = tex_bilinear(P, o1, w1, h1, u, v);
-    const float s = 1.0f - t;
-    return make_float3(s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z);
+    const float la2 = ax * ax + ay * ay, lb2 = bx * bx + by * by;
+    if (P.tex_filter == kFilterTrilinear) {
+        const float rho2 = fmaxf(la2, lb2);
+        return tex_trilinear_at(P, d, rho2 > 0.0f ? 0.5f * log2f(rho2) : 0.0f, u, v);
+    }
+    const int N = 4 << (P.tex_filter - kFilterAniso4);
+    const bool xMajor = la2 >= lb2;
+    const float mu = xMajor ? dux : duy, mv = xMajor ? dvx : dvy;
+    const float maj2 = xMajor ? la2 : lb2, n2 = (float)(N * N);
+    float min2 = xMajor ? lb2 : la2;
+    if (min2 * n2 < maj2) min2 = maj2 / n2;
+    const float lod = min2 > 0.0f ? 0.5f * log2f(min2) : 0.0f;
+    float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+    for (int k = 0; k < N; k++) {
+        const float sk = (float)(2 * k + 1 - N) / (float)(2 * N);
+        const float3 c = tex_trilinear_at(P, d, lod, u + sk * mu, v + sk * mv);
+        acc.x += c.x;
+        acc.y += c.y;
+        acc.z += c.z;
+    }
+    const float inv = 1.0f / (float)N;
+    return make_float3(acc.x * inv, acc.y * inv, acc.z * inv);
 }
 
 // Perspective-correct texture coordinate of the triangle at pixel (x, y)
@@ -1629,16 +1739,18 @@ __device__ __forceinline__ RTri broadcast_rtri(const RTri& t, int l) {
 }
 
 // One raster work item: references [seg << segLog2, (seg + 1) << segLog2) of
-// a tile.  Each wave streams its own 64-reference slices through the tile's LDS
+// a tile, segLog2 being the tile's (tile_item).  Each wave streams its own 64-reference slices through the tile's LDS
 // key buffer (no workgroup barrier per slice): small triangles take one lane
 // each, medium ones one quad per lane, tile-level ones the whole wave.  The
 // item's keys go to partials[workIndex].
 template <bool kMS>
 __device__ __forceinline__ void raster_item(const BatchParams& Q, const FrameParams* __restrict__ jobs,
-                                            const DeviceBuffers& B, uint32_t workIndex, uint32_t segLog2,
+                                            const DeviceBuffers& B, uint32_t workIndex,
                                             uint64_t* keys, uint16_t (*sOwner)[kOwnerCap]) {
     const uint32_t workItem = __builtin_amdgcn_readfirstlane(B.work[workIndex]);
-    const uint32_t gtile = workItem & (kMaxBatchTiles - 1), seg = workItem >> kTileBits;
+    // segFlag: the segment index, with kSingleSegment >> kTileBits above it
+    const uint32_t gtile = workItem & (kMaxBatchTiles - 1), segFlag = workItem >> kTileBits;
+    const uint32_t seg = segFlag & ((1u << kSegIndexBits) - 1);
     const FrameParams& P = jobs[job_of(Q, Q.tile_job, gtile)];
     const uint32_t tile = gtile - P.tile_base;
     const int tx = (int)(tile % P.tiles_x), ty = (int)(tile / P.tiles_x);
@@ -1646,7 +1758,9 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
     const int tmaxX = min(tminX + kTileSize, (int)P.width), tmaxY = min(tminY + kTileSize, (int)P.height);
     const uint32_t start = B.tile_start[gtile];
     const uint32_t n = B.tile_start[gtile + 1] - start;
+    const uint32_t tileItem = B.tile_item[gtile], segLog2 = tileItem >> kImageBits;
     const uint32_t segBegin = seg << segLog2, segEnd = min(n, (seg + 1) << segLog2);
+
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     // Multisampling keeps kPassSamples samples' keys in LDS and takes the
     // item's references once per group of samples.
@@ -1810,10 +1924,16 @@ __device__ __forceinline__ void raster_item(const BatchParams& Q, const FramePar
             }
         }
         __syncthreads();
-        uint64_t* g = B.partials + ((size_t)(B.tile_item[gtile] + seg) * S + sp.base) * (kTileSize * kTileSize);
+        uint64_t* g = B.partials + ((size_t)((tileItem & kImageMask) + seg) * S + sp.base) * (kTileSize * kTileSize);
+        // A tile of one segment has no other image to merge with, and k_shade
+        // recomputes the depth: the low words of the keys (the winners'
+        // orders; 0xFFFFFFFF where nothing was drawn) are its image.
+        const bool orders = !kMS && (segFlag >> kSegIndexBits);
         for (uint32_t i = threadIdx.x; i < sp.n * kTileSize * kTileSize; i += kBlock) {
             const uint32_t plane = i / (kTileSize * kTileSize), p = i % (kTileSize * kTileSize);
-            g[i] = keys[plane * kKeyPlane<kMS> + (p / kTileSize) * kKeyStride<kMS> + p % kTileSize];
+            const uint64_t k = keys[plane * kKeyPlane<kMS> + (p / kTileSize) * kKeyStride<kMS> + p % kTileSize];
+            if (orders) reinterpret_cast<uint32_t*>(g)[i] = (uint32_t)k;
+            else g[i] = k;
         }
         __syncthreads();
     }
@@ -1850,7 +1970,7 @@ __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, c
     __shared__ uint16_t sOwner[kBlock / kWave][kOwnerCap];
     __shared__ uint32_t sItem;
     if (B.counters[kCntOverflow] & kOvRefs) return;
-    const uint32_t nItems = B.counters[kCntWorkItems], segLog2 = B.counters[kCntSegLog2];
+    const uint32_t nItems = B.counters[kCntWorkItems];
     // Single-sample batches: the empty tiles are cleared here, a share per
     // workgroup, before any item is taken (k_shade skips them).  Lane l looks
     // at tile blockIdx.x + l * gridDim.x of each pass; every wave computes the
@@ -1874,7 +1994,7 @@ __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, c
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(sItem);
         if (item >= nItems) break;
-        raster_item<kMS>(Q, jobs, B, item, segLog2, keys, sOwner);
+        raster_item<kMS>(Q, jobs, B, item, keys, sOwner);
         __syncthreads();
     }
 }
@@ -1950,11 +2070,15 @@ __global__ __launch_bounds__(512) void k_shade(BatchParams Q, const FrameParams*
     const int x = (int)(tile % P.tiles_x) * kTileSize + (int)(pix & 31u);
     const int y = (int)(tile / P.tiles_x) * kTileSize + (int)(pix >> 5);
     if (x >= (int)P.width || y >= (int)P.height) return;
-    const uint32_t nseg = tile_segments(n, B.counters[kCntSegLog2]), S = P.samples;
-    const uint64_t* g = B.partials + (size_t)B.tile_item[gtile] * S * kPix + pix;
+    const uint32_t tileItem = B.tile_item[gtile];
+    const uint32_t nseg = tile_segments(n, tileItem >> kImageBits), S = P.samples;
+    const uint64_t* g = B.partials + (size_t)(tileItem & kImageMask) * S * kPix + pix;
     const size_t o = P.pix_base + (size_t)(P.height - 1 - y) * P.width + x;
     if (S == 1) {
-        const uint64_t key = n ? tile_key(g, nseg, 1, 0) : bg_key();
+        // A single-segment tile's image holds 32-bit low words (k_raster).
+        const uint64_t key = !n ? bg_key()
+                             : nseg == 1 ? (uint64_t)reinterpret_cast<const uint32_t*>(g - pix)[pix] | 0xBF80000000000000ull
+                                         : tile_key(g, nseg, 1, 0);
         if (key == bg_key()) {
             B.color[o] = 0u;
             B.depth[o] = 1.0f;

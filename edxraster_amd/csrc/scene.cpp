@@ -7,6 +7,7 @@
 // Winding: front faces are clockwise on screen (det > 0 in
 // RasterTriangle::Setup, EDXRaster/Core/RasterTriangle.h:49-51).
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -282,6 +283,107 @@ void edxr_raster_matrix(uint32_t width, uint32_t height, float out[16]) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// The placement of a loaded mesh (ObjMesh::LoadFromObj / LoadSphere /
+// LoadPlane take pos, scl, rot): p' = T(pos) Rz Ry Rx S(scl) p with the
+// rotation in degrees, n' = normalize(Rz Ry Rx (n / scl)), in double.
+struct Placement {
+    double R[3][3], pos[3], scl[3];
+    Placement(const float p[3], const float s[3], const float r[3]) {
+        const double d2r = 3.14159265358979323846 / 180.0;
+        const double cx = cos(r[0] * d2r), sx = sin(r[0] * d2r), cy = cos(r[1] * d2r), sy = sin(r[1] * d2r),
+                     cz = cos(r[2] * d2r), sz = sin(r[2] * d2r);
+        const double M[3][3] = {{cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx},
+                                {sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx},
+                                {-sy, cy * sx, cy * cx}};
+        memcpy(R, M, sizeof(R));
+        for (int i = 0; i < 3; i++) {
+            pos[i] = p[i];
+            scl[i] = s[i];
+        }
+    }
+    // Places model-space (p, n) and stores it as vertex v of out.
+    void apply(const double p[3], const double n[3], edxr_host_mesh* out, uint32_t v) const {
+        double sp[3], sn[3], wp[3], wn[3];
+        for (int j = 0; j < 3; j++) {
+            sp[j] = p[j] * scl[j];
+            sn[j] = n[j] / scl[j];
+        }
+        for (int i = 0; i < 3; i++) {
+            wp[i] = R[i][0] * sp[0] + R[i][1] * sp[1] + R[i][2] * sp[2] + pos[i];
+            wn[i] = R[i][0] * sn[0] + R[i][1] * sn[1] + R[i][2] * sn[2];
+        }
+        const double len = sqrt(wn[0] * wn[0] + wn[1] * wn[1] + wn[2] * wn[2]);
+        const double inv = len > 0.0 ? 1.0 / len : 0.0;
+        out->px[v] = (float)wp[0];
+        out->py[v] = (float)wp[1];
+        out->pz[v] = (float)wp[2];
+        out->nx[v] = (float)(wn[0] * inv);
+        out->ny[v] = (float)(wn[1] * inv);
+        out->nz[v] = (float)(wn[2] * inv);
+    }
+};
+
+// Texture coordinates, white vertex colours and texture slot 0 for every
+// triangle: the vertex and texture layout of a generated Mesh
+// (Vertex_PositionNormalTex, one ConstantTexture2D, Utils/Mesh.cpp:36-70).
+bool add_generated_extras(edxr_host_mesh* m) {
+    m->tu = (float*)malloc(sizeof(float) * (m->num_vertices ? m->num_vertices : 1));
+    m->tv = (float*)malloc(sizeof(float) * (m->num_vertices ? m->num_vertices : 1));
+    m->tex_ids = (uint32_t*)calloc(m->num_triangles ? m->num_triangles : 1, sizeof(uint32_t));
+    if (!m->tu || !m->tv || !m->tex_ids) return false;
+    for (uint32_t v = 0; v < m->num_vertices; v++) m->cr[v] = m->cg[v] = m->cb[v] = 1.0f;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edxr_load_sphere(const float pos[3], const float scl[3], const float rot[3], float radius, uint32_t slices,
+                     uint32_t stacks, edxr_host_mesh* out) {
+    if (!out || !pos || !scl || !rot || slices < 3 || stacks < 2) return EDXR_E_INVALID;
+    const uint64_t V = (uint64_t)(stacks + 1) * (slices + 1), T = 2ull * slices * stacks;
+    if (!alloc_mesh(out, V, T) || !add_generated_extras(out)) {
+        edxr_scene_free(out);
+        return EDXR_E_NOMEM;
+    }
+    const double origin[3] = {0.0, 0.0, 0.0};
+    build_sphere(out, 0, 0, slices, stacks, radius, origin);
+    const Placement X(pos, scl, rot);
+    const uint32_t row = slices + 1;
+    for (uint32_t v = 0; v < out->num_vertices; v++) {
+        const double p[3] = {out->px[v], out->py[v], out->pz[v]}, n[3] = {out->nx[v], out->ny[v], out->nz[v]};
+        X.apply(p, n, out, v);
+        out->tu[v] = (float)(v % row) / (float)slices;
+        out->tv[v] = (float)(v / row) / (float)stacks;
+    }
+    return EDXR_OK;
+}
+
+int edxr_load_plane(const float pos[3], const float scl[3], const float rot[3], float length, edxr_host_mesh* out) {
+    if (!out || !pos || !scl || !rot || !(length > 0.0f)) return EDXR_E_INVALID;
+    if (!alloc_mesh(out, 4, 2) || !add_generated_extras(out)) {
+        edxr_scene_free(out);
+        return EDXR_E_NOMEM;
+    }
+    const double h = 0.5 * (double)length;
+    const double P[4][3] = {{-h, 0.0, -h}, {h, 0.0, -h}, {-h, 0.0, h}, {h, 0.0, h}};
+    const double n[3] = {0.0, 1.0, 0.0};
+    const Placement X(pos, scl, rot);
+    for (uint32_t v = 0; v < 4; v++) {
+        X.apply(P[v], n, out, v);
+        out->tu[v] = (float)(v & 1u);
+        out->tv[v] = (float)(v >> 1);
+    }
+    const uint32_t I[6] = {0, 2, 1, 1, 2, 3};  // clockwise on screen seen from +y
+    memcpy(out->indices, I, sizeof(I));
+    return EDXR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
 struct ObjMaterial {
     std::string name;
     float kd[3] = {0.9f, 0.9f, 0.9f};
@@ -518,33 +620,15 @@ int edxr_load_obj(const char* path, const float pos[3], const float scl[3], cons
     }
     for (size_t t = 0; t < T; t++)
         out->tex_ids[t] = corners[3 * t].m != UINT32_MAX ? corners[3 * t].m : (uint32_t)mats.size();
-    const double d2r = 3.14159265358979323846 / 180.0;
-    const double cx = cos(rot[0] * d2r), sx = sin(rot[0] * d2r), cy = cos(rot[1] * d2r), sy = sin(rot[1] * d2r),
-                 cz = cos(rot[2] * d2r), sz = sin(rot[2] * d2r);
-    // R = Rz Ry Rx
-    const double R[3][3] = {{cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx},
-                            {sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx},
-                            {-sy, cy * sx, cy * cx}};
+    const Placement X(pos, scl, rot);
     for (size_t v = 0; v < first.size(); v++) {
         const Corner& c = corners[first[v]];
         double p[3], n[3];
         for (int j = 0; j < 3; j++) {
-            p[j] = P[3 * c.p + j] * scl[j];
-            n[j] = (c.n != UINT32_MAX ? N[3 * c.n + j] : AN[3 * c.p + j]) / scl[j];
+            p[j] = P[3 * c.p + j];
+            n[j] = c.n != UINT32_MAX ? N[3 * c.n + j] : AN[3 * c.p + j];
         }
-        double wp[3], wn[3];
-        for (int i = 0; i < 3; i++) {
-            wp[i] = R[i][0] * p[0] + R[i][1] * p[1] + R[i][2] * p[2] + pos[i];
-            wn[i] = R[i][0] * n[0] + R[i][1] * n[1] + R[i][2] * n[2];
-        }
-        const double len = sqrt(wn[0] * wn[0] + wn[1] * wn[1] + wn[2] * wn[2]);
-        const double inv = len > 0.0 ? 1.0 / len : 0.0;
-        out->px[v] = (float)wp[0];
-        out->py[v] = (float)wp[1];
-        out->pz[v] = (float)wp[2];
-        out->nx[v] = (float)(wn[0] * inv);
-        out->ny[v] = (float)(wn[1] * inv);
-        out->nz[v] = (float)(wn[2] * inv);
+        X.apply(p, n, out, (uint32_t)v);
         static const float kDefaultKd[3] = {0.9f, 0.9f, 0.9f};
         const float* kd = c.m != UINT32_MAX ? mats[c.m].kd : kDefaultKd;
         out->cr[v] = kd[0];
@@ -654,6 +738,126 @@ int edxr_load_bmp(const char* path, uint32_t* width, uint32_t* height, uint8_t**
     *height = H;
     *rgba = img;
     return EDXR_OK;
+}
+
+// Truevision TGA: image types 2 (uncompressed true colour) and 10 (run-length
+// encoded true colour), 24 or 32 bits per pixel, BGR(A) bytes, no colour map;
+// rows bottom-up unless bit 5 of the descriptor (top-left origin) is set, and
+// returned bottom-up as the BMP reader returns them.
+int edxr_load_tga(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba) {
+    if (!path || !width || !height || !rgba) return EDXR_E_INVALID;
+    *rgba = nullptr;
+    std::string d;
+    if (!read_file(path, d)) return EDXR_E_IO;
+    if (d.size() < 18) return EDXR_E_FORMAT;
+    const uint8_t* b = (const uint8_t*)d.data();
+    const uint32_t idLen = b[0], cmapType = b[1], type = b[2];
+    const uint32_t W = b[12] | (b[13] << 8), H = b[14] | (b[15] << 8), bpp = b[16], desc = b[17];
+    if (cmapType != 0 || (type != 2 && type != 10) || (bpp != 24 && bpp != 32) || !W || !H || W > 16384 ||
+        H > 16384 || (desc & 0x10))
+        return EDXR_E_FORMAT;
+    const size_t px = bpp / 8, n = (size_t)W * H;
+    size_t at = 18 + idLen;
+    std::vector<uint8_t> raw(n * px);
+    if (type == 2) {
+        if (at + raw.size() > d.size()) return EDXR_E_FORMAT;
+        memcpy(raw.data(), b + at, raw.size());
+    } else {
+        for (size_t i = 0; i < n;) {
+            if (at >= d.size()) return EDXR_E_FORMAT;
+            const uint32_t h = b[at++], count = (h & 0x7F) + 1;
+            if (i + count > n) return EDXR_E_FORMAT;
+            if (h & 0x80) {  // one pixel repeated
+                if (at + px > d.size()) return EDXR_E_FORMAT;
+                for (uint32_t k = 0; k < count; k++) memcpy(&raw[(i + k) * px], b + at, px);
+                at += px;
+            } else {         // count literal pixels
+                if (at + count * px > d.size()) return EDXR_E_FORMAT;
+                memcpy(&raw[i * px], b + at, count * px);
+                at += count * px;
+            }
+            i += count;
+        }
+    }
+    uint8_t* img = (uint8_t*)malloc(4 * n);
+    if (!img) return EDXR_E_NOMEM;
+    const bool topDown = (desc & 0x20) != 0;
+    for (uint32_t y = 0; y < H; y++) {
+        const uint8_t* src = raw.data() + (size_t)(topDown ? H - 1 - y : y) * W * px;
+        for (uint32_t x = 0; x < W; x++) {
+            const uint8_t* p = src + x * px;
+            uint8_t* q = img + 4ull * ((size_t)y * W + x);
+            q[0] = p[2];
+            q[1] = p[1];
+            q[2] = p[0];
+            q[3] = px == 4 ? p[3] : 255;
+        }
+    }
+    *width = W;
+    *height = H;
+    *rgba = img;
+    return EDXR_OK;
+}
+
+// Binary PPM (P6, maxval 255): rows top-down in the file, returned bottom-up.
+int edxr_load_ppm(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba) {
+    if (!path || !width || !height || !rgba) return EDXR_E_INVALID;
+    *rgba = nullptr;
+    std::string d;
+    if (!read_file(path, d)) return EDXR_E_IO;
+    size_t at = 0;
+    // the header's four tokens: magic, width, height, maxval ('#' comments skipped)
+    auto token = [&](std::string& t) {
+        t.clear();
+        while (at < d.size()) {
+            if (d[at] == '#') {
+                while (at < d.size() && d[at] != '\n') at++;
+            } else if (isspace((unsigned char)d[at])) {
+                at++;
+            } else {
+                break;
+            }
+        }
+        while (at < d.size() && !isspace((unsigned char)d[at])) t += d[at++];
+        return !t.empty();
+    };
+    std::string magic, ws, hs, ms;
+    if (!token(magic) || magic != "P6" || !token(ws) || !token(hs) || !token(ms)) return EDXR_E_FORMAT;
+    const long W = strtol(ws.c_str(), nullptr, 10), H = strtol(hs.c_str(), nullptr, 10);
+    if (ms != "255" || W <= 0 || H <= 0 || W > 16384 || H > 16384 || at >= d.size()) return EDXR_E_FORMAT;
+    at++;  // the single whitespace byte after maxval
+    const size_t n = (size_t)W * H;
+    if (at + 3 * n > d.size()) return EDXR_E_FORMAT;
+    uint8_t* img = (uint8_t*)malloc(4 * n);
+    if (!img) return EDXR_E_NOMEM;
+    const uint8_t* b = (const uint8_t*)d.data() + at;
+    for (long y = 0; y < H; y++)
+        for (long x = 0; x < W; x++) {
+            const uint8_t* p = b + 3 * ((size_t)(H - 1 - y) * W + x);
+            uint8_t* q = img + 4 * ((size_t)y * W + x);
+            q[0] = p[0];
+            q[1] = p[1];
+            q[2] = p[2];
+            q[3] = 255;
+        }
+    *width = (uint32_t)W;
+    *height = (uint32_t)H;
+    *rgba = img;
+    return EDXR_OK;
+}
+
+// The diffuse-map reader of ImageTexture (Utils/Mesh.cpp:26-27): BMP, TGA or
+// PPM by the file's first bytes ("BM", "P6", else TGA).
+int edxr_load_image(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba) {
+    if (!path || !width || !height || !rgba) return EDXR_E_INVALID;
+    FILE* f = fopen(path, "rb");
+    if (!f) return EDXR_E_IO;
+    char m[2] = {0, 0};
+    const size_t got = fread(m, 1, 2, f);
+    fclose(f);
+    if (got == 2 && m[0] == 'B' && m[1] == 'M') return edxr_load_bmp(path, width, height, rgba);
+    if (got == 2 && m[0] == 'P' && m[1] == '6') return edxr_load_ppm(path, width, height, rgba);
+    return edxr_load_tga(path, width, height, rgba);
 }
 
 void edxr_free(void* p) { free(p); }

@@ -26,15 +26,25 @@ def _mat(m):
     return a
 
 
-def load_bmp(path):
-    """(H, W, 4) uint8 RGBA of an uncompressed 24/32-bit BMP, rows bottom-up (v = 0 first)."""
+def _load_rgba(fn, path):
     w, h = C.c_uint32(), C.c_uint32()
     p = C.POINTER(C.c_uint8)()
-    _check(None, L.lib().edxr_load_bmp(os.fsencode(path), C.byref(w), C.byref(h), C.byref(p)))
+    _check(None, fn(os.fsencode(path), C.byref(w), C.byref(h), C.byref(p)))
     try:
         return np.ctypeslib.as_array(p, shape=(h.value, w.value, 4)).copy()
     finally:
         L.lib().edxr_free(p)
+
+
+def load_bmp(path):
+    """(H, W, 4) uint8 RGBA of an uncompressed 24/32-bit BMP, rows bottom-up (v = 0 first)."""
+    return _load_rgba(L.lib().edxr_load_bmp, path)
+
+
+def load_image(path):
+    """(H, W, 4) uint8 RGBA of a BMP, TGA (types 2 and 10) or binary PPM
+    diffuse map, rows bottom-up (v = 0 first)."""
+    return _load_rgba(L.lib().edxr_load_image, path)
 
 
 class Mesh:
@@ -89,14 +99,36 @@ class Mesh:
         L.lib().edxr_scene_free(C.byref(hm))
         return Mesh(*planes, idx, **extra)
 
-    # Mesh::LoadSphere (EDXRaster/Utils/Mesh.cpp:53-70)
+    # A UV sphere with seeded vertex colours (the synthetic benchmark scenes).
     @staticmethod
-    def load_sphere(slices=64, stacks=64, radius=1.0, center=(0.0, 0.0, 0.0), color_seed=1):
+    def sphere(slices=64, stacks=64, radius=1.0, center=(0.0, 0.0, 0.0), color_seed=1):
         hm = L.HostMesh()
         c = np.asarray(center, dtype=np.float32)
         rc = L.lib().edxr_scene_sphere(C.byref(hm), slices, stacks, radius, L.fptr(c), color_seed)
         _check(None, rc)
         return Mesh._from_host(hm)
+
+    # Mesh::LoadSphere (EDXRaster/Utils/Mesh.h:39-44, Utils/Mesh.cpp:53-70):
+    # texture coordinates, white vertex colours, texture slot 0 everywhere.
+    @staticmethod
+    def load_sphere(pos, scl, rot, radius, slices=64, stacks=64):
+        hm = L.HostMesh()
+        p, s, r = (np.asarray(v, dtype=np.float32) for v in (pos, scl, rot))
+        _check(None, L.lib().edxr_load_sphere(L.fptr(p), L.fptr(s), L.fptr(r), radius, slices, stacks, C.byref(hm)))
+        return Mesh._from_host(hm)
+
+    # Mesh::LoadPlane (EDXRaster/Utils/Mesh.h:34-37, Utils/Mesh.cpp:36-51).
+    @staticmethod
+    def load_plane(pos, scl, rot, length):
+        hm = L.HostMesh()
+        p, s, r = (np.asarray(v, dtype=np.float32) for v in (pos, scl, rot))
+        _check(None, L.lib().edxr_load_plane(L.fptr(p), L.fptr(s), L.fptr(r), length, C.byref(hm)))
+        return Mesh._from_host(hm)
+
+    def get_bounds(self):
+        """Mesh::GetBounds: (min, max) corners of the vertex positions."""
+        P = self.positions()
+        return P.min(0), P.max(0)
 
     # Mesh::LoadPlane stand-in: a noise heightfield (EDXRaster/Utils/Mesh.cpp:36-51)
     @staticmethod
@@ -124,13 +156,19 @@ class Mesh:
         return [(m.name.decode(), tuple(m.kd), m.map_kd.decode()) for m in arr[:n]]
 
     # The texture slots of an OBJ's triangles (tex_ids): ImageTexture for a
-    # material with a .bmp diffuse map, ConstantTexture2D(Kd) otherwise, and a
-    # last 0.9-grey slot for faces without a material.
+    # material whose diffuse map reads as BMP, TGA or PPM, ConstantTexture2D(Kd)
+    # otherwise, and a last 0.9-grey slot for faces without a material.
     @staticmethod
     def obj_texture_slots(path):
         slots = []
         for name, kd, mp in Mesh.obj_materials(path):
-            slots.append(load_bmp(mp) if mp.lower().endswith(".bmp") and os.path.exists(mp) else kd)
+            img = None
+            if mp and os.path.exists(mp):
+                try:
+                    img = load_image(mp)
+                except L.EdxrError:
+                    img = None
+            slots.append(kd if img is None else img)
         return slots + [(0.9, 0.9, 0.9)]
 
     @staticmethod

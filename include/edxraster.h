@@ -63,7 +63,18 @@ enum edxr_shader_kind {
 enum edxr_texture_filter {
     EDXR_FILTER_NEAREST = 0,
     EDXR_FILTER_LINEAR = 1,     /* bilinear, level 0 */
-    EDXR_FILTER_TRILINEAR = 2   /* default (RenderStates::DefaultSettings) */
+    EDXR_FILTER_TRILINEAR = 2,  /* default (RenderStates::DefaultSettings):
+                                 * lod = log2 of the longer texel-space
+                                 * differential, two bilinear levels blended */
+    /* N:1 anisotropic, N = 4, 8, 16 (the viewer's filter choices,
+     * RealtimeViewer/Main.cpp:99-108): N trilinear taps spread evenly along
+     * the longer texel-space differential (tap k at offset (2k + 1 - N) / 2N
+     * of its uv vector), each at lod = log2 of the shorter differential's
+     * length raised to at least 1/N of the longer, averaged in tap order.
+     * EDXUtil's filter is not available: this is the library's declaration. */
+    EDXR_FILTER_ANISO_4X = 3,
+    EDXR_FILTER_ANISO_8X = 4,
+    EDXR_FILTER_ANISO_16X = 5
 };
 
 /* Pixel-shader parameters.  The reference values are ambient 0.2, kd 3,
@@ -333,6 +344,23 @@ int edxr_scene_heightfield(edxr_host_mesh* out, uint32_t n, float amplitude, uin
                            uint64_t color_seed);
 /* 16x16 grid of spheres (segments x segments), radii U[0.5,2], one mesh. */
 int edxr_scene_sphere_grid(edxr_host_mesh* out, uint32_t grid, uint32_t segments, uint64_t seed);
+/* Mesh::LoadSphere (Utils/Mesh.cpp:53-70) with ObjMesh::LoadSphere's
+ * arguments: a UV sphere of the given radius at the origin (the topology of
+ * edxr_scene_sphere: (stacks+1)*(slices+1) vertices, 2*slices*stacks
+ * triangles, outward faces clockwise on screen), texture coordinates
+ * (j / slices, i / stacks) for slice j of stack i, white vertex colours,
+ * texture slot 0 on every triangle, placed as edxr_load_obj places a file:
+ * p' = T(pos) Rz Ry Rx S(scl) p, rotation in degrees.  Free with
+ * edxr_scene_free.  EDXUtil's ObjMesh is not available, so the topology and
+ * the texture coordinates are this library's declaration. */
+int edxr_load_sphere(const float pos[3], const float scl[3], const float rot[3], float radius, uint32_t slices,
+                     uint32_t stacks, edxr_host_mesh* out);
+/* Mesh::LoadPlane (Utils/Mesh.cpp:36-51): a square of side `length` in the
+ * y = 0 plane centred at the origin, normal +y, corners (-l/2, 0, -l/2),
+ * (l/2, 0, -l/2), (-l/2, 0, l/2), (l/2, 0, l/2) with texture coordinates
+ * (0,0), (1,0), (0,1), (1,1), triangles (0, 2, 1) and (1, 2, 3) (front faces
+ * seen from +y), placed as edxr_load_sphere.  Declared, as above. */
+int edxr_load_plane(const float pos[3], const float scl[3], const float rot[3], float length, edxr_host_mesh* out);
 /* Appends b to a (a's indices unchanged, b's shifted). */
 int edxr_scene_merge(edxr_host_mesh* a, const edxr_host_mesh* b);
 void edxr_scene_free(edxr_host_mesh* m);
@@ -365,6 +393,14 @@ int64_t edxr_obj_materials(const char* path, edxr_obj_material* out, uint32_t ca
  * for .bmp diffuse maps) as RGBA8 with its rows in file order (bottom-up,
  * which is v = 0 first).  *rgba is allocated here: release with edxr_free. */
 int edxr_load_bmp(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba);
+/* Truevision TGA, image types 2 (uncompressed) and 10 (run-length encoded),
+ * 24- or 32-bit BGR(A), either row order; output as edxr_load_bmp's. */
+int edxr_load_tga(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba);
+/* Binary PPM (P6, maxval 255); output as edxr_load_bmp's (alpha 255). */
+int edxr_load_ppm(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba);
+/* The diffuse-map reader of ImageTexture (Utils/Mesh.cpp:26-27): BMP, PPM or
+ * TGA, told apart by the file's first bytes ("BM", "P6", otherwise TGA). */
+int edxr_load_image(const char* path, uint32_t* width, uint32_t* height, uint8_t** rgba);
 void edxr_free(void* p);
 /* Left-handed look-at view matrix and D3D-style projection (z in [0,1]),
  * raster matrix x_r = (x+1)W/2, y_r = (1-y)H/2.  Computed in double. */

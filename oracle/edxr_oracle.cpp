@@ -240,6 +240,17 @@ static Vec3 Bilinear(const TextureSlot& s, int level, float u, float v) {
 }
 
 /* Sample(texCoord, differentials[2]) with the state's filter. */
+/* The trilinear blend of the two mip levels around lod (clamped to the chain). */
+static Vec3 TrilinearAt(const TextureSlot& s, float lod, float u, float v) {
+    const int nl = (int)s.levels.size();
+    lod = std::min(std::max(lod, 0.0f), (float)(nl - 1));
+    const float fl = std::floor(lod);
+    const int l0 = (int)fl, l1 = std::min(l0 + 1, nl - 1);
+    const float t = lod - fl, g = 1.0f - t;
+    const Vec3 a = Bilinear(s, l0, u, v), b = Bilinear(s, l1, u, v);
+    return Vec3{g * a.x + t * b.x, g * a.y + t * b.y, g * a.z + t * b.z};
+}
+
 static Vec3 SampleSlot(const TextureSlot& s, int filter, float u, float v, float dux, float dvx, float duy, float dvy) {
     if (s.constant) return Vec3{s.color[0], s.color[1], s.color[2]};
     const int w = s.lw[0], h = s.lh[0];
@@ -248,17 +259,34 @@ static Vec3 SampleSlot(const TextureSlot& s, int filter, float u, float v, float
         return Vec3{c[0], c[1], c[2]};
     }
     if (filter == ORACLE_FILTER_LINEAR) return Bilinear(s, 0, u, v);
-    /* trilinear: lod = log2 of the longer texel-space differential */
     const float ax = dux * (float)w, ay = dvx * (float)h, bx = duy * (float)w, by = dvy * (float)h;
-    const float rho2 = std::max(ax * ax + ay * ay, bx * bx + by * by);
-    float lod = rho2 > 0.0f ? 0.5f * std::log2(rho2) : 0.0f;
-    const int nl = (int)s.levels.size();
-    lod = std::min(std::max(lod, 0.0f), (float)(nl - 1));
-    const float fl = std::floor(lod);
-    const int l0 = (int)fl, l1 = std::min(l0 + 1, nl - 1);
-    const float t = lod - fl, g = 1.0f - t;
-    const Vec3 a = Bilinear(s, l0, u, v), b = Bilinear(s, l1, u, v);
-    return Vec3{g * a.x + t * b.x, g * a.y + t * b.y, g * a.z + t * b.z};
+    const float la2 = ax * ax + ay * ay, lb2 = bx * bx + by * by;
+    if (filter == ORACLE_FILTER_TRILINEAR) {
+        /* lod = log2 of the longer texel-space differential */
+        const float rho2 = std::max(la2, lb2);
+        return TrilinearAt(s, rho2 > 0.0f ? 0.5f * std::log2(rho2) : 0.0f, u, v);
+    }
+    /* N:1 anisotropic: N trilinear taps spread evenly along the longer
+     * differential (tap k at offset (2k + 1 - N) / 2N of it), at the lod of
+     * the shorter one raised to at least 1/N of the longer, averaged in tap
+     * order. */
+    const int N = 4 << (filter - ORACLE_FILTER_ANISO_4X);
+    const bool xMajor = la2 >= lb2;
+    const float mu = xMajor ? dux : duy, mv = xMajor ? dvx : dvy;
+    const float maj2 = xMajor ? la2 : lb2, n2 = (float)(N * N);
+    float min2 = xMajor ? lb2 : la2;
+    if (min2 * n2 < maj2) min2 = maj2 / n2;
+    const float lod = min2 > 0.0f ? 0.5f * std::log2(min2) : 0.0f;
+    Vec3 acc{0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < N; k++) {
+        const float sk = (float)(2 * k + 1 - N) / (float)(2 * N);
+        const Vec3 c = TrilinearAt(s, lod, u + sk * mu, v + sk * mv);
+        acc.x += c.x;
+        acc.y += c.y;
+        acc.z += c.z;
+    }
+    const float inv = 1.0f / (float)N;
+    return Vec3{acc.x * inv, acc.y * inv, acc.z * inv};
 }
 
 /* 28.4 snapping: RasterTriangle.h:35-40 converts x * 16.0 to int by

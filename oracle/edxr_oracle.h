@@ -26,7 +26,14 @@ enum {
     ORACLE_SHADER_TEXTURED_LAMBERT = 3 /* LambertianAlbedoPixelShader, texture slots */
 };
 
-enum { ORACLE_FILTER_NEAREST = 0, ORACLE_FILTER_LINEAR = 1, ORACLE_FILTER_TRILINEAR = 2 };
+enum {
+    ORACLE_FILTER_NEAREST = 0,
+    ORACLE_FILTER_LINEAR = 1,
+    ORACLE_FILTER_TRILINEAR = 2,
+    ORACLE_FILTER_ANISO_4X = 3, /* N:1 anisotropic, N = 4, 8, 16 */
+    ORACLE_FILTER_ANISO_8X = 4,
+    ORACLE_FILTER_ANISO_16X = 5
+};
 
 /* A texture slot (Texture2D<Color>, EDXUtil, absent; the declared semantics
  * of the library header, restated independently here): RGBA8 rows from v = 0,

@@ -16,6 +16,7 @@ U32 = C.POINTER(C.c_uint32)
 
 SHADER_LAMBERT_ALBEDO, SHADER_BLINN_PHONG_REF, SHADER_VC_BLINN_PHONG, SHADER_TEXTURED_LAMBERT = 0, 1, 2, 3
 FILTER_NEAREST, FILTER_LINEAR, FILTER_TRILINEAR = 0, 1, 2
+FILTER_ANISO_4X, FILTER_ANISO_8X, FILTER_ANISO_16X = 3, 4, 5
 
 
 class TextureC(C.Structure):

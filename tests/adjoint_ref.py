@@ -86,7 +86,7 @@ def image_fp64(theta, indices, view, ids, shininess_int=None):
     h = Lh[None, :] + vh
     hh = h / h.norm(dim=1, keepdim=True)
     sa = (nh * hh).sum(1)
-    k = int(round(float(s))) if shininess_int is None else shininess_int
+    k = int(round(float(s.detach()))) if shininess_int is None else shininess_int
     # sa^s with the integer exponent of the forward pass; its s-derivative is
     # sa^s ln(sa) for sa > 0 and 0 otherwise (first-order term at s = k).
     logsa = torch.where(sa > 0, torch.log(torch.clamp(sa, min=1e-300)), torch.zeros_like(sa))

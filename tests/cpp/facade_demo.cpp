@@ -2,8 +2,10 @@
 // viewer's OnInit / OnRender (RealtimeViewer/Main.cpp:32-75) -- Initialize,
 // LoadSphere or LoadMesh, SetTransform, RenderMesh, GetBackBuffer -- without
 // the window.  Writes the back buffer (RGBA8, bottom-up rows) to argv[1].
-// With an OBJ path as argv[2] it renders that mesh (320 x 240) with its
-// texture slots and the textured Lambertian shader.
+// Without an OBJ path it renders the seeded 64x32 colour sphere with the
+// vertex-colour Blinn-Phong shader (an extension); with an OBJ path as
+// argv[2] it renders that mesh (320 x 240) with its texture slots and the
+// reference's textured Lambertian shader.
 #include <cstdio>
 #include <vector>
 
@@ -26,14 +28,13 @@ int main(int argc, char** argv) {
         const float center[3] = {0.0f, 0.0f, 0.0f};
         const float up[3] = {0.0f, 1.0f, 0.0f};
         if (obj) {
-            const float pos[3] = {0.0f, 0.0f, 0.0f}, scl[3] = {1.0f, 1.0f, 1.0f}, rot[3] = {0.0f, 0.0f, 0.0f};
-            mesh.LoadMesh(pos, scl, rot, argv[2]);
+            mesh.LoadMesh(Vector3::ZERO, Vector3::UNIT_SCALE, Vector3::ZERO, argv[2]);
             const float eye[3] = {0.0f, 1.2f, -2.2f};
             renderer.SetTransform(Matrix::LookAt(eye, center, up),
                                   Matrix::Perspective(60.0f, (float)W / H, 0.1f, 100.0f), Matrix::ToRaster(W, H));
             renderer.SetPixelShader(TexturedLambertianPixelShader());
         } else {
-            mesh.LoadSphere(center, 1.0f, 64, 32, 1);
+            mesh.LoadColoredSphere(center, 1.0f, 64, 32, 1);
             const float eye[3] = {0.0f, 0.0f, -3.0f};
             renderer.SetTransform(Matrix::LookAt(eye, center, up),
                                   Matrix::Perspective(60.0f, (float)W / H, 0.1f, 100.0f), Matrix::ToRaster(W, H));

@@ -108,10 +108,15 @@ def rel_err(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n_views,res,hf_n,seg", [(2, 96, 12, (12, 8)), (4, 256, 71, (80, 64))],
-                         ids=["tiny", "c5_scene_4views"])
-def test_gpu_adjoint_matches_fp64_reference(built, n_views, res, hf_n, seg):
+@pytest.mark.parametrize("n_views,res,hf_n,seg,stride", [(2, 96, 12, (12, 8), 1), (4, 256, 71, (80, 64), 1),
+                                                         (64, 512, 71, (80, 64), 8)],
+                         ids=["tiny", "c5_scene_4views_256", "c5_scene_8of64views_512"])
+def test_gpu_adjoint_matches_fp64_reference(built, n_views, res, hf_n, seg, stride):
+    """GPU gradients against the FP64 reference, within 1e-3 relative L2 error
+    per parameter group; the last case is the differentiable workload's own
+    scene and resolution (every 8th of its 64 views at 512 x 512)."""
     mesh, views = configs.diff_scene(n_views, res, hf_n=hf_n, sphere_segments=seg)
+    views = views[::stride]
     g = upstream(views)
     gg, ids, fimg = gpu_grads(mesh, views, g)
     # visibility of the GPU forward equals the oracle's

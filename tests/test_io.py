@@ -101,7 +101,7 @@ f 1 2 4
 
 
 def test_obj_round_trip_of_a_sphere(built, tmp_path):
-    s = Mesh.load_sphere(24, 12, 1.5, (0.25, -0.5, 2.0), color_seed=5)
+    s = Mesh.sphere(24, 12, 1.5, (0.25, -0.5, 2.0), color_seed=5)
     lines = ["v %.9g %.9g %.9g" % tuple(p) for p in s.positions()]
     lines += ["vn %.9g %.9g %.9g" % (a, b, c) for a, b, c in zip(s.nx, s.ny, s.nz)]
     lines += ["f %d//%d %d//%d %d//%d" % (a + 1, a + 1, b + 1, b + 1, c + 1, c + 1) for a, b, c in s.indices.reshape(-1, 3)]
@@ -237,15 +237,17 @@ def test_bmp_reader_round_trip_and_top_down(built, tmp_path):
         load_bmp(tmp_path / "bad.bmp")
 
 
-def write_textured_obj(tmp_path):
+def write_textured_obj(tmp_path, place=None):
     """An OBJ heightfield with texture coordinates and a .bmp diffuse map
-    (ImageTexture) beside a Kd-only material (ConstantTexture2D)."""
+    (ImageTexture) beside a Kd-only material (ConstantTexture2D); `place`
+    maps the (V, 3) positions before they are written."""
     (tmp_path / "tex").mkdir()
     img = configs.checker_texture(128, 8, 3)
     assert L.lib().edxr_write_bmp(os.fsencode(tmp_path / "tex" / "c.bmp"), img.ctypes.data, 128, 128) == 0
     write(tmp_path / "m.mtl", "newmtl a\nKd 0.4 0.4 0.4\nmap_Kd tex/c.bmp\nnewmtl b\nKd 0.9 0.5 0.2\n")
     g = Mesh.load_heightfield(48, 0.3, 7, 9)
-    lines = ["mtllib m.mtl"] + ["v %.9g %.9g %.9g" % tuple(p) for p in g.positions()]
+    P = g.positions() if place is None else place(g.positions().astype(np.float64))
+    lines = ["mtllib m.mtl"] + ["v %.9g %.9g %.9g" % tuple(p) for p in P]
     lines += ["vt %.9g %.9g" % ((x + 1) * 2, (z + 1) * 2) for x, z in zip(g.px, g.pz)]
     for k, (a, b, c) in enumerate(g.indices.reshape(-1, 3)):
         if k % 200 == 0:
@@ -278,3 +280,82 @@ def test_obj_with_bmp_diffuse_map_renders_like_the_oracle(built, tmp_path):
     info = assert_frame_parity(r, o, "obj textured")
     assert info["covered"] > 1000
     r.close()
+
+
+def tga_bytes(img, rle=False, top_down=False, bits=32):
+    """A TGA file of an (H, W, 4) RGBA image whose rows are bottom-up (v = 0
+    first), written independently of the library: type 2 or type 10 (with
+    runs of repeated pixels where the image repeats), BGR(A) bytes."""
+    H, W, _ = img.shape
+    px = bits // 8
+    rows = img[::-1] if top_down else img
+    pix = np.concatenate([rows[..., 2::-1], rows[..., 3:4]], axis=-1)[..., :px].reshape(-1, px)
+    hdr = bytes([0, 0, 10 if rle else 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, W & 255, W >> 8, H & 255, H >> 8, bits,
+                 (0x20 if top_down else 0) | (8 if bits == 32 else 0)])
+    if not rle:
+        return hdr + pix.tobytes()
+    out = bytearray(hdr)
+    i, n = 0, pix.shape[0]
+    while i < n:
+        j = i + 1
+        while j < n and j - i < 128 and np.array_equal(pix[j], pix[i]):
+            j += 1
+        if j - i >= 2:  # a run
+            out += bytes([0x80 | (j - i - 1)]) + pix[i].tobytes()
+            i = j
+        else:           # literals up to the next repeat
+            j = i + 1
+            while j < n and j - i < 128 and not (j + 1 < n and np.array_equal(pix[j], pix[j + 1])):
+                j += 1
+            out += bytes([j - i - 1]) + pix[i:j].tobytes()
+            i = j
+    return bytes(out)
+
+
+@pytest.mark.parametrize("rle", [False, True], ids=["uncompressed", "rle"])
+@pytest.mark.parametrize("top_down", [False, True], ids=["bottom_up", "top_down"])
+@pytest.mark.parametrize("bits", [24, 32])
+def test_tga_reader_round_trip(built, tmp_path, rle, top_down, bits):
+    from edxraster_amd import load_image
+    rng = np.random.default_rng(bits + 2 * rle + top_down)
+    img = rng.integers(0, 256, (13, 21, 4), dtype=np.uint8)
+    img[3:7, :, :] = img[3, 0]  # rows of one colour: runs for the RLE writer
+    if bits == 24:
+        img[..., 3] = 255
+    p = tmp_path / "t.tga"
+    p.write_bytes(tga_bytes(img, rle, top_down, bits))
+    np.testing.assert_array_equal(load_image(p), img)
+
+
+def test_ppm_reader_round_trip_and_errors(built, tmp_path):
+    from edxraster_amd import load_image
+    rng = np.random.default_rng(7)
+    img = rng.integers(0, 256, (9, 14, 4), dtype=np.uint8)
+    img[..., 3] = 255
+    p = tmp_path / "t.ppm"
+    # rows top-down in the file, a comment in the header
+    p.write_bytes(b"P6\n# diffuse map\n14 9\n255\n" + img[::-1, :, :3].tobytes())
+    np.testing.assert_array_equal(load_image(p), img)
+    bad = tmp_path / "bad.ppm"
+    bad.write_bytes(b"P6\n14 9\n65535\n" + bytes(9 * 14 * 6))
+    with pytest.raises(L.EdxrError):
+        load_image(bad)
+    short = tmp_path / "short.tga"
+    short.write_bytes(tga_bytes(img)[:-5])
+    with pytest.raises(L.EdxrError):
+        load_image(short)
+
+
+def test_obj_with_tga_and_ppm_diffuse_maps_binds_images(built, tmp_path):
+    """Mesh::LoadMesh's ImageTexture for .tga and .ppm diffuse maps: the slots
+    hold the decoded images, not the Kd fallback."""
+    img = configs.checker_texture(16, 4, 5)
+    (tmp_path / "a.tga").write_bytes(tga_bytes(img, rle=True))
+    (tmp_path / "b.ppm").write_bytes(b"P6\n16 16\n255\n" + img[::-1, :, :3].tobytes())
+    write(tmp_path / "m.mtl", "newmtl a\nKd 0.1 0.2 0.3\nmap_Kd a.tga\nnewmtl b\nKd 0.4 0.5 0.6\nmap_Kd b.ppm\n")
+    write(tmp_path / "q.obj", "mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl a\nf 1 2 3\nusemtl b\nf 1 3 2\n")
+    slots = Mesh.obj_texture_slots(str(tmp_path / "q.obj"))
+    assert len(slots) == 3
+    np.testing.assert_array_equal(slots[0], img)
+    np.testing.assert_array_equal(slots[1][..., :3], img[..., :3])
+    assert (slots[1][..., 3] == 255).all()

@@ -60,7 +60,7 @@ def scenes():
     mv, p, r = configs.camera(w, h, (0.0, 0.0, -4.0), (0.0, 0.0, 0.0))
     for seed in (1, 2, 3):
         out.append(("random%d" % seed, rand_scene(seed), w, h, mv, p, r))
-    sph = Mesh.load_sphere(16, 8, 1.0, (0, 0, 0), color_seed=9)
+    sph = Mesh.sphere(16, 8, 1.0, (0, 0, 0), color_seed=9)
     mv, p, r = configs.camera(w, h, (0.0, 0.4, -3.0), (0.0, 0.0, 0.0))
     out.append(("sphere16x8", sph, w, h, mv, p, r))
     hf = Mesh.load_heightfield(10, 0.2, height_seed=4, color_seed=5)
@@ -424,7 +424,7 @@ def test_binning_invariants(built, name, mesh, w, h, mv, p, r):
 
 # ---------------------------------------------------------------- scenes
 def test_scene_sizes_and_winding(built):
-    s = Mesh.load_sphere(64, 32, 1.0, (0, 0, 0), color_seed=1)
+    s = Mesh.sphere(64, 32, 1.0, (0, 0, 0), color_seed=1)
     assert (s.num_vertices, s.num_triangles) == (2145, 4096)
     hf = Mesh.load_heightfield(11, 0.2, 3, 33)
     assert (hf.num_vertices, hf.num_triangles) == (121, 200)

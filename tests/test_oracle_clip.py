@@ -167,7 +167,7 @@ def test_fp64_clip_known_answers():
 
 # ---------------------------------------------------------------- shaders
 def shader_scene(w=160, h=120):
-    mesh = Mesh.load_sphere(48, 24, 1.0, (0.0, 0.0, 0.0), color_seed=3)
+    mesh = Mesh.sphere(48, 24, 1.0, (0.0, 0.0, 0.0), color_seed=3)
     mv, p, r = configs.camera(w, h, (0.4, 0.8, -2.6), (0.0, 0.0, 0.0))
     return configs.Workload("sphere_48x24_%dx%d" % (w, h), w, h, mesh, mv, p, r)
 

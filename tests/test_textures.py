@@ -110,6 +110,105 @@ def test_trilinear_minified_by_four_is_level_two(built):
     assert np.abs(lin - want).max() > 8
 
 
+def band_ramp_texture():
+    """128 x 64 RGBA8: red in bands of 4 rows (255 on rows with (j // 4) even,
+    0 otherwise), green a ramp of 2 per texel column (2 i), blue 128."""
+    tex = np.zeros((64, 128, 4), np.uint8)
+    tex[:, :, 0] = np.where((np.arange(64) // 4) % 2 == 0, 255, 0)[:, None]
+    tex[:, :, 1] = (2 * np.arange(128))[None, :]
+    tex[:, :, 2] = 128
+    tex[:, :, 3] = 255
+    return tex
+
+
+def band_profile(level, v):
+    """FP64 red of the band texture at mip level `level`, bilinear in v: the
+    level's rows are 2x2 box means (pairs of rows for this u-constant red),
+    texel centres at (j + 0.5) / h, rows wrap."""
+    rows = np.where((np.arange(64) // 4) % 2 == 0, 1.0, 0.0)
+    for _ in range(level):
+        rows = 0.5 * (rows[0::2] + rows[1::2])
+    h = rows.shape[0]
+    y = v * h - 0.5
+    j0 = np.floor(y).astype(np.int64)
+    f = y - j0
+    return (1.0 - f) * rows[j0 % h] + f * rows[(j0 + 1) % h]
+
+
+def aniso_rect(w, h, x0, y0, x1, y1, u0, du, dv):
+    """Screen-aligned rectangle with u = u0 + du * (x - x0) and v = dv * (y - y0)
+    in texture coordinates per pixel (affine: identity camera, w = 1)."""
+    tris = [[(x0, y0), (x1, y0), (x0, y1)], [(x1, y0), (x1, y1), (x0, y1)]]
+    m = mesh_from_pixels(tris, w, h)
+    pts = np.asarray(tris, np.float64).reshape(-1, 2)
+    m.tu = (u0 + (pts[:, 0] - x0) * du).astype(F)
+    m.tv = ((pts[:, 1] - y0) * dv).astype(F)
+    m.tex_ids = np.zeros(2, np.uint32)
+    return m
+
+
+def render_lit(mesh, w, h, textures, tex_filter):
+    """Oracle render with the light along the normal (diffuse = 1.2 * 3 / pi)."""
+    from oracle.oracle import Oracle, SHADER_TEXTURED_LAMBERT
+    o = Oracle()
+    identity_render(o, mesh, w, h, shader=SHADER_TEXTURED_LAMBERT, light=(0.0, 0.0, -1.0), textures=textures,
+                    tex_filter=tex_filter, shininess=0.0)
+    return o.color()[::-1, :, :3].astype(np.int32)
+
+
+LIT = 1.2 * 3.0 / np.pi
+
+
+@pytest.mark.parametrize("tex_filter", [3, 4, 5], ids=["aniso4", "aniso8", "aniso16"])
+def test_anisotropic_keeps_the_minor_axis_detail(built, tex_filter):
+    """Footprint 4 texels along u, 1/2 texel along v (ratio 8:1 at most).
+    Anisotropic filtering samples at the minor axis' lod 0 (raised to 4/N
+    texels, still <= 1): the red bands come out at level 0, where trilinear
+    (lod 2 from the major axis) blurs them; the N taps are centred on the
+    pixel's coordinate, so the green ramp reads its value there."""
+    w, h, x0, y0, x1, y1 = 40, 64, 4, 8, 28, 56
+    du, dv = 4.0 / 128, 0.5 / 64
+    m = aniso_rect(w, h, x0, y0, x1, y1, 16.0 / 128, du, dv)
+    tex = band_ramp_texture()
+    got = render_lit(m, w, h, [tex], tex_filter)
+    ys, xs = np.mgrid[y0:y1, x0:x1]
+    u = 16.0 / 128 + (xs + 0.5 - x0) * du
+    v = (ys + 0.5 - y0) * dv
+    red = band_profile(0, v)
+    green = 2.0 * (u * 128 - 0.5) / 255.0          # the ramp is linear: taps centred on u average to it
+    want_r = np.trunc(np.clip(LIT * red, 0, 1) * 255 + 0.5)
+    want_g = np.trunc(np.clip(LIT * green, 0, 1) * 255 + 0.5)
+    assert np.abs(got[y0:y1, x0:x1, 0] - want_r).max() <= 1
+    assert np.abs(got[y0:y1, x0:x1, 1] - want_g).max() <= 1
+    # trilinear blurs the bands (lod 2) ...
+    tri = render_lit(m, w, h, [tex], 2)
+    assert np.abs(tri[y0:y1, x0:x1, 0] - want_r).max() > 50
+    # ... and taps placed from the pixel's coordinate forward (k / N instead of
+    # (2k + 1 - N) / 2N of the differential) would read a ramp 1.5 texels on
+    N = 4 << (tex_filter - 3)
+    shifted = np.trunc(np.clip(LIT * 2.0 * ((u + (N - 1) / (2.0 * N) * du) * 128 - 0.5) / 255.0, 0, 1) * 255 + 0.5)
+    assert np.abs(shifted - want_g).max() >= 3
+
+
+@pytest.mark.parametrize("tex_filter,level", [(3, 2), (4, 1), (5, 0)], ids=["aniso4", "aniso8", "aniso16"])
+def test_anisotropic_ratio_clamp_sets_the_level(built, tex_filter, level):
+    """Footprint 16 texels along u, 1/2 along v (32:1): N:1 filtering raises
+    the minor axis to 16 / N texels, so 4x reads level 2, 8x level 1 and 16x
+    level 0 of the (u-constant) red bands."""
+    w, h, x0, y0, x1, y1 = 40, 64, 4, 8, 36, 56
+    du, dv = 16.0 / 128, 0.5 / 64
+    m = aniso_rect(w, h, x0, y0, x1, y1, 0.0, du, dv)
+    got = render_lit(m, w, h, [band_ramp_texture()], tex_filter)
+    ys, xs = np.mgrid[y0:y1, x0:x1]
+    v = (ys + 0.5 - y0) * dv
+    want = np.trunc(np.clip(LIT * band_profile(level, v), 0, 1) * 255 + 0.5)
+    assert np.abs(got[y0:y1, x0:x1, 0] - want).max() <= 1
+    others = [l for l in (0, 1, 2) if l != level]
+    for l in others:  # the neighbouring levels differ
+        alt = np.trunc(np.clip(LIT * band_profile(l, v), 0, 1) * 255 + 0.5)
+        assert np.abs(got[y0:y1, x0:x1, 0] - alt).max() > 20
+
+
 def test_oracle_rejects_bad_texture_slots(built):
     from oracle.oracle import Oracle
     m = textured_rect(0, 0, 8, 8, 8, 8, 1.0, 1.0, slot=2)
@@ -133,7 +232,7 @@ def render_gpu_and_oracle(w, msaa=0, hierarchical=True):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tex_filter", [0, 1, 2])
+@pytest.mark.parametrize("tex_filter", [0, 1, 2, 3, 4, 5])
 def test_gpu_textured_parity(built, tex_filter):
     from tests.test_gpu_parity import assert_frame_parity
     w = configs.textured_scene(tex_filter=tex_filter)
