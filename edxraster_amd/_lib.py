@@ -111,6 +111,7 @@ SIGNATURES = [
     ("edxr_default_material", None, [C.c_int32, C.POINTER(Material)]),
     ("edxr_set_state", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     ("edxr_render_mesh", C.c_int, [C.c_void_p, C.POINTER(Mesh)]),
+    ("edxr_render_mesh_noalloc", C.c_int, [C.c_void_p, C.POINTER(Mesh), C.c_void_p, C.c_void_p, C.c_void_p]),
     ("edxr_read_color", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("edxr_read_depth", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     ("edxr_read_triangle_ids", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

@@ -216,6 +216,7 @@ struct edxr_ctx {
     uint32_t numCUs = 256;
     uint32_t clipQueueCap = 0, extraRecCap = 0, extraVertCap = 0, refCap = 0;
     uint32_t debugCapacity = 0;     // EDXR_STATE_DEBUG_CAPACITY
+    uint8_t* scratch = nullptr;     // device address of this device's no-allocation scratch
     uint64_t renderGeneration = 0;  // advanced by every render_mesh / render_batch
     uint32_t* hostCounters = nullptr;
 
@@ -630,6 +631,20 @@ JobInput context_job(const edxr_ctx* c, const MeshPtrs& M, uint32_t V, uint32_t 
     return in;
 }
 
+// Sub-ranges of the no-allocation scratch, 256-byte aligned, taken in order;
+// `used` keeps counting past the end so the caller can report the need.
+struct ScratchCarver {
+    uint8_t* base;
+    size_t used = 0;
+    template <class T>
+    T* take(uint64_t n) {
+        const size_t at = (used + 255) & ~size_t(255);
+        used = at + sizeof(T) * std::max<uint64_t>(n, 1);
+        return used <= kScratchBytes ? reinterpret_cast<T*>(base + at) : nullptr;
+    }
+    bool fits() const { return used <= kScratchBytes; }
+};
+
 // Uploads the host-memory meshes of a batch into one device buffer and
 // rewrites their pointers (device meshes are used in place).
 int stage_batch_meshes(edxr_ctx* c, const edxr_job* jobs, uint32_t n, std::vector<JobInput>& in) {
@@ -713,6 +728,9 @@ int edxr_create(edxr_ctx** out, uint32_t width, uint32_t height) {
     edxr_default_material(c->shader, &c->mat);
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = init_kernels();
+    // The no-allocation scratch of this device (the current one): its
+    // address is looked up here, once per device, never in a render call.
+    if (e == hipSuccess) e = scratch_address(&c->scratch);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hostCounters, sizeof(uint32_t) * kCntSlots, hipHostMallocDefault);
     if (e == hipSuccess) e = c->counters.ensure(kCntSlots);
@@ -919,6 +937,140 @@ int edxr_render_mesh(edxr_ctx* c, const edxr_mesh* m) {
     if (c->writeFrames && (r = edxr_write_frame_to_file(c)) != EDXR_OK) return r;
     c->frameCount++;
     return EDXR_OK;
+}
+
+// The frame of edxr_render_mesh without allocating: job table, setup records,
+// bins, work list and key images are carved from the device's static scratch
+// (kScratchBytes), the outputs are the caller's.  Eager launches only (no
+// graph, no profiling events); a capacity overflow re-runs with larger
+// capacities while they fit the scratch.
+int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void* depth, void* ids) {
+    if (!c || !m) return EDXR_E_INVALID;
+    if (!color || !depth || !ids) return c->fail(EDXR_E_INVALID, "colour, depth and ID outputs are required");
+    if (!c->haveTransform) return c->fail(EDXR_E_STATE, "edxr_set_transform was not called");
+    if (m->memory != EDXR_MEM_DEVICE) return c->fail(EDXR_E_INVALID, "the allocation-free render needs a device mesh");
+    if (c->msaaLog2) return c->fail(EDXR_E_STATE, "the allocation-free render is single-sample (EDXR_STATE_MSAA 0)");
+    int r = check_mesh(c, m, c->shader);
+    if (r != EDXR_OK) return r;
+    if (!c->scratch) return c->fail(EDXR_E_HIP, "no scratch address for this device");
+    const uint32_t V = m->num_vertices, T = m->num_triangles;
+    if ((uint64_t)T + 65536 > kRecordMask) return c->fail(EDXR_E_LIMIT, "too many triangles");
+
+    const float* const f[11] = {m->px, m->py, m->pz, m->nx, m->ny, m->nz, m->cr, m->cg, m->cb, m->tu, m->tv};
+    const JobInput in = context_job(c, mesh_ptrs(f, m->indices, m->tex_ids), V, T);
+    // The single job of the frame, as build_plan places it (every base 0),
+    // with the chunk x tile matrix held to 1/16 of the scratch.
+    FrameParams P = job_params(in, c->tilesX, c->tilesY);
+    P.num_chunks = choose_chunks(c, T, P.num_tiles, true, kScratchBytes / 16);
+    P.chunk_slots = P.num_chunks ? (T + P.num_chunks - 1) / P.num_chunks : 0;
+    BatchParams Q;
+    memset(&Q, 0, sizeof(Q));
+    Q.num_jobs = 1;
+    Q.total_verts = V;
+    Q.total_slots = T;
+    Q.total_tiles = P.num_tiles;
+    Q.total_chunks = P.num_chunks;
+    Q.vert_blocks = (V + kJobTableBlock - 1) / kJobTableBlock;
+    Q.slot_blocks = (T + kJobTableBlock - 1) / kJobTableBlock;
+    Q.max_job_tiles = P.num_tiles;
+    Q.max_job_chunks = P.num_chunks;
+    Q.max_samples = 1;
+    Q.any_textured = P.shader == kTexturedLambert ? 1u : 0u;
+    const uint64_t countEntries = (uint64_t)P.num_chunks * P.num_tiles;
+    const uint64_t groupEntries = (uint64_t)((P.num_chunks + kChunkGroup - 1) / kChunkGroup) * P.num_tiles;
+
+    // Capacities of this call (never the context's, which may be larger).
+    uint32_t clipCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16)), recCap = 4096, vertCap = 8192;
+    uint32_t refCap = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * P.num_tiles + 1024, 0xFFFFFFF0ull);
+    // Few raster items: each holds an 8 KB key image in the scratch.
+    const uint32_t rasterBlocks = c->numCUs * 2;
+    hipError_t e;
+    c->stats = edxr_stats{};
+    for (uint32_t attempt = 0; attempt < 8; attempt++) {
+        if ((uint64_t)T + recCap > kRecordMask) return c->fail(EDXR_E_LIMIT, "record capacity exceeded");
+        const uint64_t workCap = (uint64_t)P.num_tiles + std::max(rasterBlocks, refCap / kSegRefs) + 2;
+        ScratchCarver s{c->scratch};
+        FrameParams* jobs = s.take<FrameParams>(1);
+        DeviceBuffers B;
+        memset(&B, 0, sizeof(B));
+        B.tile_totals = s.take<uint32_t>(P.num_tiles);
+        B.tile_start = s.take<uint32_t>((uint64_t)P.num_tiles + 1);
+        B.tile_item = s.take<uint32_t>(P.num_tiles);
+        B.tile_big = s.take<uint32_t>(P.num_tiles);
+        B.rrec = s.take<RasterRec>((uint64_t)T + recCap);
+        B.srec = s.take<ShadeRec>((uint64_t)T + recCap);
+        B.desc = s.take<uint32_t>(T);
+        B.slot_refs = s.take<uint16_t>(T);
+        B.chunk_box = s.take<uint32_t>(2ull * std::max<uint32_t>(P.num_chunks, 1));
+        B.group_sums = s.take<uint32_t>(groupEntries);
+        B.clip_queue = s.take<uint32_t>(clipCap);
+        B.bin_counts = s.take<uint32_t>(countEntries);
+        B.refs = s.take<uint32_t>(refCap);
+        B.work = s.take<uint32_t>(workCap);
+        B.partials = s.take<uint64_t>(workCap * kTileSize * kTileSize);
+        B.verts = s.take<ShadeVert>((uint64_t)V + vertCap);
+        B.sverts = s.take<ScreenVert>(V);
+        if (!s.fits()) {
+            char msg[160];
+            snprintf(msg, sizeof(msg),
+                     "the frame needs %.1f MiB of device scratch, more than the %llu MiB the allocation-free "
+                     "render has (use edxr_render_mesh)",
+                     s.used / 1048576.0, (unsigned long long)(kScratchBytes >> 20));
+            return c->fail(EDXR_E_LIMIT, msg);
+        }
+        B.clip_queue_cap = clipCap;
+        B.extra_rec_cap = recCap;
+        B.extra_vert_cap = vertCap;
+        B.ref_cap = refCap;
+        B.work_cap = (uint32_t)workCap;
+        B.raster_blocks = rasterBlocks;
+        B.raster_grid = c->numCUs * raster_resident_per_cu(false);
+        B.counters = c->counters.p;
+        B.color = static_cast<uint32_t*>(color);
+        B.depth = static_cast<float*>(depth);
+        B.ids = static_cast<uint32_t*>(ids);
+        B.color_f32 = nullptr;
+        // k_reset writes the job table from its argument: no upload.
+        if ((e = launch_frame(Q, jobs, &P, B, c->stream, nullptr)) != hipSuccess) return c->hip_fail(e, "kernel launch");
+        if ((e = hipMemcpyAsync(c->hostCounters, c->counters.p, sizeof(uint32_t) * kCntSlots, hipMemcpyDeviceToHost,
+                                c->stream)) != hipSuccess)
+            return c->hip_fail(e, "hipMemcpyAsync(counters)");
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
+        const uint32_t* h = c->hostCounters;
+        const uint32_t ov = h[kCntOverflow];
+        if (ov & kOvPolygon) return c->fail(EDXR_E_LIMIT, "clip polygon exceeded 9 vertices");
+        if (!ov) {
+            if (h[kCntBadIndex])
+                return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices or texture slot >= texture count");
+            uint64_t rejected = 0, mainRecords = 0;
+            for (uint32_t k = 0; k < kStripes; k++) {
+                rejected += h[kCntStriped + kStRejected * kStripes + k];
+                mainRecords += h[kCntStriped + kStMainRecords * kStripes + k];
+            }
+            c->stats.triangles_in = T;
+            c->stats.trivially_rejected = rejected;
+            c->stats.clipped = h[kCntClipped];
+            c->stats.near_clipped = h[kCntNearClipped];
+            c->stats.setup_records = mainRecords + h[kCntExtraRecords];
+            c->stats.refs = h[kCntTotalRefs];
+            c->stats.clip_vertices = h[kCntExtraVertices];
+            c->stats.retries = attempt;
+            c->stats.bin_chunks = P.num_chunks;
+            return EDXR_OK;
+        }
+        auto grow = [](uint32_t need) {
+            return (uint32_t)std::min<uint64_t>((uint64_t)need + need / 4 + 1024, 0xFFFFFFF0ull);
+        };
+        if (ov & kOvClipQueue) {
+            clipCap = grow(h[kCntClipQueue]);
+            recCap = std::max(recCap, grow(2 * recCap));
+            vertCap = std::max(vertCap, grow(2 * vertCap));
+        }
+        if (ov & kOvExtraRecords) recCap = grow(std::max(h[kCntExtraRecords], 2 * recCap));
+        if (ov & kOvExtraVertices) vertCap = grow(std::max(h[kCntExtraVertices], 2 * vertCap));
+        if (ov & kOvRefs) refCap = grow(std::max(h[kCntTotalRefs], refCap));
+    }
+    return c->fail(EDXR_E_LIMIT, "capacity retries exhausted");
 }
 
 // Texture slots with their mip chains (RGBA floats, b / 255 at level 0; each

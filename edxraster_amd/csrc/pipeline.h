@@ -292,6 +292,16 @@ hipError_t launch_adjoint(const BatchParams& Q, const FrameParams* jobs, const G
                           double* vacc, uint64_t vacc_size, hipStream_t s);
 // Kernel attributes (dynamic LDS limits) and occupancy; once per process before launches.
 hipError_t init_kernels();
+
+// Device scratch of the allocation-free frame (edxr_render_mesh_noalloc): a
+// file-scope __device__ array in pipeline.hip, one copy per device, held until
+// the process ends.  Every buffer of a frame other than its inputs, outputs
+// and the context's counters is carved from it per call.
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr int kMaxScratchDevices = 64;
+// The device address of the current device's copy, looked up once per device
+// (with that device current) and cached; *out is null on error.
+hipError_t scratch_address(uint8_t** out);
 // Workgroups of the persistent raster kernel that one CU holds at once.
 uint32_t raster_resident_per_cu(bool multisample);
 

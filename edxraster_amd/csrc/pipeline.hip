@@ -32,7 +32,14 @@
 // rounded division and sqrt, no FMA contraction (-ffp-contract=off).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "pipeline.h"
+
+// Scratch of the allocation-free frame (edxr_render_mesh_noalloc), one copy
+// per device.  Host code never uses this name as an address: the device
+// address comes from hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) uint8_t scratch[edxr::kScratchBytes];
 
 namespace edxr {
 
@@ -2176,6 +2183,28 @@ hipError_t init_kernels() {
 
 uint32_t raster_resident_per_cu(bool multisample) {
     return (uint32_t)std::max(1, g_rasterResident[multisample ? 1 : 0]);
+}
+
+namespace {
+std::mutex g_scratchLock;
+uint8_t* g_scratch[kMaxScratchDevices] = {};  // device address of each device's scratch copy
+}  // namespace
+
+hipError_t scratch_address(uint8_t** out) {
+    *out = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchLock);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = static_cast<uint8_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FrameParams* job0, const DeviceBuffers& B,

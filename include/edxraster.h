@@ -238,6 +238,24 @@ int edxr_set_state(edxr_ctx* ctx, int32_t key, int32_t value);
 /* Renderer::RenderMesh (Core/Renderer.cpp:100-118).  Synchronous: returns
  * after the frame is complete on the device. */
 int edxr_render_mesh(edxr_ctx* ctx, const edxr_mesh* mesh);
+/* No counterpart in the reference: edxr_render_mesh for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Renders a device-resident mesh
+ * (memory = EDXR_MEM_DEVICE) with the context's state into the caller's device
+ * buffers: color (W*H RGBA8 as uint32), depth (W*H float), ids (W*H uint32),
+ * in edxr_read_color's layout, identical to what edxr_render_mesh produces.
+ * Every intermediate buffer comes from one static device scratch of 64 MiB
+ * per device, held until the process ends; a frame that needs more (about
+ * 200k triangles at 1920x1080, fewer at higher resolutions) fails with
+ * EDXR_E_LIMIT before anything is launched.  Single-sample only
+ * (EDXR_STATE_MSAA 0: else EDXR_E_STATE), eager launches (EDXR_STATE_GRAPH
+ * and EDXR_STATE_PROFILE do not apply), no float colour output.  It calls no
+ * allocator; edxr_create has already set up the scratch's address.  All calls
+ * on one device share the scratch: calls that overlap on one device (two
+ * contexts, two threads) race.  Synchronous.  Fills edxr_get_stats; what
+ * edxr_read_*, edxr_backward and edxr_debug_* return is unchanged (they refer
+ * to the last edxr_render_mesh / edxr_render_batch). */
+int edxr_render_mesh_noalloc(edxr_ctx* ctx, const edxr_mesh* mesh, void* color, void* depth, void* ids);
 /* Renderer::GetBackBuffer (Core/Renderer.cpp:360-363), copied to the caller:
  * RGBA8, W*H*4 bytes, bottom-up rows; background 0, alpha 255 where drawn. */
 int edxr_read_color(edxr_ctx* ctx, uint8_t* rgba, size_t bytes);
