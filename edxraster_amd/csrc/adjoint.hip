@@ -2,8 +2,8 @@
 // interpolation with visibility held fixed (no counterpart in the reference).
 //
 // For every covered pixel the winning input triangle comes from the forward
-// pass's triangle-ID buffer; its vertices are re-read from the shading-vertex
-// buffer and re-projected with the job's MVP.  The pixel's perspective-correct
+// pass's triangle-ID buffer; its vertices are re-read from the job's mesh and
+// re-projected with the job's MVP.  The pixel's perspective-correct
 // barycentrics are recomputed in continuous (unsnapped) clip space:
 //   A_i = X_i - x W_i,  B_i = Y_i - y W_i,  q = A x B,  b = q / (q0 + q1 + q2)
 // for the pixel centre (x, y) in NDC (the weights that put the clip-space
@@ -91,20 +91,21 @@ __device__ __forceinline__ uint32_t job_of(const BatchParams& Q, const uint32_t*
 // normal, colour of vertex k) and the scalars gs[0..6] (light direction after
 // normalisation, ambient, kd, ks, shininess).  Returns false for a pixel that
 // contributes nothing.
-__device__ __forceinline__ bool pixel_adjoint(const FrameParams& P, const ShadeVert* __restrict__ verts,
+__device__ __forceinline__ bool pixel_adjoint(const FrameParams& P,
                                               const uint32_t vi[3], int x, int y, const float* gpix,
                                               double gv[3][9], double gs[7]) {
     double X[3], Y[3], W[3];
     V3 p[3], n[3], c[3];
-    const double* M = nullptr;
-    double mvp[16];
-    for (int i = 0; i < 16; i++) mvp[i] = (double)P.mvp[i];
-    M = mvp;
+    double M[16];
+    for (int i = 0; i < 16; i++) M[i] = (double)P.mvp[i];
+    // the input vertices, read from the mesh (the forward stores shading
+    // vertices only for vertices inside the frustum)
+    const MeshPtrs& Mh = P.mesh;
     for (int k = 0; k < 3; k++) {
-        const ShadeVert sv = verts[P.vert_base + vi[k]];
-        p[k] = v3(sv.px, sv.py, sv.pz);
-        n[k] = v3(sv.nx, sv.ny, sv.nz);
-        c[k] = v3(sv.cr, sv.cg, sv.cb);
+        const uint32_t v = vi[k];
+        p[k] = v3(Mh.px[v], Mh.py[v], Mh.pz[v]);
+        n[k] = v3(Mh.nx[v], Mh.ny[v], Mh.nz[v]);
+        c[k] = Mh.cr ? v3(Mh.cr[v], Mh.cg[v], Mh.cb[v]) : v3(0.0, 0.0, 0.0);
         X[k] = ((M[0] * p[k].x + M[1] * p[k].y) + M[2] * p[k].z) + M[3];
         Y[k] = ((M[4] * p[k].x + M[5] * p[k].y) + M[6] * p[k].z) + M[7];
         W[k] = ((M[12] * p[k].x + M[13] * p[k].y) + M[14] * p[k].z) + M[15];
@@ -230,7 +231,6 @@ __global__ void k_adjoint_zero(double* acc, uint64_t n) {
 // One workgroup per 32x32 tile, each wave over four of its 8x8 blocks (lane = pixel).
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                        const GradPtrs* __restrict__ grads,
-                                                       const ShadeVert* __restrict__ verts,
                                                        const uint32_t* __restrict__ ids,
                                                        const float* __restrict__ d_rgb, double* acc, double* vacc) {
     __shared__ double wsum[kAdjBlock / kWave][8];
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(BatchParams Q, const Fram
         const uint32_t* idx = P.mesh.idx + 3 * (size_t)id;
         const uint32_t vi[3] = {idx[0], idx[1], idx[2]};
         double gv[3][9], gs[7];
-        if (!pixel_adjoint(P, verts, vi, x, y, d_rgb + 3 * o, gv, gs)) continue;
+        if (!pixel_adjoint(P, vi, x, y, d_rgb + 3 * o, gv, gs)) continue;
         for (int j = 0; j < 7; j++) gsum[j] += gs[j];
         // Per-vertex sums: LDS atomics into the tile's vertex table; a vertex
         // the full table cannot take goes straight to the accumulators.
@@ -322,7 +322,7 @@ __global__ void k_adjoint_vertices(const GradPtrs* __restrict__ grads, const dou
 }
 
 hipError_t launch_adjoint(const BatchParams& Q, const FrameParams* jobs, const GradPtrs* grads,
-                          const ShadeVert* verts, const uint32_t* ids, const float* d_rgb, double* acc,
+                          const uint32_t* ids, const float* d_rgb, double* acc,
                           double* vacc, uint64_t vacc_size, hipStream_t s) {
     const uint32_t nacc = 8 * Q.num_jobs;
     hipLaunchKernelGGL(k_adjoint_zero, dim3((nacc + 255) / 256), dim3(256), 0, s, acc, (uint64_t)nacc);
@@ -330,7 +330,7 @@ hipError_t launch_adjoint(const BatchParams& Q, const FrameParams* jobs, const G
         hipLaunchKernelGGL(k_adjoint_zero, dim3((uint32_t)std::min<uint64_t>((vacc_size + 255) / 256, 4096)), dim3(256),
                            0, s, vacc, vacc_size);
     if (Q.total_tiles)
-        hipLaunchKernelGGL(k_adjoint, dim3(Q.total_tiles), dim3(kAdjBlock), 0, s, Q, jobs, grads, verts, ids, d_rgb,
+        hipLaunchKernelGGL(k_adjoint, dim3(Q.total_tiles), dim3(kAdjBlock), 0, s, Q, jobs, grads, ids, d_rgb,
                            acc, vacc);
     hipLaunchKernelGGL(k_adjoint_finish, dim3((Q.num_jobs + 63) / 64), dim3(64), 0, s, Q, jobs, grads, acc);
     hipLaunchKernelGGL(k_adjoint_vertices, dim3(Q.num_jobs), dim3(256), 0, s, grads, vacc);

@@ -1331,7 +1331,7 @@ int edxr_backward(edxr_ctx* c, const float* d_rgb, const edxr_grads* g, uint32_t
                             c->stream)) != hipSuccess)
         return c->hip_fail(e, "hipMemcpyAsync(grads)");
     const uint32_t* ids = c->rendered ? c->ids.p : c->bids.p;
-    if ((e = launch_adjoint(pl.Q, c->jobsDev.p, c->gradsDev.p, c->verts.p, ids, d_rgb, c->adjAcc.p,
+    if ((e = launch_adjoint(pl.Q, c->jobsDev.p, c->gradsDev.p, ids, d_rgb, c->adjAcc.p,
                             c->adjVertAcc.p, vaccSize, c->stream)) != hipSuccess)
         return c->hip_fail(e, "adjoint launch");
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");

@@ -288,7 +288,7 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
 // layout of color_f32; acc is 8 doubles of scratch per job, vacc the vertex
 // accumulators (vacc_size doubles, the GradPtrs' acc_base ranges).
 hipError_t launch_adjoint(const BatchParams& Q, const FrameParams* jobs, const GradPtrs* grads,
-                          const ShadeVert* verts, const uint32_t* ids, const float* d_rgb, double* acc,
+                          const uint32_t* ids, const float* d_rgb, double* acc,
                           double* vacc, uint64_t vacc_size, hipStream_t s);
 // Kernel attributes (dynamic LDS limits) and occupancy; once per process before launches.
 hipError_t init_kernels();

@@ -157,30 +157,38 @@ __global__ __launch_bounds__(kBlock) void k_vertex(BatchParams Q, const FramePar
     const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t v = slot - P.vert_base;
     if (v >= P.num_vertices) return;
-    ShadeVert o;
-    o.px = M.px[v];
-    o.py = M.py[v];
-    o.pz = M.pz[v];
-    o.nx = M.nx[v];
-    o.ny = M.ny[v];
-    o.nz = M.nz[v];
-    o.cr = M.cr ? M.cr[v] : 0.0f;
-    o.cg = M.cg ? M.cg[v] : 0.0f;
-    o.cb = M.cb ? M.cb[v] : 0.0f;
+    const float px = M.px[v], py = M.py[v], pz = M.pz[v];
     // DefaultVertexShader::Execute: projectedPos = MVP * (p, 1); the outcode,
     // the homogeneous projection and the snapped raster position are per
     // vertex, so every unclipped triangle's setup reuses them.
-    const float4 c = xform4(P.mvp, o.px, o.py, o.pz, 1.0f);
-    o.inv_w = 1.0f / c.w;
-    o.u = M.tu ? M.tu[v] : 0.0f;
-    o.v = M.tv ? M.tv[v] : 0.0f;
-    verts[slot] = o;
+    const float4 c = xform4(P.mvp, px, py, pz, 1.0f);
+    const uint32_t code = clip_code(c);
+    // Shading vertices are written for vertices inside the frustum only: an
+    // unclipped triangle has no other, and a clipped triangle's pieces use
+    // vertices made by k_clip (which reads its inputs from the mesh) or
+    // original vertices that lie inside every plane.
+    if (code == 0) {
+        ShadeVert o;
+        o.px = px;
+        o.py = py;
+        o.pz = pz;
+        o.nx = M.nx[v];
+        o.ny = M.ny[v];
+        o.nz = M.nz[v];
+        o.cr = M.cr ? M.cr[v] : 0.0f;
+        o.cg = M.cg ? M.cg[v] : 0.0f;
+        o.cb = M.cb ? M.cb[v] : 0.0f;
+        o.inv_w = 1.0f / c.w;
+        o.u = M.tu ? M.tu[v] : 0.0f;
+        o.v = M.tv ? M.tv[v] : 0.0f;
+        verts[slot] = o;
+    }
     ScreenVert sv;
     const int2 xy = snap_point(P, c.x / c.w, c.y / c.w, c.z / c.w);
     sv.x = xy.x;
     sv.y = xy.y;
     sv.z = c.z * (1.0f / c.w);
-    sv.code = clip_code(c);
+    sv.code = code;
     sverts[slot] = sv;
 }
 
@@ -385,6 +393,23 @@ __device__ __forceinline__ float4 shfl4(const float4& v, int src) {
     return make_float4(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), __shfl(v.w, src));
 }
 constexpr uint32_t kClipWaves = 16;  // waves (triangles) per workgroup pass
+// Input vertex v of a job's mesh with its shading attributes (the caller sets inv_w).
+__device__ __forceinline__ ShadeVert input_vertex(const MeshPtrs& M, uint32_t v) {
+    ShadeVert o;
+    o.px = M.px[v];
+    o.py = M.py[v];
+    o.pz = M.pz[v];
+    o.nx = M.nx[v];
+    o.ny = M.ny[v];
+    o.nz = M.nz[v];
+    o.cr = M.cr ? M.cr[v] : 0.0f;
+    o.cg = M.cg ? M.cg[v] : 0.0f;
+    o.cb = M.cb ? M.cb[v] : 0.0f;
+    o.inv_w = 0.0f;
+    o.u = M.tu ? M.tu[v] : 0.0f;
+    o.v = M.tv ? M.tv[v] : 0.0f;
+    return o;
+}
 }  // namespace
 
 __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const FrameParams* __restrict__ jobs,
@@ -400,6 +425,7 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
         const bool live = q < n;
         // ---- per wave: clip its triangle (no global writes yet) ----
         uint32_t t = 0, vi[3] = {0, 0, 0};
+        uint32_t outside = 0;  // bit k: input vertex k has a nonzero outcode
         ShadeVert pa, pb, pc;
         float4 pos = make_float4(0.0f, 0.0f, 0.0f, 1.0f), wgt = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         int cnt = 0;
@@ -413,13 +439,18 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
             vi[0] = P.vert_base + __builtin_amdgcn_readfirstlane(idx[0]);
             vi[1] = P.vert_base + __builtin_amdgcn_readfirstlane(idx[1]);
             vi[2] = P.vert_base + __builtin_amdgcn_readfirstlane(idx[2]);
-            pa = B.verts[vi[0]];
-            pb = B.verts[vi[1]];
-            pc = B.verts[vi[2]];
+            // from the mesh: k_vertex stores shading vertices inside the frustum only
+            pa = input_vertex(P.mesh, vi[0] - P.vert_base);
+            pb = input_vertex(P.mesh, vi[1] - P.vert_base);
+            pc = input_vertex(P.mesh, vi[2] - P.vert_base);
             const float4 c0 = xform4(P.mvp, pa.px, pa.py, pa.pz, 1.0f);
             const float4 c1 = xform4(P.mvp, pb.px, pb.py, pb.pz, 1.0f);
             const float4 c2 = xform4(P.mvp, pc.px, pc.py, pc.pz, 1.0f);
             const uint32_t k0 = clip_code(c0), k1 = clip_code(c1), k2 = clip_code(c2);
+            pa.inv_w = 1.0f / c0.w;
+            pb.inv_w = 1.0f / c1.w;
+            pc.inv_w = 1.0f / c2.w;
+            outside = (k0 ? 1u : 0u) | (k1 ? 2u : 0u) | (k2 ? 4u : 0u);
             const uint32_t planeCode = (k0 ^ k1) | (k1 ^ k2) | (k2 ^ k0);
             clipped++;
             if ((k0 | k1 | k2) & 16u) nearClipped++;
@@ -558,6 +589,11 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
                 o.u = comb(pa.u, pb.u, pc.u);
                 o.v = comb(pa.v, pb.v, pc.v);
                 B.verts[vid] = o;
+            } else if (lane < cnt && which >= 0 && ((outside >> which) & 1u)) {
+                // A parent vertex with a nonzero outcode that the polygon kept
+                // (an intersection weight that rounds to 1): k_vertex did not
+                // store it.  Every writer stores the same value.
+                B.verts[vid] = which == 0 ? pa : which == 1 ? pb : pc;
             }
             const uint32_t vid0 = __shfl(vid, 0), vid1 = __shfl(vid, fk - 1 < 64 ? fk - 1 : 0),
                            vid2 = __shfl(vid, fk < 64 ? fk : 0);
