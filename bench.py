@@ -104,14 +104,66 @@ def copy_bandwidth_gbs(nbytes=1 << 30, reps=10):
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
-# HBM bytes per launch of the raster/shade pass (k_raster + k_shade), from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this bench, scaled by a
-# float4 copy kernel of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); see
-# profiles/pmc_raster_shade_*.md.  None where not measured.
+# Streaming kernel of known size for the counter calibration: out = -in over
+# CALIBRATION_BYTES of float32 reads 2^28 bytes and writes 2^28 bytes.
+CALIBRATION_BYTES = 1 << 28
+CALIBRATION_KERNEL = "neg_kernel"
+
+
+def calibration_probe():
+    """One launch of the calibration kernel (profiled beside the frames)."""
+    import torch
+    a = torch.ones(CALIBRATION_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    del a, b
+
+
+# HBM bytes per launch of the raster/shade pass (k_raster + k_shade), keyed by
+# (workload, samples per pixel): FETCH_SIZE and WRITE_SIZE of the pass from two
+# rocprofv3 --pmc runs of this bench with --calibrate, each scaled by what the
+# same counter reads for the calibration kernel (bench.py --pmc-summary prints
+# the table; the commands and raw KiB are in profiles/pmc_raster_shade_traffic.md).
+# None where not measured.
 RASTER_SHADE_TRAFFIC = {
-    "sphere_grid_16x16x64_1920x1080": 61792 * 1024,   # (raster + shade FETCH) KiB x 2 + WRITE KiB, tuning 512,4
-    "heightfield_2236_3840x2160": 840512 * 1024,     # the same at tuning 4096,8
 }
+
+
+def pmc_summary(fetch_dir, write_dir):
+    """Per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of two
+    rocprofv3 --pmc runs, the calibration factors (known bytes / counter KiB
+    of the calibration kernel) and the raster/shade pass's HBM bytes."""
+    import csv
+    import glob
+
+    def means(d, counter):
+        acc = {}
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row["Kernel_Name"]
+                    key = ("k_raster<true>" if "k_rasterILb1" in name or "k_raster<true" in name else
+                           "k_raster<false>" if "k_raster" in name else
+                           "k_shade<true>" if "k_shadeILb1" in name or "k_shade<true" in name else
+                           "k_shade<false>" if "k_shade" in name else
+                           "calibration" if CALIBRATION_KERNEL in name else None)
+                    if key:
+                        acc.setdefault(key, []).append(float(row["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+    fetch, nf = means(fetch_dir, "FETCH_SIZE")
+    write, nw = means(write_dir, "WRITE_SIZE")
+    known_kib = CALIBRATION_BYTES / 1024.0
+    ff = known_kib / fetch["calibration"]
+    fw = known_kib / write["calibration"]
+    rows = [k for k in ("k_raster<false>", "k_raster<true>", "k_shade<false>", "k_shade<true>") if k in fetch]
+    traffic_kib = sum(fetch[k] * ff + write.get(k, 0.0) * fw for k in rows)
+    return {"fetch_kib": fetch, "write_kib": write, "dispatches": {"fetch": nf, "write": nw},
+            "fetch_factor": ff, "write_factor": fw, "kernels": rows,
+            "traffic_bytes": int(round(traffic_kib * 1024))}
 
 
 def frame_leg(w, steps, warmup, brief=False, tuning=None, msaa=0):
@@ -170,7 +222,7 @@ def frame_leg(w, steps, warmup, brief=False, tuning=None, msaa=0):
     raster_ms = stage_avg["raster"]
     achieved = b_rs / (raster_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": RASTER_SHADE_TRAFFIC.get(w.name) if S == 1 else None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": RASTER_SHADE_TRAFFIC.get((w.name, S)),
             "kernel": "k_raster+k_shade (raster/shade pass)", "algorithmic_bytes": b_rs, "kernel_ms": raster_ms}
     # Compulsory bytes per stage (achieved GB/s = bytes / stage time):
     # setup: 36 B in + 64 B out (shading and screen vertex) per vertex, 12 B
@@ -339,7 +391,14 @@ def main():
     ap.add_argument("--msaa", type=int, default=0, help="log2 samples per pixel of the timed frame (profiling runs)")
     ap.add_argument("--tuning", default=None,
                     help="TRIS_PER_CHUNK,RASTER_BLOCKS_PER_CU instead of autotuning (e.g. for profiling runs)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also launch the counter-calibration kernel once (for rocprofv3 --pmc runs)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print the raster/shade traffic from two rocprofv3 --pmc output folders and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary)), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: the multi-GPU path is not built",
               file=sys.stderr)
@@ -398,6 +457,8 @@ def main():
             out["textured_10m"] = frame_leg(ht, max(5, args.steps // 2), 2, brief=True)
         out["batched"] = batched_leg(max(3, args.steps // 4), 2)
         out["differentiable"] = diff_leg(max(3, args.steps // 4), 2)
+    if args.calibrate:
+        calibration_probe()
     # the achievable peak beside the published one: a 1 GiB device copy
     copy_gbs = copy_bandwidth_gbs()
     out["hbm_copy_gbs"] = copy_gbs

@@ -190,7 +190,6 @@ struct edxr_ctx {
         tileBig;
     DevBuf<float> depth;
     DevBuf<uint64_t> partials;
-    DevBuf<uint16_t> slotRefs;
     DevBuf<uint32_t> chunkBox;
     DevBuf<uint32_t> groupSums;
     DevBuf<ShadeVert> verts;  // shading vertices + extraVertCap clip-created ones
@@ -480,7 +479,6 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         if ((e = c->rrec.ensure((size_t)T + c->extraRecCap)) != hipSuccess ||
             (e = c->srec.ensure((size_t)T + c->extraRecCap)) != hipSuccess ||
             (e = c->desc.ensure(std::max<uint32_t>(T, 1))) != hipSuccess ||
-            (e = c->slotRefs.ensure(std::max<uint32_t>(T, 1))) != hipSuccess ||
             (e = c->chunkBox.ensure(2 * std::max<uint32_t>(Q.total_chunks, 1))) != hipSuccess ||
             (e = c->groupSums.ensure(std::max<uint64_t>(pl.groupEntries, 1))) != hipSuccess ||
             (e = c->clipQueue.ensure(c->clipQueueCap)) != hipSuccess ||
@@ -500,7 +498,6 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.rrec = c->rrec.p;
         B.srec = c->srec.p;
         B.desc = c->desc.p;
-        B.slot_refs = c->slotRefs.p;
         B.chunk_box = c->chunkBox.p;
         B.group_sums = c->groupSums.p;
         B.clip_queue = c->clipQueue.p;
@@ -777,7 +774,6 @@ void edxr_destroy(edxr_ctx* c) {
     c->adjAcc.release();
     c->adjVertAcc.release();
     c->partials.release();
-    c->slotRefs.release();
     c->chunkBox.release();
     c->verts.release();
     c->sverts.release();
@@ -1000,7 +996,6 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
         B.rrec = s.take<RasterRec>((uint64_t)T + recCap);
         B.srec = s.take<ShadeRec>((uint64_t)T + recCap);
         B.desc = s.take<uint32_t>(T);
-        B.slot_refs = s.take<uint16_t>(T);
         B.chunk_box = s.take<uint32_t>(2ull * std::max<uint32_t>(P.num_chunks, 1));
         B.group_sums = s.take<uint32_t>(groupEntries);
         B.clip_queue = s.take<uint32_t>(clipCap);
@@ -1534,7 +1529,7 @@ int64_t edxr_debug_read_bins(edxr_ctx* c, uint64_t* offsets, uint32_t* refs3, ui
     if (e != hipSuccess) return c->hip_fail(e, "debug_read_bins");
     if (offsets)
         for (uint32_t i = 0; i <= c->nTiles; i++) offsets[i] = ts[i];
-    if (refs3 && total <= cap)
+    if (refs3 && total <= cap) {
         for (uint64_t i = 0; i < total; i++) {
             const uint32_t p = refs[i], ri = p & kRecordMask;
             const uint32_t acc = (p >> kRecordBits) & 7u;
@@ -1544,6 +1539,17 @@ int64_t edxr_debug_read_bins(edxr_ctx* c, uint64_t* offsets, uint32_t* refs3, ui
             refs3[3 * i + 1] = compact ? 0u : (rr[ri].info >> 12) & 7u;
             refs3[3 * i + 2] = acc | ((acc == 7u) ? 8u : 0u) | (big ? 16u : 0u);
         }
+        // Submission order within each tile: (triangle, fan piece) ascending;
+        // a record is at most once in a tile's list.
+        struct Ref3 {
+            uint32_t w[3];
+        };
+        Ref3* r3 = reinterpret_cast<Ref3*>(refs3);
+        for (uint32_t t = 0; t < c->nTiles; t++)
+            std::sort(r3 + ts[t], r3 + ts[t + 1], [](const Ref3& a, const Ref3& b) {
+                return a.w[0] != b.w[0] ? a.w[0] < b.w[0] : a.w[1] < b.w[1];
+            });
+    }
     return (int64_t)total;
 }
 

@@ -32,8 +32,8 @@ constexpr uint32_t kHeavyBigMin = 32;
 // segment log2 << kImageBits.
 constexpr uint32_t kImageBits = 26;
 constexpr uint32_t kImageMask = (1u << kImageBits) - 1;
-constexpr uint32_t kSlotRefsSaturated = 0xFFFF;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
+constexpr uint32_t kTilescanStaged = 8192;   // batch tiles whose counts k_bin_tilescan stages in LDS
 constexpr uint32_t kJobAlign = 1024;         // job slot alignment: vertex/setup blocks never span jobs
 constexpr uint32_t kJobTableBlock = 256;     // slots (vertices) per entry of the block -> job tables
 constexpr uint32_t kMaxChunks = 4096;
@@ -234,7 +234,6 @@ struct DeviceBuffers {
     ShadeVert* verts;      // V + extra_vert_cap shading vertices
     ScreenVert* sverts;    // V screen vertices (input vertices only)
     uint32_t extra_vert_cap;
-    uint16_t* slot_refs;   // per input triangle: tile references (saturating)
     uint32_t* chunk_box;   // per chunk of the batch: tile box of its references,
                            // (min x | min y << 16, max x | max y << 16); empty: min > max
     uint32_t* bin_counts;  // num_chunks * num_tiles, scanned in place

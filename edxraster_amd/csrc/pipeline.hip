@@ -16,9 +16,11 @@
 //   k_bin_colsum / k_bin_groupscan / k_bin_colscan / k_bin_tilescan
 //                exclusive scans -> per-(tile, chunk) offsets and the raster
 //                work list of (tile, segment) items.
-//   k_bin_scatter order-preserving scatter: refs of a tile are in ascending
-//                (triangle, fan index) order, like the reference's per-core lists
-//                concatenated in core order (Core/Renderer.cpp:250-253).
+//   k_bin_scatter scatter through LDS cursors: a tile's list holds the chunks'
+//                references in chunk order (the reference concatenates its
+//                per-core lists in core order, Core/Renderer.cpp:250-253); the
+//                depth keys carry the submission order, so the list order
+//                within a chunk does not matter.
 //   k_raster     persistent: hierarchical edge-function coverage of one (tile,
 //                segment) item at a time (CoarseRasterize / FineRasterize /
 //                TrivialAcceptTriangle, Core/Rasterizer.h:32-200,310-353) and
@@ -672,65 +674,6 @@ __device__ __forceinline__ int bin_test(const BinTri& b, int tx, int ty) {
     return f;
 }
 
-// Enumerates the (tile, ref) pairs of one slot in submission order: records in
-// fan order, tiles of a record in row-major order.
-struct RefEnum {
-    uint32_t first, count, r;
-    int k;
-    bool loaded;
-    BinTri b;
-
-    // desc of slot s: either a compact small-triangle descriptor (the tile
-    // bbox; the record is slot s) or (first record << 3) | count.
-    __device__ __forceinline__ void init(uint32_t desc, uint32_t s) {
-        r = 0;
-        k = 0;
-        if (desc & kDescCompact) {
-            first = s;
-            count = 1;
-            loaded = true;
-            b.minTX = (int)(desc & 0x3FFFu);
-            b.minTY = (int)((desc >> 14) & 0x3FFFu);
-            b.w = 1 + (int)((desc >> 28) & 1u);
-            b.h = 1 + (int)((desc >> 29) & 1u);
-            b.small = true;
-        } else {
-            first = desc >> 3;
-            count = desc & 7u;
-            loaded = false;
-        }
-    }
-    // Returns false when exhausted.
-    __device__ __forceinline__ bool next(const FrameParams& P, const RasterRec* rrec, uint32_t& tile,
-                                         uint32_t& payload) {
-        while (r < count) {
-            if (!loaded) {
-                load_bin_tri(P, rrec[first + r], b);
-                loaded = true;
-                k = 0;
-            }
-            const int n = b.w * b.h;
-            while (k < n) {
-                const int tx = b.minTX + k % b.w, ty = b.minTY + k / b.w;
-                k++;
-                if (b.small) {
-                    tile = (uint32_t)ty * P.tiles_x + (uint32_t)tx;
-                    payload = first + r;
-                    return true;
-                }
-                const int f = bin_test(b, tx, ty);
-                if (f < 0) continue;
-                tile = (uint32_t)ty * P.tiles_x + (uint32_t)tx;
-                payload = (first + r) | ((uint32_t)f << kRecordBits) | (1u << 31);
-                return true;
-            }
-            r++;
-            loaded = false;
-        }
-        return false;
-    }
-};
-
 // Reference k (row-major over the bbox) of a compact descriptor: its tile.
 __device__ __forceinline__ uint32_t compact_tile(uint32_t desc, uint32_t k, uint32_t tilesX) {
     const uint32_t w = 1u + ((desc >> 28) & 1u);
@@ -740,87 +683,135 @@ __device__ __forceinline__ uint32_t compact_refs(uint32_t desc) {
     return (1u + ((desc >> 28) & 1u)) * (1u + ((desc >> 29) & 1u));
 }
 
-// Block-wide exclusive scan of one value per thread (256 threads).
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
-    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
-    uint32_t x = v;
+// A slot's references: its records, the tiles of a record in its tile bbox.
+// A compact descriptor is its record's tile bbox (at most 2x2 tiles, no corner
+// tests; the record is the slot).  Otherwise the descriptor is
+// (first record << 3) | count, and the binning kernels list it for a wave:
+// the wave's lanes test a record's bbox tiles 64 at a time, the next record
+// loaded while the current one is tested.  emit(tile, tx, ty, payload) runs on
+// the lane that found the reference; there is no order among them.
+// Whether any of a thread's descriptors of a step is non-compact.
+template <uint32_t kN>
+__device__ __forceinline__ int any_non_compact(const uint32_t (&d)[kN]) {
+    bool any = false;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d);
-        if (lane >= (uint32_t)d) x += y;
+    for (uint32_t u = 0; u < kN; u++) any |= d[u] != 0 && !(d[u] & kDescCompact);
+    return any ? 1 : 0;
+}
+
+template <class Emit>
+__device__ __forceinline__ void wave_enum_desc(const FrameParams& P, const RasterRec* rrec, uint32_t desc,
+                                               Emit&& emit) {
+    const uint32_t first = desc >> 3, count = desc & 7u;
+    const uint32_t lane = __lane_id();
+    RasterRec next = rrec[first];
+    for (uint32_t r = 0; r < count; r++) {
+        BinTri b;
+        load_bin_tri(P, next, b);
+        if (r + 1 < count) next = rrec[first + r + 1];
+        const uint32_t n = (uint32_t)(b.w * b.h);
+        if (n == 0) continue;
+        const uint32_t w = (uint32_t)b.w, rec = first + r;
+        // tile k = lane + 64 i at (k % w, k / w), stepped
+        uint32_t cx = lane % w, cy = lane / w;
+        const uint32_t dx = kWave % w, dy = kWave / w;
+        for (uint32_t k = lane; k < n; k += kWave) {
+            const int tx = b.minTX + (int)cx, ty = b.minTY + (int)cy;
+            const int f = b.small ? 0 : bin_test(b, tx, ty);
+            if (f >= 0)
+                emit((uint32_t)ty * P.tiles_x + (uint32_t)tx, tx, ty,
+                     b.small ? rec : rec | ((uint32_t)f << kRecordBits) | (1u << 31));
+            cx += dx;
+            cy += dy;
+            if (cx >= w) {
+                cx -= w;
+                cy++;
+            }
+        }
     }
-    if (lane == kWave - 1) wsum[wave] = x;
-    __syncthreads();
-    uint32_t base = 0;
-    total = 0;
-    for (uint32_t w = 0; w < kBlock / kWave; w++) {
-        if (w < wave) base += wsum[w];
-        total += wsum[w];
-    }
-    __syncthreads();
-    return base + x - v;
 }
 
 }  // namespace
 
 __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                       DeviceBuffers B) {
+    constexpr uint32_t kUnroll = 4;
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t sBox[4];  // min x, min y, max x, max y of the chunk's tiles
+    __shared__ uint32_t sBig[kUnroll * kBlock], sBigN;  // a step's non-compact descriptors
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;  // chunk of the job
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) hist[i] = 0;
     if (threadIdx.x == 0) {
         sBox[0] = sBox[1] = 0xFFFFu;
         sBox[2] = sBox[3] = 0u;
+        sBigN = 0;
     }
     __syncthreads();
     uint32_t bx0 = 0xFFFFu, by0 = 0xFFFFu, bx1 = 0u, by1 = 0u;  // this thread's references
     const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
     const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
     // Four slots per thread and step, their descriptors loaded together so
-    // the loads' latencies overlap.
-    constexpr uint32_t kUnroll = 4;
+    // the loads' latencies overlap.  Compact descriptors (at most 2x2 tiles)
+    // are counted by their lanes; the others are listed and shared out over
+    // the waves at the end of the step.
     const uint32_t tilesX = P.tiles_x;
-    for (uint32_t sb = s0 + threadIdx.x; sb < s1; sb += kUnroll * kBlock) {
+    // A tile's count of heavy references rides in the high half of its
+    // histogram entry while a chunk cannot reach 2^16 references per tile (at
+    // most 7 records per slot); one global atomic per tile then adds it to
+    // tile_big, instead of one per reference.
+    const bool packBig = (uint64_t)P.chunk_slots * 7u <= 0xFFFFu;
+    auto bump = [&](uint32_t tile, bool heavy) {
+        if (packBig) {
+            atomicAdd(&hist[tile], heavy ? 0x10001u : 1u);
+        } else {
+            atomicAdd(&hist[tile], 1u);
+            if (heavy) atomicAdd(&B.tile_big[P.tile_base + tile], 1u);
+        }
+    };
+    // The next step's descriptors are loaded before this step's work.
+    uint32_t dn[kUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kUnroll; u++) {
+        const uint32_t s = s0 + threadIdx.x + u * kBlock;
+        dn[u] = s < s1 ? B.desc[s] : 0u;
+    }
+    for (uint32_t sb = s0 + threadIdx.x; sb - threadIdx.x < s1; sb += kUnroll * kBlock) {
         uint32_t d[kUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; u++) d[u] = sb + u * kBlock < s1 ? B.desc[sb + u * kBlock] : 0u;
+        for (uint32_t u = 0; u < kUnroll; u++) {
+            d[u] = dn[u];
+            const uint32_t sn = sb + (kUnroll + u) * kBlock;
+            dn[u] = sn < s1 ? B.desc[sn] : 0u;
+        }
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; u++) {
-            const uint32_t s = sb + u * kBlock;
-            if (s >= s1) break;
-            uint32_t R = 0;
             if (d[u] & kDescCompact) {
                 // at most 2x2 tiles, no corner tests
-                R = compact_refs(d[u]);
-                for (uint32_t k = 0; k < R; k++) {
-                    const uint32_t tile = compact_tile(d[u], k, tilesX);
-                    atomicAdd(&hist[tile], 1u);
-                    if (d[u] & kDescHeavy) atomicAdd(&B.tile_big[P.tile_base + tile], 1u);
-                }
+                const uint32_t R = compact_refs(d[u]);
+                for (uint32_t k = 0; k < R; k++) bump(compact_tile(d[u], k, tilesX), (d[u] & kDescHeavy) != 0);
                 const uint32_t tx = d[u] & 0x3FFFu, ty = (d[u] >> 14) & 0x3FFFu;
                 bx0 = min(bx0, tx);
                 by0 = min(by0, ty);
                 bx1 = max(bx1, tx + ((d[u] >> 28) & 1u));
                 by1 = max(by1, ty + ((d[u] >> 29) & 1u));
-            } else {
-                RefEnum e;
-                e.init(d[u], s);
-                uint32_t tile, payload;
-                while (e.next(P, B.rrec, tile, payload)) {
-                    atomicAdd(&hist[tile], 1u);
-                    if (payload >> 31) atomicAdd(&B.tile_big[P.tile_base + tile], 1u);
-                    R++;
-                    const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
-                    bx0 = min(bx0, tx);
-                    by0 = min(by0, ty);
-                    bx1 = max(bx1, tx);
-                    by1 = max(by1, ty);
-                }
+            } else if (d[u]) {
+                sBig[atomicAdd(&sBigN, 1u)] = d[u];
             }
-            // Saturated counts are recomputed by the scatter.
-            B.slot_refs[s] = (uint16_t)min(R, (uint32_t)kSlotRefsSaturated);
+        }
+        if (__syncthreads_or(any_non_compact(d))) {
+            const uint32_t nBig = sBigN;
+            __syncthreads();  // every thread has read the count
+            if (threadIdx.x == 0) sBigN = 0;
+            for (uint32_t i = threadIdx.x / kWave; i < nBig; i += kBlock / kWave)
+                wave_enum_desc(P, B.rrec, sBig[i], [&](uint32_t tile, int tx, int ty, uint32_t payload) {
+                    bump(tile, (payload >> 31) != 0);
+                    bx0 = min(bx0, (uint32_t)tx);
+                    by0 = min(by0, (uint32_t)ty);
+                    bx1 = max(bx1, (uint32_t)tx);
+                    by1 = max(by1, (uint32_t)ty);
+                });
+            __syncthreads();  // the list is rewritten by the next step
         }
     }
     // The chunk's tile box (k_bin_scatter sets up cursors only inside it).
@@ -849,7 +840,9 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
         const uint32_t bw = x1 - x0 + 1, nb = bw * (y1 - y0 + 1);
         for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
             const uint32_t t = (y0 + i / bw) * tilesX + x0 + i % bw;
-            row[t] = hist[t];
+            const uint32_t h = hist[t];
+            row[t] = packBig ? h & 0xFFFFu : h;
+            if (packBig && (h >> 16)) atomicAdd(&B.tile_big[P.tile_base + t], h >> 16);
         }
     }
 }
@@ -1026,33 +1019,31 @@ __device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) 
 // largest first.  A segment's key image is at the tile's image base
 // (tile_item) plus its segment index, whatever its place in the list.
 __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuffers B) {
+    extern __shared__ uint32_t sTiles[];  // staged: [nt] totals, then [nt] big counts
     __shared__ uint32_t sScan6[6][16];
     __shared__ uint32_t sMax, sTotal, sSegs, sExtra;
     const uint32_t nt = Q.total_tiles;
     const uint32_t per = (nt + 1023) / 1024;
     const uint32_t t0 = min(nt, threadIdx.x * per), t1 = min(nt, t0 + per);
     if (threadIdx.x == 0) sMax = sTotal = sSegs = sExtra = 0;
+    // Up to kTilescanStaged tiles, the totals and big counts are staged in LDS
+    // with coalesced loads that are all in flight together; the passes below
+    // walk each thread's contiguous tiles several times.
+    const bool staged = nt <= kTilescanStaged;
+    if (staged) {
+        for (uint32_t t = threadIdx.x; t < nt; t += 1024) {
+            sTiles[t] = B.tile_totals[t];
+            sTiles[nt + t] = B.tile_big[t];
+        }
+    }
     __syncthreads();
-    // Up to 16 tiles per thread (16384 tiles) the totals are read once, all
-    // loads in flight together, and kept in registers for the later passes.
-    constexpr uint32_t kCache = 16;
-    const bool cached = per <= kCache;
-    uint32_t cache[kCache];
+    auto total_of = [&](uint32_t t) { return staged ? sTiles[t] : B.tile_totals[t]; };
+    auto big_of = [&](uint32_t t) { return staged ? sTiles[nt + t] : B.tile_big[t]; };
     uint32_t localRefs = 0, localMax = 0;
-    if (cached) {
-#pragma unroll
-        for (uint32_t k = 0; k < kCache; k++) cache[k] = t0 + k < t1 ? B.tile_totals[t0 + k] : 0u;
-#pragma unroll
-        for (uint32_t k = 0; k < kCache; k++) {
-            localRefs += cache[k];
-            localMax = max(localMax, cache[k]);
-        }
-    } else {
-        for (uint32_t t = t0; t < t1; t++) {
-            const uint32_t n = B.tile_totals[t];
-            localRefs += n;
-            localMax = max(localMax, n);
-        }
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t n = total_of(t);
+        localRefs += n;
+        localMax = max(localMax, n);
     }
     // Total and deepest tile: wave reductions, then one LDS atomic per wave.
     uint32_t waveRefs = localRefs, waveMax = localMax;
@@ -1081,11 +1072,6 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
         while (s < segLog2 && tile_segments(n, s) > (1u << kSegIndexBits)) s++;
         return s;
     };
-    uint32_t cacheBig[kCache];
-    if (cached) {
-#pragma unroll
-        for (uint32_t k = 0; k < kCache; k++) cacheBig[k] = t0 + k < t1 ? B.tile_big[t0 + k] : 0u;
-    }
     // The finer segments of heavy tiles are taken only while the work list
     // has room for them: segments with and without them, summed over the block.
     {
@@ -1095,12 +1081,7 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
             segs += plain;
             extra += tile_segments(n, tile_log2(n, big, true)) - plain;
         };
-        if (cached) {
-#pragma unroll
-            for (uint32_t k = 0; k < kCache; k++) add(cache[k], cacheBig[k]);
-        } else {
-            for (uint32_t t = t0; t < t1; t++) add(B.tile_totals[t], B.tile_big[t]);
-        }
+        for (uint32_t t = t0; t < t1; t++) add(total_of(t), big_of(t));
         for (int d = 1; d < 64; d <<= 1) {
             segs += __shfl_xor(segs, d);
             extra += __shfl_xor(extra, d);
@@ -1121,12 +1102,7 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
         const uint32_t r = n & ((1u << s) - 1);
         if (r) cnt[1 + (r >> (s - 2))]++;
     };
-    if (cached) {
-#pragma unroll
-        for (uint32_t k = 0; k < kCache; k++) count(cache[k], cacheBig[k]);
-    } else {
-        for (uint32_t t = t0; t < t1; t++) count(B.tile_totals[t], B.tile_big[t]);
-    }
+    for (uint32_t t = t0; t < t1; t++) count(total_of(t), big_of(t));
     // the tile-reference prefix rides along as a sixth value
     uint32_t pos[6] = {cnt[0], cnt[1], cnt[2], cnt[3], cnt[4], localRefs}, tot[6];
     scan1024_n<6>(pos, sScan6, tot);
@@ -1158,13 +1134,7 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
             seg += nf + (r ? 1u : 0u);
         }
     };
-    if (cached) {
-#pragma unroll
-        for (uint32_t k = 0; k < kCache; k++)
-            if (t0 + k < t1) place(t0 + k, cache[k], cacheBig[k]);
-    } else {
-        for (uint32_t t = t0; t < t1; t++) place(t, B.tile_totals[t], B.tile_big[t]);
-    }
+    for (uint32_t t = t0; t < t1; t++) place(t, total_of(t), big_of(t));
     if (threadIdx.x == 0) {
         B.tile_start[nt] = total;
         B.counters[kCntTotalRefs] = total;
@@ -1174,102 +1144,72 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     }
 }
 
-// Order-preserving scatter.  Each block walks its chunk in rounds of 256
-// slots; the refs of a round are enumerated in submission order in windows of
-// 256, ranked per tile with wave ballots and written at the tile's cursor.
+// Scatter: each reference goes to its tile's cursor, which starts at the
+// tile's list offset plus the counts of the earlier chunks (k_bin_colscan), so
+// a tile's list holds the chunks in submission order.  Within a chunk the
+// order is that of the LDS cursor increments: the raster's depth keys carry
+// the submission order, so a tile's winner does not depend on its list order.
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const FrameParams* __restrict__ jobs,
                                                         DeviceBuffers B) {
+    constexpr uint32_t kUnroll = 4;
     extern __shared__ uint32_t cursor[];
-    __shared__ uint32_t stageTile[kBlock], stagePay[kBlock];
-    __shared__ uint32_t wsum[kBlock / kWave];
+    __shared__ uint32_t sBig[kUnroll * kBlock], sBigN;  // a step's non-compact descriptors
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;
-    const uint32_t* offs = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
+    const uint32_t b0 = B.chunk_box[2 * blockIdx.x], b1 = B.chunk_box[2 * blockIdx.x + 1];
+    const uint32_t x0 = b0 & 0xFFFFu, y0 = b0 >> 16, x1 = b1 & 0xFFFFu, y1 = b1 >> 16;
+    if (x0 > x1 || y0 > y1) return;  // the chunk has no references
     {
         // Cursors of the tiles inside the chunk's tile box: every reference of
         // the chunk falls in it (k_bin_count).
-        const uint32_t b0 = B.chunk_box[2 * blockIdx.x], b1 = B.chunk_box[2 * blockIdx.x + 1];
-        const uint32_t x0 = b0 & 0xFFFFu, y0 = b0 >> 16, x1 = b1 & 0xFFFFu, y1 = b1 >> 16;
-        if (x0 <= x1 && y0 <= y1) {
-            const uint32_t bw = x1 - x0 + 1, nb = bw * (y1 - y0 + 1);
-            for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
-                const uint32_t t = (y0 + i / bw) * P.tiles_x + x0 + i % bw;
-                cursor[t] = B.tile_start[P.tile_base + t] + offs[t];
-            }
+        const uint32_t* offs = B.bin_counts + P.count_base + (size_t)chunk * P.num_tiles;
+        const uint32_t bw = x1 - x0 + 1, nb = bw * (y1 - y0 + 1);
+        for (uint32_t i = threadIdx.x; i < nb; i += kBlock) {
+            const uint32_t t = (y0 + i / bw) * P.tiles_x + x0 + i % bw;
+            cursor[t] = B.tile_start[P.tile_base + t] + offs[t];
         }
     }
+    if (threadIdx.x == 0) sBigN = 0;
     __syncthreads();
-    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     const uint32_t s0 = P.slot_base + chunk * P.chunk_slots;
     const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
     const uint32_t tilesX = P.tiles_x;
-    // The next round's descriptor and count are loaded before this round's
-    // work, so their latency overlaps it.
-    uint32_t descNext = s0 + threadIdx.x < s1 ? B.desc[s0 + threadIdx.x] : 0u;
-    uint32_t refsNext = s0 + threadIdx.x < s1 ? B.slot_refs[s0 + threadIdx.x] : 0u;
-    for (uint32_t rb = s0; rb < s1; rb += kBlock) {
-        const uint32_t s = rb + threadIdx.x;
-        const uint32_t desc = descNext;
-        uint32_t R = refsNext, tile, payload;
-        const uint32_t sn = s + kBlock;
-        descNext = sn < s1 ? B.desc[sn] : 0u;
-        refsNext = sn < s1 ? B.slot_refs[sn] : 0u;
-        RefEnum e;
-        e.init(desc, s);
-        if (R == kSlotRefsSaturated) {
-            R = 0;
-            while (e.next(P, B.rrec, tile, payload)) R++;
+    // The next step's descriptors are loaded before this step's work.
+    uint32_t dn[kUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kUnroll; u++) {
+        const uint32_t s = s0 + threadIdx.x + u * kBlock;
+        dn[u] = s < s1 ? B.desc[s] : 0u;
+    }
+    for (uint32_t sb = s0 + threadIdx.x; sb - threadIdx.x < s1; sb += kUnroll * kBlock) {
+        uint32_t d[kUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; u++) {
+            d[u] = dn[u];
+            const uint32_t sn = sb + (kUnroll + u) * kBlock;
+            dn[u] = sn < s1 ? B.desc[sn] : 0u;
         }
-        uint32_t total;
-        const uint32_t base = block_exclusive_scan(R, wsum, total);
-        e.init(desc, s);
-        for (uint32_t win = 0; win < total; win += kBlock) {
-            // Emit this thread's refs that fall in [win, win + 256).
-            const uint32_t lo = max(base, win), hi = min(base + R, win + kBlock);
-            if (desc & kDescCompact) {
-                for (uint32_t pos = lo; pos < hi; pos++) {
-                    stageTile[pos - win] = compact_tile(desc, pos - base, tilesX);
-                    stagePay[pos - win] = s;
-                }
-            } else {
-                for (uint32_t pos = lo; pos < hi; pos++) {
-                    e.next(P, B.rrec, tile, payload);
-                    stageTile[pos - win] = tile;
-                    stagePay[pos - win] = payload;
-                }
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; u++) {
+            const uint32_t s = sb + u * kBlock;
+            if (d[u] & kDescCompact) {
+                const uint32_t R = compact_refs(d[u]);
+                for (uint32_t k = 0; k < R; k++) B.refs[atomicAdd(&cursor[compact_tile(d[u], k, tilesX)], 1u)] = s;
+            } else if (d[u]) {
+                sBig[atomicAdd(&sBigN, 1u)] = d[u];
             }
-            __syncthreads();
-            const bool valid = win + threadIdx.x < total;
-            const uint32_t key = valid ? stageTile[threadIdx.x] : 0xFFFFFFFFu;
-            const uint32_t pay = valid ? stagePay[threadIdx.x] : 0u;
-            // Wave-level match: leader lane and rank among equal keys.
-            uint64_t remaining = __ballot(valid);
-            int myLeader = -1;
-            uint32_t myRank = 0, myCnt = 0;
-            while (remaining) {
-                const int leader = __ffsll((unsigned long long)remaining) - 1;
-                const uint32_t k = __shfl(key, leader);
-                const uint64_t m = __ballot(valid && key == k);
-                if (valid && key == k) {
-                    myLeader = leader;
-                    myRank = (uint32_t)__popcll(m & lanemask_lt());
-                    myCnt = (uint32_t)__popcll(m);
-                }
-                remaining &= ~m;
-            }
-            // Waves take their turn in order so ranks follow submission order.
-            uint32_t myBase = 0;
-            for (uint32_t w = 0; w < kBlock / kWave; w++) {
-                if (w == wave && valid && myLeader == (int)lane) {
-                    myBase = cursor[key];
-                    cursor[key] = myBase + myCnt;
-                }
-                __syncthreads();
-            }
-            const uint32_t bse = __shfl(myBase, myLeader < 0 ? 0 : myLeader);
-            if (valid) B.refs[bse + myRank] = pay;
-            __syncthreads();
+        }
+        // Other descriptors: listed and shared out over the waves.
+        if (__syncthreads_or(any_non_compact(d))) {
+            const uint32_t nBig = sBigN;
+            __syncthreads();  // every thread has read the count
+            if (threadIdx.x == 0) sBigN = 0;
+            for (uint32_t i = threadIdx.x / kWave; i < nBig; i += kBlock / kWave)
+                wave_enum_desc(P, B.rrec, sBig[i], [&](uint32_t tile, int, int, uint32_t payload) {
+                    B.refs[atomicAdd(&cursor[tile], 1u)] = payload;
+                });
+            __syncthreads();  // the list is rewritten by the next step
         }
     }
 }
@@ -2209,6 +2149,9 @@ hipError_t init_kernels() {
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
     if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_bin_tilescan, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(2 * sizeof(uint32_t) * kTilescanStaged));
+    if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&g_rasterResident[0],
                                                          reinterpret_cast<const void*>(&k_raster<false>), kBlock, 0);
     if (e == hipSuccess)
@@ -2270,7 +2213,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     hipLaunchKernelGGL(k_bin_groupscan, dim3((Q.max_job_tiles + kBlock - 1) / kBlock, 1, Q.num_jobs), dim3(kBlock), 0,
                        s, Q, jobs, B);
     if (Q.total_chunks) hipLaunchKernelGGL(k_bin_colscan, colGrid, dim3(kBlock), 0, s, Q, jobs, B);
-    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, Q, B);
+    const size_t tsLds = Q.total_tiles <= kTilescanStaged ? 2 * sizeof(uint32_t) * Q.total_tiles : 0;
+    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), tsLds, s, Q, B);
     mark(3);
     if (Q.total_chunks) hipLaunchKernelGGL(k_bin_scatter, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
     mark(4);

@@ -324,7 +324,11 @@ int edxr_backward(edxr_ctx* ctx, const float* d_rgb, const edxr_grads* grads, ui
  * writes at most cap records. */
 int64_t edxr_debug_read_records(edxr_ctx* ctx, uint32_t* out, uint64_t cap);
 /* Per-tile reference lists: offsets[num_tiles+1] and per reference
- * {tri_id, sub, flags} (flags: bit0-2 accept edge, bit3 trivial, bit4 big).
+ * {tri_id, sub, flags} (flags: bit0-2 accept edge, bit3 trivial, bit4 big),
+ * each tile's references in submission order ((tri_id, sub) ascending, as the
+ * reference's per-core lists concatenated in core order).  The device lists
+ * keep one binning chunk's references of a tile in no fixed order; the depth
+ * keys carry the submission order, so the frame does not depend on it.
  * Returns the reference count or a negative error. */
 int64_t edxr_debug_read_bins(edxr_ctx* ctx, uint64_t* offsets, uint32_t* refs3, uint64_t cap);
 
