@@ -522,6 +522,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         // work queue has drained.  The tuning's workgroups per CU set the
         // segment size only (raster_blocks).
         B.raster_grid = c->numCUs * raster_resident_per_cu(Q.max_samples > 1);
+        B.num_cus = c->numCUs;
         B.counters = c->counters.p;
         B.color = color;
         B.depth = depth;
@@ -1020,6 +1021,7 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
         B.work_cap = (uint32_t)workCap;
         B.raster_blocks = rasterBlocks;
         B.raster_grid = c->numCUs * raster_resident_per_cu(false);
+        B.num_cus = c->numCUs;
         B.counters = c->counters.p;
         B.color = static_cast<uint32_t*>(color);
         B.depth = static_cast<float*>(depth);

@@ -249,6 +249,7 @@ struct DeviceBuffers {
     uint64_t* partials;    // work_cap * samples * 1024 (depth, order) keys, one tile image per item and sample
     uint32_t raster_blocks;  // work items the segment size is chosen for
     uint32_t raster_grid;    // launched raster workgroups: the resident capacity
+    uint32_t num_cus;        // compute units of the device
     uint32_t* counters;
     uint32_t* color;       // every job's pixels at its pix_base (resolved)
     float* depth;          // samples planes of the job's pixels (single frame), sample-major

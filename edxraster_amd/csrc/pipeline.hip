@@ -595,7 +595,11 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
                 // A parent vertex with a nonzero outcode that the polygon kept
                 // (an intersection weight that rounds to 1): k_vertex did not
                 // store it.  Every writer stores the same value.
-                B.verts[vid] = which == 0 ? pa : which == 1 ? pb : pc;
+                // (three stores, not a selected struct, so that the three
+                // vertices are not spilled for a dynamic index)
+                if (which == 0) B.verts[vid] = pa;
+                else if (which == 1) B.verts[vid] = pb;
+                else B.verts[vid] = pc;
             }
             const uint32_t vid0 = __shfl(vid, 0), vid1 = __shfl(vid, fk - 1 < 64 ? fk - 1 : 0),
                            vid2 = __shfl(vid, fk < 64 ? fk : 0);
@@ -2200,7 +2204,9 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     if (Q.slot_blocks) {
         hipLaunchKernelGGL(k_setup, dim3((Q.slot_blocks + kSetupSlots - 1) / kSetupSlots), dim3(kBlock), 0, s, Q,
                            jobs, B);
-        const uint32_t clipBlocks = max(1u, min(1024u, (B.clip_queue_cap + kClipWaves - 1) / kClipWaves));
+        // One workgroup per CU at most (its occupancy): the queue's length is
+        // known only on the device, and each workgroup loops over it.
+        const uint32_t clipBlocks = max(1u, min(B.num_cus, (B.clip_queue_cap + kClipWaves - 1) / kClipWaves));
         hipLaunchKernelGGL(k_clip, dim3(clipBlocks), dim3(kClipWaves * 64), 0, s, Q, jobs, B);
     }
     mark(1);
