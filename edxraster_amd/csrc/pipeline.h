@@ -34,6 +34,10 @@ constexpr uint32_t kImageBits = 26;
 constexpr uint32_t kImageMask = (1u << kImageBits) - 1;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
 constexpr uint32_t kTilescanStaged = 8192;   // batch tiles whose counts k_bin_tilescan stages in LDS
+// Single-sample shading rebuilds a winner's screen values from its shading
+// vertices, instead of gathering them, in tiles of this many references or
+// more (about one triangle per two pixels of a 32x32 tile).
+constexpr uint32_t kShadeRecomputeRefs = 512;
 constexpr uint32_t kJobAlign = 1024;         // job slot alignment: vertex/setup blocks never span jobs
 constexpr uint32_t kJobTableBlock = 256;     // slots (vertices) per entry of the block -> job tables
 constexpr uint32_t kMaxChunks = 4096;

@@ -1339,7 +1339,8 @@ __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2,
         const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
         if (!(d <= 1.0f)) return;   // never passes d <= D with D <= 1 (NaN included)
         const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)klo;
-        if (key < *slot) atomicMin((unsigned long long*)slot, (unsigned long long)key);
+        // (the atomic alone is cheaper than a read, a compare and the atomic)
+        atomicMin((unsigned long long*)slot, (unsigned long long)key);
         return;
     }
     for (uint32_t k = 0; k < sp.n; k++) {
@@ -1995,30 +1996,51 @@ __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, c
 // triangle ID (Core/Renderer.cpp:272-350).  Empty tiles of single-sample
 // batches are cleared by k_raster.
 namespace {
-// Record and shading-vertex slots of the winner with submission order `order`:
+// Record and shading vertices of the winner with submission order `order`:
 // clipped pieces from their stored record, unclipped triangles rebuilt from
-// their vertices' screen values (1/det recomputed as setup computed it).
-__device__ __forceinline__ void winner_tri(const FrameParams& P, const DeviceBuffers& B, uint32_t order, RTri& t,
-                                           uint32_t& v0i, uint32_t& v1i, uint32_t& v2i) {
+// their vertices' screen values (1/det as setup computed it).  With
+// recompute, the screen values come from the shading vertices' positions
+// (transform, divide and snap exactly as k_vertex computes them) instead of
+// a second gather: cheaper where neighbouring pixels have different winners
+// (deep tiles of small triangles), dearer where they share them.
+__device__ __forceinline__ void winner_tri(const FrameParams& P, const DeviceBuffers& B, uint32_t order, bool recompute,
+                                           RTri& t, ShadeVert& v0, ShadeVert& v1, ShadeVert& v2) {
     const uint32_t tri = order >> 4;  // triangle slot
     if (order & 1u) {
         const uint32_t ri = (B.desc[tri] >> 3) + ((order >> 1) & 7u);
         const ShadeRec s = B.srec[ri];
-        v0i = s.v0;
-        v1i = s.v1;
-        v2i = s.v2;
+        v0 = B.verts[s.v0];
+        v1 = B.verts[s.v1];
+        v2 = B.verts[s.v2];
         load_rtri(B.rrec[ri], t);
     } else {
         const uint32_t* idx = P.mesh.idx + 3 * (size_t)(tri - P.slot_base);
-        v0i = P.vert_base + idx[0];
-        v1i = P.vert_base + idx[1];
-        v2i = P.vert_base + idx[2];
-        const ScreenVert s0 = B.sverts[v0i], s1 = B.sverts[v1i], s2 = B.sverts[v2i];
-        t.x0 = s0.x; t.y0 = s0.y; t.x1 = s1.x; t.y1 = s1.y; t.x2 = s2.x; t.y2 = s2.y;
+        const uint32_t v0i = P.vert_base + idx[0], v1i = P.vert_base + idx[1], v2i = P.vert_base + idx[2];
+        if (recompute) {
+            v0 = B.verts[v0i];
+            v1 = B.verts[v1i];
+            v2 = B.verts[v2i];
+            auto screen = [&](const ShadeVert& v, int& x, int& y, float& z) {
+                const float4 c = xform4(P.mvp, v.px, v.py, v.pz, 1.0f);
+                const int2 xy = snap_point(P, c.x / c.w, c.y / c.w, c.z / c.w);
+                x = xy.x;
+                y = xy.y;
+                z = c.z * (1.0f / c.w);
+            };
+            screen(v0, t.x0, t.y0, t.z0);
+            screen(v1, t.x1, t.y1, t.z1);
+            screen(v2, t.x2, t.y2, t.z2);
+        } else {
+            const ScreenVert s0 = B.sverts[v0i], s1 = B.sverts[v1i], s2 = B.sverts[v2i];
+            t.x0 = s0.x; t.y0 = s0.y; t.x1 = s1.x; t.y1 = s1.y; t.x2 = s2.x; t.y2 = s2.y;
+            t.z0 = s0.z; t.z1 = s1.z; t.z2 = s2.z;
+            v0 = B.verts[v0i];
+            v1 = B.verts[v1i];
+            v2 = B.verts[v2i];
+        }
         t.B1 = t.y1 - t.y2; t.C1 = t.x2 - t.x1;
         t.B2 = t.y2 - t.y0; t.C2 = t.x0 - t.x2;
         t.inv_det = 1.0f / (float)(t.C2 * t.B1 - t.C1 * t.B2);
-        t.z0 = s0.z; t.z1 = s1.z; t.z2 = s2.z;
     }
 }
 
@@ -2076,11 +2098,10 @@ __global__ __launch_bounds__(512) void k_shade(BatchParams Q, const FrameParams*
         // The key's low word is the winner's submission order (make_order).
         const uint32_t order = 0xFFFFFFFEu - (uint32_t)key;
         RTri t;
-        uint32_t v0i, v1i, v2i;
-        winner_tri(P, B, order, t, v0i, v1i, v2i);
+        ShadeVert v0, v1, v2;
+        winner_tri(P, B, order, n >= kShadeRecomputeRefs, t, v0, v1, v2);
         float l0, l1, d;
         bary_depth(t, x * 16 + 8, y * 16 + 8, l0, l1, d);
-        const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
         const float3 c = shade_any<kTex>(P, t, order, x, y, l0, l1, v0, v1, v2);
         B.color[o] = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
         if (B.color_f32) {
@@ -2101,7 +2122,10 @@ __global__ __launch_bounds__(512) void k_shade(BatchParams Q, const FrameParams*
     const size_t plane = (size_t)P.width * P.height;
     const int cx = x * 16 + 8, cy = y * 16 + 8;
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // The last winner's record and colour are kept: neighbouring samples
+    // mostly share their winner.
     uint32_t cachedOrder = 0xFFFFFFFFu, cachedColor = 0u;
+    RTri t;
     for (uint32_t s = 0; s < S; s++) {
         const uint64_t key = n ? tile_key(g, nseg, S, s) : bg_key();
         uint32_t rgba = 0u;
@@ -2110,21 +2134,19 @@ __global__ __launch_bounds__(512) void k_shade(BatchParams Q, const FrameParams*
             B.ids[o + s * plane] = 0xFFFFFFFFu;
         } else {
             const uint32_t order = 0xFFFFFFFEu - (uint32_t)key;
-            RTri t;
-            uint32_t v0i, v1i, v2i;
-            winner_tri(P, B, order, t, v0i, v1i, v2i);
-            const int2 off = c_sample_offsets[P.sample_level][s];
             float l0, l1, d;
-            bary_depth(t, cx + off.x, cy + off.y, l0, l1, d);
-            B.depth[o + s * plane] = d;
-            B.ids[o + s * plane] = (order >> 4) - P.slot_base;
             if (order != cachedOrder) {
+                ShadeVert v0, v1, v2;
+                winner_tri(P, B, order, false, t, v0, v1, v2);
                 bary_depth(t, cx, cy, l0, l1, d);
-                const ShadeVert v0 = B.verts[v0i], v1 = B.verts[v1i], v2 = B.verts[v2i];
                 const float3 c = shade_any<kTex>(P, t, order, x, y, l0, l1, v0, v1, v2);
                 cachedColor = to_byte(c.x) | (to_byte(c.y) << 8) | (to_byte(c.z) << 16) | (255u << 24);
                 cachedOrder = order;
             }
+            const int2 off = c_sample_offsets[P.sample_level][s];
+            bary_depth(t, cx + off.x, cy + off.y, l0, l1, d);
+            B.depth[o + s * plane] = d;
+            B.ids[o + s * plane] = (order >> 4) - P.slot_base;
             rgba = cachedColor;
         }
         for (int ch = 0; ch < 4; ch++) acc[ch] += (float)((rgba >> (8 * ch)) & 255u) / 255.0f;
