@@ -270,6 +270,34 @@ int set_size(edxr_ctx* c, uint32_t w, uint32_t h) {
 // count/scatter grids fill the GPU, capped so the job's chunk x tile count
 // matrix (allocated in full, touched only inside each chunk's tile box) stays
 // within matrixBytes.
+// Sums of the striped counters of a frame (h: the counters read back).  The
+// clip stage allocates from kAllocStripes regions; a capacity that holds its
+// fullest stripe's demand in every region is that demand times the stripes.
+struct CounterSums {
+    uint64_t rejected = 0, mainRecords = 0, clipped = 0, nearClipped = 0, extraVerts = 0, extraRecs = 0;
+    uint32_t vertDemand = 0, recDemand = 0;
+};
+CounterSums counter_sums(const uint32_t* h) {
+    CounterSums cs;
+    for (uint32_t s = 0; s < kStripes; s++) {
+        cs.rejected += h[kCntStriped + kStRejected * kStripes + s];
+        cs.mainRecords += h[kCntStriped + kStMainRecords * kStripes + s];
+        cs.clipped += h[kCntStriped + kStClipped * kStripes + s];
+        cs.nearClipped += h[kCntStriped + kStNearClipped * kStripes + s];
+    }
+    uint32_t vmax = 0, rmax = 0;
+    for (uint32_t s = 0; s < kAllocStripes; s++) {
+        const uint32_t v = h[kCntAllocStriped + s], r = h[kCntAllocStriped + kAllocStripes + s];
+        cs.extraVerts += v;
+        cs.extraRecs += r;
+        vmax = std::max(vmax, v);
+        rmax = std::max(rmax, r);
+    }
+    cs.vertDemand = (uint32_t)std::min<uint64_t>((uint64_t)vmax * kAllocStripes, 0xFFFFFFF0ull);
+    cs.recDemand = (uint32_t)std::min<uint64_t>((uint64_t)rmax * kAllocStripes, 0xFFFFFFF0ull);
+    return cs;
+}
+
 uint32_t choose_chunks(const edxr_ctx* c, uint32_t T, uint32_t nTiles, bool single,
                        uint64_t matrixBytes = 256ull << 20) {
     if (T == 0) return 0;
@@ -465,8 +493,9 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         c->clipQueueCap = c->extraRecCap = c->extraVertCap = c->refCap = c->debugCapacity;
     } else {
         if (!c->clipQueueCap) c->clipQueueCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16));
-        if (!c->extraRecCap) c->extraRecCap = 4096;
-        if (!c->extraVertCap) c->extraVertCap = 8192;
+        // (the clip stage takes them in kAllocStripes regions)
+        if (!c->extraRecCap) c->extraRecCap = 16384;
+        if (!c->extraVertCap) c->extraVertCap = 16384;
         const uint32_t refWant = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * Q.total_tiles + 1024, 0xFFFFFFF0ull);
         c->refCap = std::max(c->refCap, refWant);
     }
@@ -564,19 +593,16 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         if (!ov) {
             if (h[kCntBadIndex])
                 return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices or texture slot >= texture count");
-            uint64_t rejected = 0, mainRecords = 0, tin = 0;
-            for (uint32_t s = 0; s < kStripes; s++) {
-                rejected += h[kCntStriped + kStRejected * kStripes + s];
-                mainRecords += h[kCntStriped + kStMainRecords * kStripes + s];
-            }
+            uint64_t tin = 0;
             for (const FrameParams& P : pl.jobs) tin += P.num_triangles;
+            const CounterSums cs = counter_sums(h);
             c->stats.triangles_in = tin;
-            c->stats.trivially_rejected = rejected;
-            c->stats.clipped = h[kCntClipped];
-            c->stats.near_clipped = h[kCntNearClipped];
-            c->stats.setup_records = mainRecords + h[kCntExtraRecords];
+            c->stats.trivially_rejected = cs.rejected;
+            c->stats.clipped = cs.clipped;
+            c->stats.near_clipped = cs.nearClipped;
+            c->stats.setup_records = cs.mainRecords + cs.extraRecs;
             c->stats.refs = h[kCntTotalRefs];
-            c->stats.clip_vertices = h[kCntExtraVertices];
+            c->stats.clip_vertices = cs.extraVerts;
             c->stats.retries = attempt;
             c->stats.bin_chunks = Q.total_chunks;
             if (prof) {
@@ -592,8 +618,9 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
             return (uint32_t)std::min<uint64_t>((uint64_t)need + need / 4 + 1024, 0xFFFFFFF0ull);
         };
         if (ov & kOvClipQueue) c->clipQueueCap = grow(h[kCntClipQueue]);
-        if (ov & kOvExtraRecords) c->extraRecCap = grow(std::max(h[kCntExtraRecords], 2 * c->extraRecCap));
-        if (ov & kOvExtraVertices) c->extraVertCap = grow(std::max(h[kCntExtraVertices], 2 * c->extraVertCap));
+        const CounterSums cs = counter_sums(h);
+        if (ov & kOvExtraRecords) c->extraRecCap = grow(std::max(cs.recDemand, 2 * c->extraRecCap));
+        if (ov & kOvExtraVertices) c->extraVertCap = grow(std::max(cs.vertDemand, 2 * c->extraVertCap));
         if (ov & kOvRefs) c->refCap = grow(std::max(h[kCntTotalRefs], c->refCap));
         // A clip-queue overflow hides the clip-stage demand: grow those too.
         if (ov & kOvClipQueue) {
@@ -977,7 +1004,7 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
     const uint64_t groupEntries = (uint64_t)((P.num_chunks + kChunkGroup - 1) / kChunkGroup) * P.num_tiles;
 
     // Capacities of this call (never the context's, which may be larger).
-    uint32_t clipCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16)), recCap = 4096, vertCap = 8192;
+    uint32_t clipCap = std::max<uint32_t>(1024, std::min<uint32_t>(T, 1u << 16)), recCap = 16384, vertCap = 16384;
     uint32_t refCap = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * P.num_tiles + 1024, 0xFFFFFFF0ull);
     // Few raster items: each holds an 8 KB key image in the scratch.
     const uint32_t rasterBlocks = c->numCUs * 2;
@@ -1039,18 +1066,14 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
         if (!ov) {
             if (h[kCntBadIndex])
                 return c->fail(EDXR_E_INDEX, "triangle index >= num_vertices or texture slot >= texture count");
-            uint64_t rejected = 0, mainRecords = 0;
-            for (uint32_t k = 0; k < kStripes; k++) {
-                rejected += h[kCntStriped + kStRejected * kStripes + k];
-                mainRecords += h[kCntStriped + kStMainRecords * kStripes + k];
-            }
+            const CounterSums cs = counter_sums(h);
             c->stats.triangles_in = T;
-            c->stats.trivially_rejected = rejected;
-            c->stats.clipped = h[kCntClipped];
-            c->stats.near_clipped = h[kCntNearClipped];
-            c->stats.setup_records = mainRecords + h[kCntExtraRecords];
+            c->stats.trivially_rejected = cs.rejected;
+            c->stats.clipped = cs.clipped;
+            c->stats.near_clipped = cs.nearClipped;
+            c->stats.setup_records = cs.mainRecords + cs.extraRecs;
             c->stats.refs = h[kCntTotalRefs];
-            c->stats.clip_vertices = h[kCntExtraVertices];
+            c->stats.clip_vertices = cs.extraVerts;
             c->stats.retries = attempt;
             c->stats.bin_chunks = P.num_chunks;
             return EDXR_OK;
@@ -1063,8 +1086,9 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
             recCap = std::max(recCap, grow(2 * recCap));
             vertCap = std::max(vertCap, grow(2 * vertCap));
         }
-        if (ov & kOvExtraRecords) recCap = grow(std::max(h[kCntExtraRecords], 2 * recCap));
-        if (ov & kOvExtraVertices) vertCap = grow(std::max(h[kCntExtraVertices], 2 * vertCap));
+        const CounterSums cs = counter_sums(h);
+        if (ov & kOvExtraRecords) recCap = grow(std::max(cs.recDemand, 2 * recCap));
+        if (ov & kOvExtraVertices) vertCap = grow(std::max(cs.vertDemand, 2 * vertCap));
         if (ov & kOvRefs) refCap = grow(std::max(h[kCntTotalRefs], refCap));
     }
     return c->fail(EDXR_E_LIMIT, "capacity retries exhausted");

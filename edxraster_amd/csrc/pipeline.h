@@ -199,26 +199,24 @@ struct __attribute__((aligned(16))) ShadeRec {
 
 enum CounterSlot {
     kCntClipQueue = 0,
-    kCntExtraRecords,
-    kCntExtraVertices,
     kCntOverflow,        // bit0 clip queue, bit1 extra records, bit2 extra vertices, bit3 refs, bit4 polygon
     kCntBadIndex,
-    kCntRejected,
-    kCntClipped,
-    kCntNearClipped,
-    kCntMainRecords,
     kCntTotalRefs,
     kCntWorkItems,       // raster work items (tile segments)
     kCntWorkNext,        // next work item to claim
     kCntSegLog2,         // log2 of the tile references per work item of this frame
     kCntStriped = 16,    // high-rate counters: kStripes slots each, summed on the host
-    kCntSlots = kCntStriped + 2 * 32
+    kCntAllocStriped = kCntStriped + 4 * 32,  // k_clip's allocations: kAllocStripes vertices, then records
+    kCntSlots = kCntAllocStriped + 2 * 16
 };
 
 // Counters bumped by many workgroups are striped over kStripes addresses so
 // the atomics do not serialise on one cache line.
 constexpr uint32_t kStripes = 32;
-enum StripedCounter { kStRejected = 0, kStMainRecords = 1 };
+enum StripedCounter { kStRejected = 0, kStMainRecords = 1, kStClipped = 2, kStNearClipped = 3 };
+// k_clip's workgroups allocate extra vertices and records from kAllocStripes
+// counters, each over its own region (1 / kAllocStripes of the capacity).
+constexpr uint32_t kAllocStripes = 16;
 
 enum OverflowBit {
     kOvClipQueue = 1,

@@ -422,6 +422,9 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
     const uint32_t wave = threadIdx.x / 64;
     const uint32_t n = min(B.counters[kCntClipQueue], B.clip_queue_cap);
     uint32_t clipped = 0, nearClipped = 0;  // this wave's statistics (lane 0)
+    // The workgroup's allocation stripe and the stripes' regions.
+    const uint32_t stripe = blockIdx.x % kAllocStripes;
+    const uint32_t vRegion = B.extra_vert_cap / kAllocStripes, rRegion = B.extra_rec_cap / kAllocStripes;
     for (uint32_t base = blockIdx.x * kClipWaves; base < n; base += gridDim.x * kClipWaves) {
         const uint32_t q = base + wave;
         const bool live = q < n;
@@ -548,17 +551,22 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
                 tv += sNew[w];
                 tr += sRec[w];
             }
-            const uint32_t vb = tv ? atomicAdd(&B.counters[kCntExtraVertices], tv) : 0u;
-            const uint32_t rb = tr ? atomicAdd(&B.counters[kCntExtraRecords], tr) : 0u;
+            // One allocation per workgroup pass, from its stripe's region of
+            // the extra vertices and records: every live workgroup reaches
+            // this point at about the same time, and atomics on one address
+            // would serialise.
+            const uint32_t vb = tv ? atomicAdd(&B.counters[kCntAllocStriped + stripe], tv) : 0u;
+            const uint32_t rb = tr ? atomicAdd(&B.counters[kCntAllocStriped + kAllocStripes + stripe], tr) : 0u;
             for (uint32_t w = 0; w < kClipWaves; w++) {
-                sVBase[w] += vb;
-                sRBase[w] += rb;
+                sVBase[w] += stripe * vRegion + vb;
+                sRBase[w] += stripe * rRegion + rb;
             }
         }
         __syncthreads();
         if (threadIdx.x == 0 && (sStat[0] | sStat[1])) {
-            atomicAdd(&B.counters[kCntClipped], sStat[0]);
-            if (sStat[1]) atomicAdd(&B.counters[kCntNearClipped], sStat[1]);
+            const uint32_t st = blockIdx.x & (kStripes - 1);
+            atomicAdd(&B.counters[kCntStriped + kStClipped * kStripes + st], sStat[0]);
+            if (sStat[1]) atomicAdd(&B.counters[kCntStriped + kStNearClipped * kStripes + st], sStat[1]);
         }
         // ---- per wave: write the new vertices, the records and the descriptor ----
         if (live) {
@@ -566,10 +574,10 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
             const uint32_t vbase = sVBase[wave], rbase = sRBase[wave];
             // As the serial clipper: the polygon's new vertices are created
             // whenever it survives, its records when it has fans.
-            const bool vfit = vbase + nNew <= B.extra_vert_cap;
+            const bool vfit = vbase + nNew <= (stripe + 1) * vRegion;
             if (!vfit && lane == 0) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvExtraVertices);
             bool keep = vfit && nrec > 0;
-            if (keep && rbase + nrec > B.extra_rec_cap) {
+            if (keep && rbase + nrec > (stripe + 1) * rRegion) {
                 if (lane == 0) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvExtraRecords);
                 keep = false;
             }
