@@ -698,10 +698,9 @@ __device__ __forceinline__ uint32_t compact_refs(uint32_t desc) {
 // A slot's references: its records, the tiles of a record in its tile bbox.
 // A compact descriptor is its record's tile bbox (at most 2x2 tiles, no corner
 // tests; the record is the slot).  Otherwise the descriptor is
-// (first record << 3) | count, and the binning kernels list it for a wave:
-// the wave's lanes test a record's bbox tiles 64 at a time, the next record
-// loaded while the current one is tested.  emit(tile, tx, ty, payload) runs on
-// the lane that found the reference; there is no order among them.
+// (first record << 3) | count: the binning kernels list such descriptors per
+// step and hand them to block_enum_listed.
+
 // Whether any of a thread's descriptors of a step is non-compact.
 template <uint32_t kN>
 __device__ __forceinline__ int any_non_compact(const uint32_t (&d)[kN]) {
@@ -711,35 +710,72 @@ __device__ __forceinline__ int any_non_compact(const uint32_t (&d)[kN]) {
     return any ? 1 : 0;
 }
 
+// The references of record rec (its tile bbox loaded in b) whose bbox tiles
+// are k0, k0 + dk, ... (row-major over the bbox), passed to emit(tile, tx,
+// ty, payload).
 template <class Emit>
-__device__ __forceinline__ void wave_enum_desc(const FrameParams& P, const RasterRec* rrec, uint32_t desc,
-                                               Emit&& emit) {
-    const uint32_t first = desc >> 3, count = desc & 7u;
-    const uint32_t lane = __lane_id();
-    RasterRec next = rrec[first];
-    for (uint32_t r = 0; r < count; r++) {
-        BinTri b;
-        load_bin_tri(P, next, b);
-        if (r + 1 < count) next = rrec[first + r + 1];
-        const uint32_t n = (uint32_t)(b.w * b.h);
-        if (n == 0) continue;
-        const uint32_t w = (uint32_t)b.w, rec = first + r;
-        // tile k = lane + 64 i at (k % w, k / w), stepped
-        uint32_t cx = lane % w, cy = lane / w;
-        const uint32_t dx = kWave % w, dy = kWave / w;
-        for (uint32_t k = lane; k < n; k += kWave) {
-            const int tx = b.minTX + (int)cx, ty = b.minTY + (int)cy;
-            const int f = b.small ? 0 : bin_test(b, tx, ty);
-            if (f >= 0)
-                emit((uint32_t)ty * P.tiles_x + (uint32_t)tx, tx, ty,
-                     b.small ? rec : rec | ((uint32_t)f << kRecordBits) | (1u << 31));
-            cx += dx;
-            cy += dy;
-            if (cx >= w) {
-                cx -= w;
-                cy++;
-            }
+__device__ __forceinline__ void enum_record_tiles(const FrameParams& P, const BinTri& b, uint32_t rec, uint32_t k0,
+                                                  uint32_t dk, Emit&& emit) {
+    const uint32_t n = (uint32_t)(b.w * b.h);
+    if (n == 0) return;
+    const uint32_t w = (uint32_t)b.w;
+    uint32_t cx = k0 % w, cy = k0 / w;
+    const uint32_t dx = dk % w, dy = dk / w;
+    for (uint32_t k = k0; k < n; k += dk) {
+        const int tx = b.minTX + (int)cx, ty = b.minTY + (int)cy;
+        const int f = b.small ? 0 : bin_test(b, tx, ty);
+        if (f >= 0)
+            emit((uint32_t)ty * P.tiles_x + (uint32_t)tx, tx, ty,
+                 b.small ? rec : rec | ((uint32_t)f << kRecordBits) | (1u << 31));
+        cx += dx;
+        cy += dy;
+        if (cx >= w) {
+            cx -= w;
+            cy++;
         }
+    }
+}
+
+// Records of a record's bbox this size or smaller are enumerated by one
+// thread; larger ones by a whole wave, 64 tiles at a time.
+constexpr uint32_t kThreadTiles = 16;
+// Wide records listed per step: beyond this many, a thread enumerates its
+// record itself.
+constexpr uint32_t kWideCap = 256;
+
+// All references of the nBig listed non-compact descriptors, for the whole
+// workgroup (every thread must call it): threads take (descriptor, record)
+// pairs, 8 per descriptor (at most 7 records), so that the records' loads
+// are in flight together; each thread enumerates a small record's tiles
+// itself and lists a wide one, which a wave then enumerates.  There is no
+// order among the emitted references.
+template <class Emit>
+__device__ __forceinline__ void block_enum_listed(const FrameParams& P, const RasterRec* rrec, const uint32_t* sBig,
+                                                  uint32_t nBig, uint32_t* sWide, uint32_t* sWideN, Emit&& emit) {
+    constexpr uint32_t kPerDesc = 8;
+    for (uint32_t base = 0; base < nBig * kPerDesc; base += kBlock) {
+        const uint32_t j = base + threadIdx.x, i = j / kPerDesc, r = j % kPerDesc;
+        const uint32_t d = i < nBig ? sBig[i] : 0u;
+        if (r < (d & 7u)) {
+            const uint32_t rec = (d >> 3) + r;
+            BinTri b;
+            load_bin_tri(P, rrec[rec], b);
+            const uint32_t n = (uint32_t)(b.w * b.h);
+            uint32_t slot = kWideCap;
+            if (n > kThreadTiles) {
+                slot = atomicAdd(sWideN, 1u);
+                if (slot < kWideCap) sWide[slot] = rec;
+            }
+            if (slot >= kWideCap) enum_record_tiles(P, b, rec, 0, 1, emit);
+        }
+    }
+    __syncthreads();
+    const uint32_t nWide = min(*sWideN, kWideCap);
+    for (uint32_t i = threadIdx.x / kWave; i < nWide; i += kBlock / kWave) {
+        const uint32_t rec = sWide[i];
+        BinTri b;
+        load_bin_tri(P, rrec[rec], b);
+        enum_record_tiles(P, b, rec, __lane_id(), kWave, emit);
     }
 }
 
@@ -751,6 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t sBox[4];  // min x, min y, max x, max y of the chunk's tiles
     __shared__ uint32_t sBig[kUnroll * kBlock], sBigN;  // a step's non-compact descriptors
+    __shared__ uint32_t sWide[kWideCap], sWideN;         // and their wide records
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;  // chunk of the job
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) hist[i] = 0;
@@ -765,8 +802,8 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
     const uint32_t s1 = min(P.slot_base + P.num_triangles, s0 + P.chunk_slots);
     // Four slots per thread and step, their descriptors loaded together so
     // the loads' latencies overlap.  Compact descriptors (at most 2x2 tiles)
-    // are counted by their lanes; the others are listed and shared out over
-    // the waves at the end of the step.
+    // are counted by their lanes; the others are listed and enumerated by the
+    // whole workgroup at the end of the step.
     const uint32_t tilesX = P.tiles_x;
     // A tile's count of heavy references rides in the high half of its
     // histogram entry while a chunk cannot reach 2^16 references per tile (at
@@ -814,16 +851,20 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
         if (__syncthreads_or(any_non_compact(d))) {
             const uint32_t nBig = sBigN;
             __syncthreads();  // every thread has read the count
-            if (threadIdx.x == 0) sBigN = 0;
-            for (uint32_t i = threadIdx.x / kWave; i < nBig; i += kBlock / kWave)
-                wave_enum_desc(P, B.rrec, sBig[i], [&](uint32_t tile, int tx, int ty, uint32_t payload) {
-                    bump(tile, (payload >> 31) != 0);
-                    bx0 = min(bx0, (uint32_t)tx);
-                    by0 = min(by0, (uint32_t)ty);
-                    bx1 = max(bx1, (uint32_t)tx);
-                    by1 = max(by1, (uint32_t)ty);
-                });
-            __syncthreads();  // the list is rewritten by the next step
+            if (threadIdx.x == 0) {
+                sBigN = 0;
+                sWideN = 0;
+            }
+            __syncthreads();
+            block_enum_listed(P, B.rrec, sBig, nBig, sWide, &sWideN,
+                              [&](uint32_t tile, int tx, int ty, uint32_t payload) {
+                                  bump(tile, (payload >> 31) != 0);
+                                  bx0 = min(bx0, (uint32_t)tx);
+                                  by0 = min(by0, (uint32_t)ty);
+                                  bx1 = max(bx1, (uint32_t)tx);
+                                  by1 = max(by1, (uint32_t)ty);
+                              });
+            __syncthreads();  // the lists are rewritten by the next step
         }
     }
     // The chunk's tile box (k_bin_scatter sets up cursors only inside it).
@@ -1166,6 +1207,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
     constexpr uint32_t kUnroll = 4;
     extern __shared__ uint32_t cursor[];
     __shared__ uint32_t sBig[kUnroll * kBlock], sBigN;  // a step's non-compact descriptors
+    __shared__ uint32_t sWide[kWideCap], sWideN;         // and their wide records
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;
@@ -1212,16 +1254,20 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
                 sBig[atomicAdd(&sBigN, 1u)] = d[u];
             }
         }
-        // Other descriptors: listed and shared out over the waves.
+        // Other descriptors: listed and enumerated by the whole workgroup.
         if (__syncthreads_or(any_non_compact(d))) {
             const uint32_t nBig = sBigN;
             __syncthreads();  // every thread has read the count
-            if (threadIdx.x == 0) sBigN = 0;
-            for (uint32_t i = threadIdx.x / kWave; i < nBig; i += kBlock / kWave)
-                wave_enum_desc(P, B.rrec, sBig[i], [&](uint32_t tile, int, int, uint32_t payload) {
-                    B.refs[atomicAdd(&cursor[tile], 1u)] = payload;
-                });
-            __syncthreads();  // the list is rewritten by the next step
+            if (threadIdx.x == 0) {
+                sBigN = 0;
+                sWideN = 0;
+            }
+            __syncthreads();
+            block_enum_listed(P, B.rrec, sBig, nBig, sWide, &sWideN,
+                              [&](uint32_t tile, int, int, uint32_t payload) {
+                                  B.refs[atomicAdd(&cursor[tile], 1u)] = payload;
+                              });
+            __syncthreads();  // the lists are rewritten by the next step
         }
     }
 }
