@@ -112,6 +112,16 @@ __device__ __forceinline__ bool setup_record(const FrameParams& P, float ax, flo
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+// Inclusive prefix sum over the wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = __lane_id();
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(v, d);
+        if (lane >= (uint32_t)d) v += y;
+    }
+    return v;
+}
+
 
 __device__ __forceinline__ uint32_t wave_count(bool p) { return (uint32_t)__popcll(__ballot(p)); }
 
@@ -744,20 +754,42 @@ constexpr uint32_t kThreadTiles = 16;
 constexpr uint32_t kWideCap = 256;
 
 // All references of the nBig listed non-compact descriptors, for the whole
-// workgroup (every thread must call it): threads take (descriptor, record)
-// pairs, 8 per descriptor (at most 7 records), so that the records' loads
-// are in flight together; each thread enumerates a small record's tiles
-// itself and lists a wide one, which a wave then enumerates.  There is no
-// order among the emitted references.
+// workgroup (every thread must call it; sScan holds kBlock + kBlock / kWave
+// words): the descriptors' records are numbered, kBlock descriptors at a
+// time, and each thread takes records, so that the records' loads are in
+// flight together.  A thread enumerates a record of up to kThreadTiles bbox
+// tiles itself and lists a wider one, which a wave then enumerates.  There is
+// no order among the emitted references.
 template <class Emit>
 __device__ __forceinline__ void block_enum_listed(const FrameParams& P, const RasterRec* rrec, const uint32_t* sBig,
-                                                  uint32_t nBig, uint32_t* sWide, uint32_t* sWideN, Emit&& emit) {
-    constexpr uint32_t kPerDesc = 8;
-    for (uint32_t base = 0; base < nBig * kPerDesc; base += kBlock) {
-        const uint32_t j = base + threadIdx.x, i = j / kPerDesc, r = j % kPerDesc;
-        const uint32_t d = i < nBig ? sBig[i] : 0u;
-        if (r < (d & 7u)) {
-            const uint32_t rec = (d >> 3) + r;
+                                                  uint32_t nBig, uint32_t* sScan, uint32_t* sWide, uint32_t* sWideN,
+                                                  Emit&& emit) {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
+    uint32_t* sPre = sScan;            // record number of the batch's first record of each descriptor
+    uint32_t* sWaveSum = sScan + kBlock;
+    for (uint32_t b0 = 0; b0 < nBig; b0 += kBlock) {
+        const uint32_t nb = min(nBig - b0, kBlock);
+        const uint32_t c = threadIdx.x < nb ? sBig[b0 + threadIdx.x] & 7u : 0u;
+        const uint32_t incl = wave_incl_scan(c);
+        if (lane == kWave - 1) sWaveSum[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < kBlock / kWave; w++) {
+            before += w < wave ? sWaveSum[w] : 0u;
+            total += sWaveSum[w];
+        }
+        sPre[threadIdx.x] = before + incl - c;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < total; j += kBlock) {
+            // the descriptor whose records hold j: every listed count is >= 1,
+            // so the numbers increase strictly over [0, nb)
+            uint32_t lo = 0, hi = nb - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (sPre[mid] <= j) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint32_t rec = (sBig[b0 + lo] >> 3) + (j - sPre[lo]);
             BinTri b;
             load_bin_tri(P, rrec[rec], b);
             const uint32_t n = (uint32_t)(b.w * b.h);
@@ -768,14 +800,14 @@ __device__ __forceinline__ void block_enum_listed(const FrameParams& P, const Ra
             }
             if (slot >= kWideCap) enum_record_tiles(P, b, rec, 0, 1, emit);
         }
+        __syncthreads();  // the numbering is rewritten by the next batch
     }
-    __syncthreads();
     const uint32_t nWide = min(*sWideN, kWideCap);
-    for (uint32_t i = threadIdx.x / kWave; i < nWide; i += kBlock / kWave) {
+    for (uint32_t i = wave; i < nWide; i += kBlock / kWave) {
         const uint32_t rec = sWide[i];
         BinTri b;
         load_bin_tri(P, rrec[rec], b);
-        enum_record_tiles(P, b, rec, __lane_id(), kWave, emit);
+        enum_record_tiles(P, b, rec, lane, kWave, emit);
     }
 }
 
@@ -788,6 +820,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
     __shared__ uint32_t sBox[4];  // min x, min y, max x, max y of the chunk's tiles
     __shared__ uint32_t sBig[kUnroll * kBlock], sBigN;  // a step's non-compact descriptors
     __shared__ uint32_t sWide[kWideCap], sWideN;         // and their wide records
+    __shared__ uint32_t sScan[kBlock + kBlock / kWave];
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;  // chunk of the job
     for (uint32_t i = threadIdx.x; i < P.num_tiles; i += kBlock) hist[i] = 0;
@@ -856,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
                 sWideN = 0;
             }
             __syncthreads();
-            block_enum_listed(P, B.rrec, sBig, nBig, sWide, &sWideN,
+            block_enum_listed(P, B.rrec, sBig, nBig, sScan, sWide, &sWideN,
                               [&](uint32_t tile, int tx, int ty, uint32_t payload) {
                                   bump(tile, (payload >> 31) != 0);
                                   bx0 = min(bx0, (uint32_t)tx);
@@ -1208,6 +1241,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
     extern __shared__ uint32_t cursor[];
     __shared__ uint32_t sBig[kUnroll * kBlock], sBigN;  // a step's non-compact descriptors
     __shared__ uint32_t sWide[kWideCap], sWideN;         // and their wide records
+    __shared__ uint32_t sScan[kBlock + kBlock / kWave];
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;
@@ -1263,7 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
                 sWideN = 0;
             }
             __syncthreads();
-            block_enum_listed(P, B.rrec, sBig, nBig, sWide, &sWideN,
+            block_enum_listed(P, B.rrec, sBig, nBig, sScan, sWide, &sWideN,
                               [&](uint32_t tile, int, int, uint32_t payload) {
                                   B.refs[atomicAdd(&cursor[tile], 1u)] = payload;
                               });
@@ -1744,16 +1778,6 @@ __device__ __forceinline__ void raster_large_wave(const FrameParams& P, const RT
             }
         }
     }
-}
-
-// Inclusive prefix sum over the wave.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = __lane_id();
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(v, d);
-        if (lane >= (uint32_t)d) v += y;
-    }
-    return v;
 }
 
 // The record of lane l, made wave-uniform.
