@@ -1445,8 +1445,7 @@ __device__ __forceinline__ void pixel_key(const RTri& t, int e0, int e1, int e2,
         const float d = l0 * t.z0 + l1 * t.z1 + l2 * t.z2;
         if (!(d <= 1.0f)) continue;
         const uint64_t key = ((uint64_t)depth_bits(d) << 32) | (uint64_t)klo;
-        uint64_t* sl = slot + k * kKeyPlane<kMS>;
-        if (key < *sl) atomicMin((unsigned long long*)sl, (unsigned long long)key);
+        atomicMin((unsigned long long*)(slot + k * kKeyPlane<kMS>), (unsigned long long)key);
     }
 }
 
