@@ -151,8 +151,8 @@ def pmc_summary(fetch_dir, write_dir):
                     name = row["Kernel_Name"]
                     key = ("k_raster<true>" if "k_rasterILb1" in name or "k_raster<true" in name else
                            "k_raster<false>" if "k_raster" in name else
-                           "k_shade<true>" if "k_shadeILb1" in name or "k_shade<true" in name else
-                           "k_shade<false>" if "k_shade" in name else
+                           "k_shade_tex" if "k_shade_tex" in name else
+                           "k_shade" if "k_shade" in name else
                            "calibration" if CALIBRATION_KERNEL in name else None)
                     if key:
                         acc.setdefault(key, []).append(float(row["Counter_Value"]))
@@ -163,7 +163,7 @@ def pmc_summary(fetch_dir, write_dir):
     known_kib = CALIBRATION_BYTES / 1024.0
     ff = known_kib / fetch["calibration"]
     fw = known_kib / write["calibration"]
-    rows = [k for k in ("k_raster<false>", "k_raster<true>", "k_shade<false>", "k_shade<true>") if k in fetch]
+    rows = [k for k in ("k_raster<false>", "k_raster<true>", "k_shade", "k_shade_tex") if k in fetch]
     traffic_kib = sum(fetch[k] * ff + write.get(k, 0.0) * fw for k in rows)
     return {"fetch_kib": fetch, "write_kib": write, "dispatches": {"fetch": nf, "write": nw},
             "fetch_factor": ff, "write_factor": fw, "kernels": rows,

@@ -2142,8 +2142,8 @@ __device__ __forceinline__ uint64_t tile_key(const uint64_t* g, uint32_t nseg, u
 }  // namespace
 
 template <bool kTex>
-__global__ __launch_bounds__(512) void k_shade(BatchParams Q, const FrameParams* __restrict__ jobs,
-                                                 DeviceBuffers B) {
+__device__ __forceinline__ void shade_body(const BatchParams& Q, const FrameParams* __restrict__ jobs,
+                                           const DeviceBuffers& B) {
     constexpr uint32_t kPix = kTileSize * kTileSize;
     if (B.counters[kCntOverflow] & kOvRefs) return;
     const uint32_t gtile = blockIdx.x >> 1;
@@ -2238,6 +2238,18 @@ __global__ __launch_bounds__(512) void k_shade(BatchParams Q, const FrameParams*
     }
 }
 
+// Untextured launches are held to 64 VGPRs, so that eight waves per SIMD are
+// resident (two of them spill to scratch outside the pixel's gathers); the
+// texture samplers need more.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_shade(
+    BatchParams Q, const FrameParams* __restrict__ jobs, DeviceBuffers B) {
+    shade_body<false>(Q, jobs, B);
+}
+__global__ __launch_bounds__(512) void k_shade_tex(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                   DeviceBuffers B) {
+    shade_body<true>(Q, jobs, B);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -2325,8 +2337,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     mark(4);
     if (Q.max_samples > 1) hipLaunchKernelGGL((k_raster<true>), dim3(B.raster_grid), dim3(kBlock), 0, s, Q, jobs, B);
     else hipLaunchKernelGGL((k_raster<false>), dim3(B.raster_grid), dim3(kBlock), 0, s, Q, jobs, B);
-    if (Q.any_textured) hipLaunchKernelGGL((k_shade<true>), dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
-    else hipLaunchKernelGGL((k_shade<false>), dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
+    if (Q.any_textured) hipLaunchKernelGGL(k_shade_tex, dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
+    else hipLaunchKernelGGL(k_shade, dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
     mark(5);
     return hipGetLastError();
 }

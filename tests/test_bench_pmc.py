@@ -19,7 +19,7 @@ def write_run(d, counter, rows):
 
 def test_pmc_summary_calibrates_and_sums(tmp_path):
     raster = "void edxr::k_raster<false>(edxr::BatchParams, edxr::FrameParams const*, edxr::DeviceBuffers)"
-    shade = "void edxr::k_shade<false>(edxr::BatchParams, edxr::FrameParams const*, edxr::DeviceBuffers)"
+    shade = "edxr::k_shade(edxr::BatchParams, edxr::FrameParams const*, edxr::DeviceBuffers)"
     neg = "void at::native::vectorized_elementwise_kernel<4, at::native::neg_kernel_cuda(...)>"
     other = "edxr::k_bin_scatter(edxr::BatchParams, edxr::FrameParams const*, edxr::DeviceBuffers)"
     known = bench.CALIBRATION_BYTES / 1024
@@ -33,4 +33,4 @@ def test_pmc_summary_calibrates_and_sums(tmp_path):
     assert s["dispatches"]["fetch"]["k_raster<false>"] == 2
     # (200 + 50) KiB x 2 + (40 + 10) KiB x 1
     assert s["traffic_bytes"] == 550 * 1024
-    assert sorted(s["kernels"]) == ["k_raster<false>", "k_shade<false>"]
+    assert sorted(s["kernels"]) == ["k_raster<false>", "k_shade"]

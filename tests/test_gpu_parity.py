@@ -171,3 +171,44 @@ def test_clip_scene_parity(built, kind):
     assert gr.shape == orr.shape
     np.testing.assert_array_equal(gr[:, :13], orr[:, :13])
     r.close()
+
+
+def wide_triangles(n=1200, width=1280, height=720, seed=23):
+    """n large, overlapping triangles in front of the camera, each over many
+    tiles: one binning chunk holds more wide records than the count and
+    scatter kernels list for their waves, so the rest are enumerated by one
+    thread each."""
+    from edxraster_amd import Mesh
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1.0, 1.0, (n, 1, 2))
+    v = c + rng.uniform(-0.9, 0.9, (n, 3, 2))
+    z = rng.uniform(-0.5, 0.5, (n, 3))
+    px, py, pz = v[..., 0].ravel(), v[..., 1].ravel(), z.ravel()
+    V = 3 * n
+    nrm = np.tile(np.array([0.0, 0.0, -1.0], np.float32), (V, 1))
+    col = rng.uniform(0.0, 1.0, (V, 3))
+    idx = np.arange(V, dtype=np.uint32)
+    mesh = Mesh(px, py, pz, nrm[:, 0], nrm[:, 1], nrm[:, 2], col[:, 0], col[:, 1], col[:, 2], idx)
+    mv, p, r = configs.camera(width, height, (0.0, 0.0, -3.0), (0.0, 0.0, 0.0))
+    return configs.Workload("wide_triangles_%d" % n, width, height, mesh, mv, p, r)
+
+
+def test_wide_triangles_bins_parity(built):
+    """Hundreds of wide records in one chunk: frame, records and bins equal
+    the oracle's, at the default chunk size and with one chunk per 64 slots."""
+    w = wide_triangles()
+    r, o, ost = render_both(w)
+    assert_frame_parity(r, o, "wide triangles")
+    goffs, grefs = r.debug_bins()
+    ooffs, orefs = o.bins()
+    np.testing.assert_array_equal(goffs, ooffs)
+    np.testing.assert_array_equal(grefs, orefs)
+    wide = (orefs[:, 2] & 16) != 0
+    assert wide.sum() > 8000, "too few tile-level references: %d" % wide.sum()
+    assert r.stats()["setup_records"] > 512  # more wide records than one step lists
+    r.set_tuning(64, 4)
+    r.render_mesh(w.mesh)
+    assert_frame_parity(r, o, "wide triangles, 64-slot chunks")
+    goffs, grefs = r.debug_bins()
+    np.testing.assert_array_equal(grefs, orefs)
+    r.close()
