@@ -215,6 +215,9 @@ struct edxr_ctx {
     uint32_t numCUs = 256;
     uint32_t clipQueueCap = 0, extraRecCap = 0, extraVertCap = 0, refCap = 0;
     uint32_t debugCapacity = 0;     // EDXR_STATE_DEBUG_CAPACITY
+    // The one-pass column scan gave up waiting once (or debug capacities make
+    // it give up at once): three-kernel column scans from then on.
+    bool legacyScans = false;
     uint8_t* scratch = nullptr;     // device address of this device's no-allocation scratch
     uint64_t renderGeneration = 0;  // advanced by every render_mesh / render_batch
     uint32_t* hostCounters = nullptr;
@@ -529,6 +532,9 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.desc = c->desc.p;
         B.chunk_box = c->chunkBox.p;
         B.group_sums = c->groupSums.p;
+        B.group_entries = pl.groupEntries;
+        B.scan_spin_limit = c->debugCapacity ? 0u : kScanSpinLimit;
+        B.legacy_scans = c->legacyScans ? 1u : 0u;
         B.clip_queue = c->clipQueue.p;
         B.clip_queue_cap = c->clipQueueCap;
         B.extra_rec_cap = c->extraRecCap;
@@ -622,6 +628,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         if (ov & kOvExtraRecords) c->extraRecCap = grow(std::max(cs.recDemand, 2 * c->extraRecCap));
         if (ov & kOvExtraVertices) c->extraVertCap = grow(std::max(cs.vertDemand, 2 * c->extraVertCap));
         if (ov & kOvRefs) c->refCap = grow(std::max(h[kCntTotalRefs], c->refCap));
+        if (ov & kOvScanWait) c->legacyScans = true;
         // A clip-queue overflow hides the clip-stage demand: grow those too.
         if (ov & kOvClipQueue) {
             c->extraRecCap = std::max(c->extraRecCap, grow(2 * c->extraRecCap));
@@ -887,7 +894,10 @@ int edxr_set_state(edxr_ctx* c, int32_t key, int32_t value) {
         case EDXR_STATE_DEBUG_CAPACITY:
             if (value < 0 || value > 65536) return c->fail(EDXR_E_INVALID, "debug capacity out of range (0..65536)");
             c->debugCapacity = (uint32_t)value;
-            if (!value) c->clipQueueCap = c->extraRecCap = c->extraVertCap = 0;  // back to the defaults
+            if (!value) {  // back to the defaults
+                c->clipQueueCap = c->extraRecCap = c->extraVertCap = 0;
+                c->legacyScans = false;
+            }
             return EDXR_OK;
         case EDXR_STATE_RASTER_BLOCKS_PER_CU:
             if (value < 0 || value > 32) return c->fail(EDXR_E_INVALID, "raster blocks per CU out of range (0..32)");
@@ -1008,6 +1018,7 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
     uint32_t refCap = (uint32_t)std::min<uint64_t>(4ull * T + 4ull * P.num_tiles + 1024, 0xFFFFFFF0ull);
     // Few raster items: each holds an 8 KB key image in the scratch.
     const uint32_t rasterBlocks = c->numCUs * 2;
+    bool legacyScans = c->legacyScans;
     hipError_t e;
     c->stats = edxr_stats{};
     for (uint32_t attempt = 0; attempt < 8; attempt++) {
@@ -1026,6 +1037,9 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
         B.desc = s.take<uint32_t>(T);
         B.chunk_box = s.take<uint32_t>(2ull * std::max<uint32_t>(P.num_chunks, 1));
         B.group_sums = s.take<uint32_t>(groupEntries);
+        B.group_entries = groupEntries;
+        B.scan_spin_limit = kScanSpinLimit;
+        B.legacy_scans = legacyScans ? 1u : 0u;
         B.clip_queue = s.take<uint32_t>(clipCap);
         B.bin_counts = s.take<uint32_t>(countEntries);
         B.refs = s.take<uint32_t>(refCap);
@@ -1090,6 +1104,7 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
         if (ov & kOvExtraRecords) recCap = grow(std::max(cs.recDemand, 2 * recCap));
         if (ov & kOvExtraVertices) vertCap = grow(std::max(cs.vertDemand, 2 * vertCap));
         if (ov & kOvRefs) refCap = grow(std::max(h[kCntTotalRefs], refCap));
+        if (ov & kOvScanWait) legacyScans = c->legacyScans = true;
     }
     return c->fail(EDXR_E_LIMIT, "capacity retries exhausted");
 }

@@ -34,6 +34,7 @@ constexpr uint32_t kImageBits = 26;
 constexpr uint32_t kImageMask = (1u << kImageBits) - 1;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
 constexpr uint32_t kTilescanStaged = 8192;   // batch tiles whose counts k_bin_tilescan stages in LDS
+constexpr uint32_t kScanSpinLimit = 1u << 20;  // k_bin_colpass look-back waits (about 30 ms) before giving up
 // Single-sample shading rebuilds a winner's screen values from its shading
 // vertices, instead of gathering them, in tiles of this many references or
 // more (about one triangle per two pixels of a 32x32 tile).
@@ -223,8 +224,12 @@ enum OverflowBit {
     kOvExtraRecords = 2,
     kOvExtraVertices = 4,
     kOvRefs = 8,
-    kOvPolygon = 16
+    kOvPolygon = 16,
+    kOvScanWait = 32     // a k_bin_colpass look-back gave up waiting: re-run with the three-kernel scan
 };
+// Overflows after which the tile lists are not valid: the scatter, raster
+// and shading kernels return at once (the frame is run again).
+constexpr uint32_t kOvNoLists = kOvRefs | kOvScanWait;
 
 struct DeviceBuffers {
     RasterRec* rrec;
@@ -239,7 +244,10 @@ struct DeviceBuffers {
     uint32_t* chunk_box;   // per chunk of the batch: tile box of its references,
                            // (min x | min y << 16, max x | max y << 16); empty: min > max
     uint32_t* bin_counts;  // num_chunks * num_tiles, scanned in place
-    uint32_t* group_sums;  // (num_chunks / kChunkGroup) * num_tiles
+    uint32_t* group_sums;  // (num_chunks / kChunkGroup) * num_tiles; k_bin_colpass's look-back status
+    uint64_t group_entries;  // entries of group_sums over all jobs
+    uint32_t scan_spin_limit;  // k_bin_colpass: waits before giving up (0: give up at once)
+    uint32_t legacy_scans;     // 1: column sums, group scan and column scan as three kernels
     uint32_t* tile_totals;
     uint32_t* tile_start;  // num_tiles + 1
     uint32_t* refs;

@@ -237,9 +237,13 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
                                                   DeviceBuffers B) {
     const FrameParams& P = jobs[job_of(Q, Q.sblock_job, blockIdx.x * kSetupSlots)];
     const MeshPtrs& M = P.mesh;
-    // k_bin_count adds the big references of every tile to zeroed counts.
+    // k_bin_count adds the big references of every tile to zeroed counts;
+    // k_bin_colpass's look-back starts from cleared status words.
     for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < Q.total_tiles; t += gridDim.x * kBlock)
         B.tile_big[t] = 0u;
+    if (!B.legacy_scans)
+        for (uint64_t i = blockIdx.x * kBlock + threadIdx.x; i < B.group_entries; i += gridDim.x * kBlock)
+            B.group_sums[i] = 0u;
     uint32_t s[kSetupSlots], i0[kSetupSlots], i1[kSetupSlots], i2[kSetupSlots];
     bool valid[kSetupSlots];
 #pragma unroll
@@ -1053,6 +1057,102 @@ __global__ __launch_bounds__(kBlock) void k_bin_colscan(BatchParams Q, const Fra
     }
 }
 
+// The three column-scan steps in one launch, by a decoupled look-back over
+// the chunk groups of each tile column.  A workgroup sums its entries per
+// tile, publishes the group sums in group_sums (value | kAggregate), adds the
+// earlier groups' published values walking back to the nearest inclusive
+// prefix, publishes its own inclusive prefix (value | kInclusive) and
+// rewrites its entries as offsets within the tile's list; the last group
+// writes the tile totals.  Each status word is written whole, so its flag and
+// value are seen together.  A wait is bounded by scan_spin_limit: past it the
+// frame is flagged (kOvScanWait) and run again with the three kernels; the
+// workgroup still publishes, so no other workgroup waits longer on it.
+namespace {
+constexpr uint32_t kAggregate = 1u << 30, kInclusive = 2u << 30, kStatusValue = (1u << 30) - 1;
+__device__ __forceinline__ uint32_t status_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_bin_colpass(BatchParams Q, const FrameParams* __restrict__ jobs,
+                                                        DeviceBuffers B) {
+    __shared__ uint32_t part[kBlock / kWave][kWave];
+    __shared__ uint32_t sExcl[kWave];
+    // Workgroups are dispatched in linear index order, x fastest, then the
+    // chunk group y: a workgroup waits only on lower indices (same job,
+    // earlier groups), which have started.
+    const uint32_t x = blockIdx.x, g = blockIdx.y;
+    const FrameParams& P = jobs[blockIdx.z];
+    if (x * kWave >= P.num_tiles) return;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
+    const uint32_t tile = x * kWave + lane;
+    const bool in = tile < P.num_tiles;
+    const uint32_t G = (P.num_chunks + kChunkGroup - 1) / kChunkGroup;
+    if (G == 0) {  // a job without triangles: its tiles are empty
+        if (g == 0 && wave == 0 && in) B.tile_totals[P.tile_base + tile] = 0u;
+        return;
+    }
+    if (g >= G) return;
+    const uint32_t c0 = min(P.num_chunks, g * kChunkGroup + wave * kWaveChunks);
+    const uint32_t c1 = min(P.num_chunks, c0 + kWaveChunks);
+    uint32_t v[kWaveChunks], waveBase, groupTotal;
+    const uint32_t stored = colgroup_load(P, B, tile, c0, c1, v);
+    colgroup_sums(v, part, waveBase, groupTotal);
+    if (wave == 0) {
+        uint32_t excl = 0;
+        if (in) {
+            uint32_t* st = B.group_sums + P.group_base + tile;
+            if (g > 0) {
+                status_store(st + (size_t)g * P.num_tiles, kAggregate | groupTotal);
+                bool failed = B.scan_spin_limit == 0;
+                uint32_t spins = 0;
+                // kLook predecessors are read at once, nearest first; the
+                // walk takes them up to the first inclusive prefix or the
+                // first one not yet published, where the next round resumes.
+                constexpr uint32_t kLook = 16;
+                for (uint32_t gp = g; gp > 0 && !failed;) {
+                    uint32_t sv[kLook];
+#pragma unroll
+                    for (uint32_t k = 0; k < kLook; k++)
+                        sv[k] = k < gp ? status_load(st + (size_t)(gp - 1 - k) * P.num_tiles) : kInclusive;
+                    uint32_t taken = 0;
+                    bool done = false;
+#pragma unroll
+                    for (uint32_t k = 0; k < kLook; k++) {
+                        if (done || taken < k || !(sv[k] >> 30)) continue;
+                        if (k < gp) excl += sv[k] & kStatusValue;
+                        taken = k + 1;
+                        done = (sv[k] & kInclusive) != 0;
+                    }
+                    if (done) break;
+                    gp -= taken;
+                    if (taken < kLook && gp > 0) {  // stopped at one not yet published
+                        if (++spins > B.scan_spin_limit) failed = true;
+                        else __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                if (failed) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvScanWait);
+            }
+            // (a prefix beyond the 30-bit status value: the three kernels count in 32 bits)
+            if ((uint64_t)excl + groupTotal > kStatusValue) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvScanWait);
+            status_store(st + (size_t)g * P.num_tiles, kInclusive | ((excl + groupTotal) & kStatusValue));
+            if (g == G - 1) B.tile_totals[P.tile_base + tile] = excl + groupTotal;
+        }
+        sExcl[lane] = excl;
+    }
+    __syncthreads();
+    if (!in) return;
+    uint32_t run = sExcl[lane] + waveBase;
+#pragma unroll
+    for (uint32_t k = 0; k < kWaveChunks; k++) {
+        if ((stored >> k) & 1u) B.bin_counts[P.count_base + (size_t)(c0 + k) * P.num_tiles + tile] = run;
+        run += v[k];
+    }
+}
+
 namespace {
 // Exclusive scans of kN values at once over the 1024 threads of a block:
 // one pair of barriers for all of them.
@@ -1242,7 +1342,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchParams Q, const Fra
     __shared__ uint32_t sBig[kUnroll * kBlock], sBigN;  // a step's non-compact descriptors
     __shared__ uint32_t sWide[kWideCap], sWideN;         // and their wide records
     __shared__ uint32_t sScan[kBlock + kBlock / kWave];
-    if (B.counters[kCntOverflow] & kOvRefs) return;
+    if (B.counters[kCntOverflow] & kOvNoLists) return;
     const FrameParams& P = jobs[job_of(Q, Q.chunk_job, blockIdx.x)];
     const uint32_t chunk = blockIdx.x - P.chunk_base;
     const uint32_t b0 = B.chunk_box[2 * blockIdx.x], b1 = B.chunk_box[2 * blockIdx.x + 1];
@@ -2034,7 +2134,7 @@ __global__ __launch_bounds__(kBlock, kMS ? 3 : 5) void k_raster(BatchParams Q, c
     __shared__ uint64_t keys[(kMS ? kPassSamples : 1) * kKeyPlane<kMS>];
     __shared__ uint16_t sOwner[kBlock / kWave][kOwnerCap];
     __shared__ uint32_t sItem;
-    if (B.counters[kCntOverflow] & kOvRefs) return;
+    if (B.counters[kCntOverflow] & kOvNoLists) return;
     const uint32_t nItems = B.counters[kCntWorkItems];
     // Single-sample batches: the empty tiles are cleared here, a share per
     // workgroup, before any item is taken (k_shade skips them).  Lane l looks
@@ -2145,7 +2245,7 @@ template <bool kTex>
 __device__ __forceinline__ void shade_body(const BatchParams& Q, const FrameParams* __restrict__ jobs,
                                            const DeviceBuffers& B) {
     constexpr uint32_t kPix = kTileSize * kTileSize;
-    if (B.counters[kCntOverflow] & kOvRefs) return;
+    if (B.counters[kCntOverflow] & kOvNoLists) return;
     const uint32_t gtile = blockIdx.x >> 1;
     const uint32_t pix = threadIdx.x + (blockIdx.x & 1u) * 512u;  // pixel of the tile
     const uint32_t start = B.tile_start[gtile];
@@ -2325,11 +2425,18 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     mark(2);
     const dim3 colGrid((Q.max_job_tiles + kWave - 1) / kWave, (Q.max_job_chunks + kChunkGroup - 1) / kChunkGroup,
                        Q.num_jobs);
-    // The group scan also writes the tile totals of jobs without triangles.
-    if (Q.total_chunks) hipLaunchKernelGGL(k_bin_colsum, colGrid, dim3(kBlock), 0, s, Q, jobs, B);
-    hipLaunchKernelGGL(k_bin_groupscan, dim3((Q.max_job_tiles + kBlock - 1) / kBlock, 1, Q.num_jobs), dim3(kBlock), 0,
-                       s, Q, jobs, B);
-    if (Q.total_chunks) hipLaunchKernelGGL(k_bin_colscan, colGrid, dim3(kBlock), 0, s, Q, jobs, B);
+    if (B.legacy_scans) {
+        // The group scan also writes the tile totals of jobs without triangles.
+        if (Q.total_chunks) hipLaunchKernelGGL(k_bin_colsum, colGrid, dim3(kBlock), 0, s, Q, jobs, B);
+        hipLaunchKernelGGL(k_bin_groupscan, dim3((Q.max_job_tiles + kBlock - 1) / kBlock, 1, Q.num_jobs),
+                           dim3(kBlock), 0, s, Q, jobs, B);
+        if (Q.total_chunks) hipLaunchKernelGGL(k_bin_colscan, colGrid, dim3(kBlock), 0, s, Q, jobs, B);
+    } else {
+        // (also writes the tile totals of jobs without triangles: a grid of
+        // one chunk group at least)
+        const dim3 passGrid(colGrid.x, max(1u, colGrid.y), colGrid.z);
+        hipLaunchKernelGGL(k_bin_colpass, passGrid, dim3(kBlock), 0, s, Q, jobs, B);
+    }
     const size_t tsLds = Q.total_tiles <= kTilescanStaged ? 2 * sizeof(uint32_t) * Q.total_tiles : 0;
     hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), tsLds, s, Q, B);
     mark(3);
