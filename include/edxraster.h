@@ -122,7 +122,10 @@ enum edxr_state_key {
      * with capacity for only n clip-queue entries, clip-created records and
      * vertices, and tile references, so that the capacity-overflow path runs
      * (flags raised on the device, buffers grown, frame re-run; edxr_stats
-     * .retries counts the re-runs).  0 (default) restores automatic sizes. */
+     * .retries counts the re-runs).  It also makes the one-pass binning
+     * column scan give up at once wherever it would wait, so that the frame
+     * is re-run with the three-kernel scan, which the context then keeps.
+     * 0 (default) restores automatic sizes and the one-pass scan. */
     EDXR_STATE_DEBUG_CAPACITY = 11
 };
 
