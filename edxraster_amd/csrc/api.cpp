@@ -1068,7 +1068,7 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
         B.depth = static_cast<float*>(depth);
         B.ids = static_cast<uint32_t*>(ids);
         B.color_f32 = nullptr;
-        // k_reset writes the job table from its argument: no upload.
+        // The first kernel writes the job table from its argument: no upload.
         if ((e = launch_frame(Q, jobs, &P, B, c->stream, nullptr)) != hipSuccess) return c->hip_fail(e, "kernel launch");
         if ((e = hipMemcpyAsync(c->hostCounters, c->counters.p, sizeof(uint32_t) * kCntSlots, hipMemcpyDeviceToHost,
                                 c->stream)) != hipSuccess)

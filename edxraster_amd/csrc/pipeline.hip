@@ -4,7 +4,9 @@
 // a plain frame is a batch of one job, a batch renders many independent jobs
 // with the same launches.  Stages (EDXRaster/Core/Renderer.cpp:100-118 order):
 //   k_vertex     one lane per vertex: SoA in, 48-byte shading vertex out
-//                (VertexProcessing, Core/Renderer.cpp:120-127).
+//                (VertexProcessing, Core/Renderer.cpp:120-127); it also clears
+//                the counters and writes a single job's table entry (k_reset
+//                does that alone for a frame without vertices).
 //   k_setup      vertex transform + outcodes + triangle setup, one lane per
 //                triangle (DefaultVertexShader::Execute Core/Shader.h:40-49,
 //                Clipper::Clip Core/Clipper.h:71-189, RasterTriangle::Setup
@@ -13,7 +15,8 @@
 //   k_clip       Sutherland-Hodgman clipping + fan setup of queued triangles
 //                (Core/Clipper.h:109-170,192-288).
 //   k_bin_count  per-chunk tile histograms (binning, Core/Renderer.cpp:161-230)
-//   k_bin_colsum / k_bin_groupscan / k_bin_colscan / k_bin_tilescan
+//   k_bin_colpass (or k_bin_colsum / k_bin_groupscan / k_bin_colscan) and
+//   k_bin_tilescan
 //                exclusive scans -> per-(tile, chunk) offsets and the raster
 //                work list of (tile, segment) items.
 //   k_bin_scatter scatter through LDS cursors: a tile's list holds the chunks'
@@ -161,10 +164,20 @@ __device__ __forceinline__ uint32_t job_of(const BatchParams& Q, const uint32_t*
 // input read coalesced, one 48-byte shading record out.  invW = 1/w of
 // DefaultVertexShader's MVP transform (Core/Shader.h:45) as the z-divide loop
 // computes it (Core/Renderer.cpp:144).
+// It also does k_reset's work, one launch fewer per frame: block 0 clears
+// the counters, and with kArgJob (a single job passed by value) every block
+// reads the job from the argument and block 0 writes it to the job table for
+// the later kernels.
 // ---------------------------------------------------------------------------
+template <bool kArgJob>
 __global__ __launch_bounds__(kBlock) void k_vertex(BatchParams Q, const FrameParams* __restrict__ jobs,
-                                                   ShadeVert* __restrict__ verts, ScreenVert* __restrict__ sverts) {
-    const FrameParams& P = jobs[job_of(Q, Q.vblock_job, blockIdx.x)];
+                                                   ShadeVert* __restrict__ verts, ScreenVert* __restrict__ sverts,
+                                                   uint32_t* counters, FrameParams* jobTable, FrameParams job0) {
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < kCntSlots; i += kBlock) counters[i] = 0;
+        if (kArgJob && threadIdx.x == 0) jobTable[0] = job0;
+    }
+    const FrameParams& P = kArgJob ? job0 : jobs[job_of(Q, Q.vblock_job, blockIdx.x)];
     const MeshPtrs& M = P.mesh;
     const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t v = slot - P.vert_base;
@@ -2409,9 +2422,14 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     };
     const FrameParams none{};
     mark(0);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, B.counters, jobs, job0 ? *job0 : none, job0 ? 1 : 0);
-    if (Q.vert_blocks)
-        hipLaunchKernelGGL(k_vertex, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts, B.sverts);
+    if (!Q.vert_blocks)
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, B.counters, jobs, job0 ? *job0 : none, job0 ? 1 : 0);
+    else if (job0)
+        hipLaunchKernelGGL(k_vertex<true>, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts, B.sverts,
+                           B.counters, jobs, *job0);
+    else
+        hipLaunchKernelGGL(k_vertex<false>, dim3(Q.vert_blocks), dim3(kBlock), 0, s, Q, jobs, B.verts, B.sverts,
+                           B.counters, jobs, none);
     if (Q.slot_blocks) {
         hipLaunchKernelGGL(k_setup, dim3((Q.slot_blocks + kSetupSlots - 1) / kSetupSlots), dim3(kBlock), 0, s, Q,
                            jobs, B);
