@@ -559,6 +559,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         B.raster_grid = c->numCUs * raster_resident_per_cu(Q.max_samples > 1);
         B.num_cus = c->numCUs;
         B.counters = c->counters.p;
+        B.host_counters = c->hostCounters;
         B.color = color;
         B.depth = depth;
         B.ids = ids;
@@ -589,9 +590,6 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
             if ((e = launch_frame(Q, c->jobsDev.p, job0, B, c->stream, prof ? c->events : nullptr)) != hipSuccess)
                 return c->hip_fail(e, "kernel launch");
         }
-        if ((e = hipMemcpyAsync(c->hostCounters, c->counters.p, sizeof(uint32_t) * kCntSlots, hipMemcpyDeviceToHost,
-                                c->stream)) != hipSuccess)
-            return c->hip_fail(e, "hipMemcpyAsync(counters)");
         if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
         const uint32_t* h = c->hostCounters;
         const uint32_t ov = h[kCntOverflow];
@@ -764,7 +762,8 @@ int edxr_create(edxr_ctx** out, uint32_t width, uint32_t height) {
     // address is looked up here, once per device, never in a render call.
     if (e == hipSuccess) e = scratch_address(&c->scratch);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hostCounters, sizeof(uint32_t) * kCntSlots, hipHostMallocDefault);
+    // coherent: the frame's last kernel writes it, the host reads it after a stream synchronisation
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hostCounters, sizeof(uint32_t) * kCntSlots, hipHostMallocCoherent);
     if (e == hipSuccess) e = c->counters.ensure(kCntSlots);
     if (e == hipSuccess) {
         int cus = 0;
@@ -1064,15 +1063,13 @@ int edxr_render_mesh_noalloc(edxr_ctx* c, const edxr_mesh* m, void* color, void*
         B.raster_grid = c->numCUs * raster_resident_per_cu(false);
         B.num_cus = c->numCUs;
         B.counters = c->counters.p;
+        B.host_counters = c->hostCounters;
         B.color = static_cast<uint32_t*>(color);
         B.depth = static_cast<float*>(depth);
         B.ids = static_cast<uint32_t*>(ids);
         B.color_f32 = nullptr;
         // The first kernel writes the job table from its argument: no upload.
         if ((e = launch_frame(Q, jobs, &P, B, c->stream, nullptr)) != hipSuccess) return c->hip_fail(e, "kernel launch");
-        if ((e = hipMemcpyAsync(c->hostCounters, c->counters.p, sizeof(uint32_t) * kCntSlots, hipMemcpyDeviceToHost,
-                                c->stream)) != hipSuccess)
-            return c->hip_fail(e, "hipMemcpyAsync(counters)");
         if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return c->hip_fail(e, "hipStreamSynchronize");
         const uint32_t* h = c->hostCounters;
         const uint32_t ov = h[kCntOverflow];

@@ -261,6 +261,7 @@ struct DeviceBuffers {
     uint32_t raster_grid;    // launched raster workgroups: the resident capacity
     uint32_t num_cus;        // compute units of the device
     uint32_t* counters;
+    uint32_t* host_counters;  // pinned, coherent host copy of counters, written by the frame's last launch
     uint32_t* color;       // every job's pixels at its pix_base (resolved)
     float* depth;          // samples planes of the job's pixels (single frame), sample-major
     uint32_t* ids;

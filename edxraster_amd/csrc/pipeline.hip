@@ -138,6 +138,15 @@ __global__ void k_reset(uint32_t* counters, FrameParams* jobs, FrameParams job0,
     if (writeJob && threadIdx.x == 0) jobs[0] = job0;
 }
 
+// The frame's last launch: the counters to the host's pinned buffer, which
+// the host reads once the stream has drained (no copy queued behind the
+// frame, whose start would wait for the frame's end and then for its own
+// launch).
+__global__ void k_publish(const uint32_t* counters, uint32_t* host) {
+    for (uint32_t i = threadIdx.x; i < kCntSlots; i += blockDim.x) host[i] = counters[i];
+    __threadfence_system();
+}
+
 // Sample offsets in 28.4 units per MSAA level, (x, y) per sample: the table of
 // FrameBuffer::MultiSampleOffsets (Core/FrameBuffer.cpp:107-191).
 __constant__ int2 c_sample_offsets[kMaxSampleLevel + 1][32] = {
@@ -2465,6 +2474,7 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
     if (Q.any_textured) hipLaunchKernelGGL(k_shade_tex, dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
     else hipLaunchKernelGGL(k_shade, dim3(2 * Q.total_tiles), dim3(512), 0, s, Q, jobs, B);
     mark(5);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(256), 0, s, B.counters, B.host_counters);
     return hipGetLastError();
 }
 
