@@ -225,6 +225,11 @@ struct edxr_ctx {
     // captured frame
     hipGraphExec_t graphExec = nullptr;
     std::vector<uint8_t> graphKey;
+    // Single-job graphs: the captured graph (kept for its first node, which
+    // takes the job by value) and the job its first node holds now.
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t graphFirst = nullptr;
+    FrameParams graphJob{};
 
     // last single frame
     bool rendered = false;
@@ -452,13 +457,18 @@ std::vector<uint8_t> launch_key(const BatchParams& Q, const DeviceBuffers& B, co
 
 void drop_graph(edxr_ctx* c) {
     if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graphExec = nullptr;
+    c->graph = nullptr;
+    c->graphFirst = nullptr;
     c->graphKey.clear();
 }
 
 // Runs a planned batch (sized, launched, checked; re-run with larger buffers
 // after a capacity overflow).  For a single job the job table is written by
-// the reset kernel, otherwise it is uploaded from pinned memory.
+// the frame's first kernel, which takes the job by value (in a graph replay,
+// by an update of that node's arguments); otherwise it is uploaded from
+// pinned memory.
 int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids, float* colorF) {
     hipError_t e;
     BatchParams& Q = pl.Q;
@@ -469,7 +479,7 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         (e = c->tileStart.ensure((size_t)Q.total_tiles + 1)) != hipSuccess ||
         (e = c->tileItem.ensure(Q.total_tiles)) != hipSuccess || (e = c->tileBig.ensure(Q.total_tiles)) != hipSuccess)
         return c->hip_fail(e, "hipMalloc(job tables)");
-    if (!single || graph) {
+    if (!single) {
         if ((e = c->jobsHost.ensure(Q.num_jobs)) != hipSuccess) return c->hip_fail(e, "hipHostMalloc(jobs)");
         memcpy(c->jobsHost.p, pl.jobs.data(), sizeof(FrameParams) * Q.num_jobs);
         if ((e = hipMemcpyAsync(c->jobsDev.p, c->jobsHost.p, sizeof(FrameParams) * Q.num_jobs,
@@ -568,21 +578,33 @@ int run_plan(edxr_ctx* c, Plan& pl, uint32_t* color, float* depth, uint32_t* ids
         const bool prof = c->profile && !graph;
         if (graph) {
             const std::vector<uint8_t> key = launch_key(Q, B, c->jobsDev.p);
+            const FrameParams* job0 = single ? &pl.jobs[0] : nullptr;
             if (!c->graphExec || key != c->graphKey) {
                 drop_graph(c);
-                hipGraph_t g = nullptr;
                 if ((e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                     return c->hip_fail(e, "hipStreamBeginCapture");
-                const hipError_t le = launch_frame(Q, c->jobsDev.p, nullptr, B, c->stream, nullptr);
-                e = hipStreamEndCapture(c->stream, &g);
+                const hipError_t le = launch_frame(Q, c->jobsDev.p, job0, B, c->stream, nullptr);
+                e = hipStreamEndCapture(c->stream, &c->graph);
                 if (le != hipSuccess) e = le;
-                if (e == hipSuccess) e = hipGraphInstantiate(&c->graphExec, g, nullptr, nullptr, 0);
-                if (g) (void)hipGraphDestroy(g);
+                if (e == hipSuccess && job0) {
+                    // the first kernel is the capture's one root
+                    size_t n = 1;
+                    hipGraphNodeType type = hipGraphNodeTypeEmpty;
+                    e = hipGraphGetRootNodes(c->graph, &c->graphFirst, &n);
+                    if (e == hipSuccess) e = hipGraphNodeGetType(c->graphFirst, &type);
+                    if (e == hipSuccess && (n != 1 || type != hipGraphNodeTypeKernel)) e = hipErrorUnknown;
+                    c->graphJob = *job0;
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0);
                 if (e != hipSuccess) {
                     drop_graph(c);
                     return c->hip_fail(e, "graph capture");
                 }
                 c->graphKey = key;
+            } else if (job0 && memcmp(job0, &c->graphJob, sizeof(FrameParams)) != 0) {
+                if ((e = set_frame_job(c->graphExec, c->graphFirst, Q, c->jobsDev.p, B, *job0)) != hipSuccess)
+                    return c->hip_fail(e, "hipGraphExecKernelNodeSetParams");
+                c->graphJob = *job0;
             }
             if ((e = hipGraphLaunch(c->graphExec, c->stream)) != hipSuccess) return c->hip_fail(e, "hipGraphLaunch");
         } else {

@@ -294,6 +294,11 @@ constexpr int kNumStages = 5;
 // (a single job, no separate upload); otherwise the table is already resident.
 hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FrameParams* job0, const DeviceBuffers& B,
                         hipStream_t s, hipEvent_t* events);
+// A graph captured from launch_frame with job0 non-null starts with the kernel
+// that takes the job by value: set_frame_job gives that node (first, the root
+// of the captured graph) another job's constants, launch layout unchanged.
+hipError_t set_frame_job(hipGraphExec_t exec, hipGraphNode_t first, const BatchParams& Q, FrameParams* jobs,
+                         const DeviceBuffers& B, const FrameParams& job0);
 // Adjoint of the shading of a rendered batch with visibility held fixed (the
 // triangle-ID buffer): d_rgb holds dLoss/d(clamped float RGB) per pixel in the
 // layout of color_f32; acc is 8 doubles of scratch per job, vacc the vertex

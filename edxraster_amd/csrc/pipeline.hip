@@ -2423,6 +2423,35 @@ hipError_t scratch_address(uint8_t** out) {
     return hipSuccess;
 }
 
+hipError_t set_frame_job(hipGraphExec_t exec, hipGraphNode_t first, const BatchParams& Q, FrameParams* jobs,
+                         const DeviceBuffers& B, const FrameParams& job0) {
+    // the arguments of launch_frame's first launch, in its order
+    BatchParams q = Q;
+    FrameParams* table = jobs;
+    ShadeVert* verts = B.verts;
+    ScreenVert* sverts = B.sverts;
+    uint32_t* counters = B.counters;
+    FrameParams job = job0;
+    int writeJob = 1;
+    hipKernelNodeParams p{};
+    p.sharedMemBytes = 0;
+    p.extra = nullptr;
+    if (Q.vert_blocks) {
+        void* args[] = {&q, &table, &verts, &sverts, &counters, &table, &job};
+        p.func = reinterpret_cast<void*>(&k_vertex<true>);
+        p.gridDim = dim3(Q.vert_blocks);
+        p.blockDim = dim3(kBlock);
+        p.kernelParams = args;
+        return hipGraphExecKernelNodeSetParams(exec, first, &p);
+    }
+    void* args[] = {&counters, &table, &job, &writeJob};
+    p.func = reinterpret_cast<void*>(&k_reset);
+    p.gridDim = dim3(1);
+    p.blockDim = dim3(64);
+    p.kernelParams = args;
+    return hipGraphExecKernelNodeSetParams(exec, first, &p);
+}
+
 hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FrameParams* job0, const DeviceBuffers& B,
                         hipStream_t s, hipEvent_t* events) {
     const size_t lds = sizeof(uint32_t) * Q.max_job_tiles;

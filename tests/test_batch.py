@@ -83,8 +83,9 @@ def test_batch_graph_replays_follow_camera_changes(built):
 
 @pytest.mark.gpu
 def test_single_job_graph_replays_follow_camera_changes(built):
-    """A single frame replayed from a captured graph takes its job (camera,
-    light) from host memory at run time: each replay equals the eager frame."""
+    """A single frame replayed from a captured graph takes each call's job
+    (camera, light; the graph's first kernel gets it by value, its arguments
+    updated between replays): each replay equals the eager frame."""
     w = configs.sphere_grid_1080p(320, 240, grid=4, segments=24)
     eager = Renderer(w.width, w.height)
     graph = Renderer(w.width, w.height)
