@@ -34,6 +34,7 @@ constexpr uint32_t kImageBits = 26;
 constexpr uint32_t kImageMask = (1u << kImageBits) - 1;
 constexpr uint32_t kChunkGroup = 64;         // chunks per block of the column scan
 constexpr uint32_t kTilescanStaged = 8192;   // batch tiles whose counts k_bin_tilescan stages in LDS
+constexpr uint32_t kTilescanWork = 8192;     // work items k_bin_tilescan stages in LDS (with the tiles)
 constexpr uint32_t kScanSpinLimit = 1u << 20;  // k_bin_colpass look-back waits (about 30 ms) before giving up
 // Single-sample shading rebuilds a winner's screen values from its shading
 // vertices, instead of gathering them, in tiles of this many references or

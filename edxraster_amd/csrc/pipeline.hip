@@ -147,6 +147,7 @@ __global__ void k_publish(const uint32_t* counters, uint32_t* host) {
     __threadfence_system();
 }
 
+
 // Sample offsets in 28.4 units per MSAA level, (x, y) per sample: the table of
 // FrameBuffer::MultiSampleOffsets (Core/FrameBuffer.cpp:107-191).
 __constant__ int2 c_sample_offsets[kMaxSampleLevel + 1][32] = {
@@ -255,8 +256,10 @@ __device__ __forceinline__ uint32_t compact_desc(const FrameParams& P, const Ras
 // Each thread takes kSetupSlots slots kBlock apart; all their index and
 // screen-vertex loads are issued before any is used, so one wave keeps
 // several dependent-load chains in flight.
-__global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FrameParams* __restrict__ jobs,
-                                                  DeviceBuffers B) {
+// Held to 64 VGPRs: eight waves per SIMD, so that a 2M-triangle frame's
+// 8,192 waves are resident at once (one round of the load chain).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_setup(
+    BatchParams Q, const FrameParams* __restrict__ jobs, DeviceBuffers B) {
     const FrameParams& P = jobs[job_of(Q, Q.sblock_job, blockIdx.x * kSetupSlots)];
     const MeshPtrs& M = P.mesh;
     // k_bin_count adds the big references of every tile to zeroed counts;
@@ -280,16 +283,26 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
             i2[u] = M.idx[3 * (size_t)t + 2];
         }
     }
-    ScreenVert sv[kSetupSlots][3];
+    // The vertices' snapped positions and outcodes; z is loaded only for the
+    // few triangles whose record is stored (a compact record is rebuilt from
+    // the screen vertices by its readers).
+    int2 xy[kSetupSlots][3];
+    uint32_t code[kSetupSlots][3];
     bool bad[kSetupSlots];
 #pragma unroll
     for (uint32_t u = 0; u < kSetupSlots; u++) {
         bad[u] = valid[u] && (i0[u] >= P.num_vertices || i1[u] >= P.num_vertices || i2[u] >= P.num_vertices);
         if (P.shader == kTexturedLambert && valid[u] && M.tex[s[u] - P.slot_base] >= P.num_tex) bad[u] = true;
         if (valid[u] && !bad[u]) {
-            sv[u][0] = B.sverts[P.vert_base + i0[u]];
-            sv[u][1] = B.sverts[P.vert_base + i1[u]];
-            sv[u][2] = B.sverts[P.vert_base + i2[u]];
+            const ScreenVert* v0 = &B.sverts[P.vert_base + i0[u]];
+            const ScreenVert* v1 = &B.sverts[P.vert_base + i1[u]];
+            const ScreenVert* v2 = &B.sverts[P.vert_base + i2[u]];
+            xy[u][0] = *reinterpret_cast<const int2*>(v0);
+            xy[u][1] = *reinterpret_cast<const int2*>(v1);
+            xy[u][2] = *reinterpret_cast<const int2*>(v2);
+            code[u][0] = v0->code;
+            code[u][1] = v1->code;
+            code[u][2] = v2->code;
         }
     }
     uint32_t nrej = 0, nbad = 0, nmain = 0;
@@ -300,16 +313,14 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
         bool rejected = false, queued = false, main = false;
         RasterRec rec;
         if (valid[u] && !bad[u]) {
-            const ScreenVert &s0 = sv[u][0], &s1 = sv[u][1], &s2 = sv[u][2];
-            if (s0.code | s1.code | s2.code) {
-                if (s0.code & s1.code & s2.code) rejected = true;
+            if (code[u][0] | code[u][1] | code[u][2]) {
+                if (code[u][0] & code[u][1] & code[u][2]) rejected = true;
                 else queued = true;
-            } else if (setup_snapped(s0.x, s0.y, s1.x, s1.y, s2.x, s2.y, rec)) {
-                // invW and the z divide (Core/Renderer.cpp:139-147).
-                rec.z0 = s0.z;
-                rec.z1 = s1.z;
-                rec.z2 = s2.z;
-                main = true;
+            } else {
+                // the cull test of RasterTriangle::Setup (setup_snapped)
+                const int2 a = xy[u][0], b = xy[u][1], c = xy[u][2];
+                main = (a.x - c.x) * (b.y - c.y) - (c.x - b.x) * (c.y - a.y) > 0;
+                rec.x0 = a.x; rec.y0 = a.y; rec.x1 = b.x; rec.y1 = b.y; rec.x2 = c.x; rec.y2 = c.y;
             }
         }
         queuedU[u] = queued;
@@ -321,6 +332,11 @@ __global__ __launch_bounds__(kBlock) void k_setup(BatchParams Q, const FramePara
                 // vertices' screen values, which the raster re-reads.
                 d = compact_desc(P, rec, s[u]);
                 if (!(d & kDescCompact)) {
+                    setup_snapped(rec.x0, rec.y0, rec.x1, rec.y1, rec.x2, rec.y2, rec);
+                    // invW and the z divide (Core/Renderer.cpp:139-147).
+                    rec.z0 = B.sverts[P.vert_base + i0[u]].z;
+                    rec.z1 = B.sverts[P.vert_base + i1[u]].z;
+                    rec.z2 = B.sverts[P.vert_base + i2[u]].z;
                     rec.order = make_order(s[u], 0, false);
                     B.rrec[s[u]] = rec;
                 }
@@ -1227,7 +1243,9 @@ __device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) 
 // largest first.  A segment's key image is at the tile's image base
 // (tile_item) plus its segment index, whatever its place in the list.
 __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuffers B) {
-    extern __shared__ uint32_t sTiles[];  // staged: [nt] totals, then [nt] big counts
+    // staged: [nt] totals, then [nt] big counts (overwritten by each tile's
+    // start and item entry), then up to kTilescanWork work items
+    extern __shared__ uint32_t sTiles[];
     __shared__ uint32_t sScan6[6][16];
     __shared__ uint32_t sMax, sTotal, sSegs, sExtra;
     const uint32_t nt = Q.total_tiles;
@@ -1324,25 +1342,45 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     pos[2] += tot[0] + tot[4] + tot[3];
     pos[1] += tot[0] + tot[4] + tot[3] + tot[2];
     const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && totalSegs <= kImageMask && segsFit;
+    // A thread's outputs are its contiguous tiles' entries: written straight to
+    // memory, each store instruction touches 64 lines.  Beyond two tiles per
+    // thread they go to LDS instead (a tile's start and item entry over its
+    // total and big count, which only this thread reads) and are copied out
+    // with coalesced stores (10M frame: scans 47 -> 44 us).
+    const bool stageOut = staged && nt > 2048;
+    const bool stageWork = stageOut && totalSegs <= kTilescanWork;
+    uint32_t* sWork = sTiles + 2 * nt;
     auto place = [&](uint32_t t, uint32_t n, uint32_t big) {
-        B.tile_start[t] = run;
+        if (stageOut) sTiles[t] = run;
+        else B.tile_start[t] = run;
         run += n;
         if (fits) {
             const uint32_t s = tile_log2(n, big, split);
             const uint32_t nf = n >> s, r = n & ((1u << s) - 1);
-            B.tile_item[t] = seg | (s << kImageBits);
+            if (stageOut) sTiles[nt + t] = seg | (s << kImageBits);
+            else B.tile_item[t] = seg | (s << kImageBits);
             // a tile of one segment is marked (kSingleSegment): its image is
             // the 32-bit winner orders
             const uint32_t single = nf + (r ? 1u : 0u) == 1 ? kSingleSegment : 0u;
-            for (uint32_t i = 0; i < nf; i++) B.work[pos[0]++] = t | (i << kTileBits) | single;
+            uint32_t* work = stageWork ? sWork : B.work;
+            for (uint32_t i = 0; i < nf; i++) work[pos[0]++] = t | (i << kTileBits) | single;
             if (r) {
                 const uint32_t c = 1 + (r >> (s - 2));
-                B.work[pos[c]++] = t | (nf << kTileBits) | single;
+                work[pos[c]++] = t | (nf << kTileBits) | single;
             }
             seg += nf + (r ? 1u : 0u);
         }
     };
     for (uint32_t t = t0; t < t1; t++) place(t, total_of(t), big_of(t));
+    if (stageOut) {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nt; t += 1024) {
+            B.tile_start[t] = sTiles[t];
+            if (fits) B.tile_item[t] = sTiles[nt + t];
+        }
+        if (fits && stageWork)
+            for (uint32_t i = threadIdx.x; i < totalSegs; i += 1024) B.work[i] = sWork[i];
+    }
     if (threadIdx.x == 0) {
         B.tile_start[nt] = total;
         B.counters[kCntTotalRefs] = total;
@@ -2387,7 +2425,7 @@ hipError_t init_kernels() {
         e = hipFuncSetAttribute((const void*)k_bin_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, maxDyn);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void*)k_bin_tilescan, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(2 * sizeof(uint32_t) * kTilescanStaged));
+                                (int)(sizeof(uint32_t) * (2 * kTilescanStaged + kTilescanWork)));
     if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&g_rasterResident[0],
                                                          reinterpret_cast<const void*>(&k_raster<false>), kBlock, 0);
@@ -2493,7 +2531,8 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
         const dim3 passGrid(colGrid.x, max(1u, colGrid.y), colGrid.z);
         hipLaunchKernelGGL(k_bin_colpass, passGrid, dim3(kBlock), 0, s, Q, jobs, B);
     }
-    const size_t tsLds = Q.total_tiles <= kTilescanStaged ? 2 * sizeof(uint32_t) * Q.total_tiles : 0;
+    const size_t tsLds =
+        Q.total_tiles <= kTilescanStaged ? sizeof(uint32_t) * (2 * (size_t)Q.total_tiles + kTilescanWork) : 0;
     hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), tsLds, s, Q, B);
     mark(3);
     if (Q.total_chunks) hipLaunchKernelGGL(k_bin_scatter, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
