@@ -3,7 +3,8 @@ sizes, resolutions and cameras in one launch sequence.
 
 GPU bar: every job of a batch equals the CPU oracle's render of that job
 alone (IDs and depth bit-exact, colour within 1 LSB), and the full 256-job
-batch equals 256 single-job renders bit for bit.  The CPU part checks the
+batch equals 256 single-job renders bit for bit and, on nine sampled jobs,
+the oracle.  The CPU part checks the
 job generator.
 """
 import numpy as np
@@ -125,9 +126,12 @@ def test_autotune_picks_a_candidate_and_keeps_results(built):
 
 @pytest.mark.gpu
 def test_full_batch_equals_single_job_renders(built):
-    """The 256-job batch (19.2M triangles) against 256 plain renders."""
+    """The 256-job batch (19.2M triangles) against 256 plain renders, and
+    nine of its jobs (every 32nd and the one with the most triangles)
+    against the oracle."""
     import torch
     from bench import device_mesh
+    from oracle.oracle import Oracle
     jobs = configs.batch_jobs()
     keep, dm = [], []
     for w in jobs:
@@ -145,6 +149,16 @@ def test_full_batch_equals_single_job_renders(built):
         assert np.array_equal(ids, single.read_triangle_ids()), w.name
         assert np.array_equal(depth.view(np.uint32), single.read_depth().view(np.uint32)), w.name
         assert np.array_equal(rgba, single.get_back_buffer()), w.name
+    o = Oracle()
+    biggest = max(range(len(jobs)), key=lambda k: jobs[k].mesh.num_triangles)
+    for k in sorted(set(range(0, len(jobs), 32)) | {biggest}):
+        w = jobs[k]
+        rgba, depth, ids = r.batch_read(k)
+        o.render_workload(w)
+        assert np.array_equal(ids, o.ids()), "%s: triangle IDs differ from the oracle" % w.name
+        assert np.array_equal(depth.view(np.uint32), o.depth().view(np.uint32)), "%s: depth differs" % w.name
+        dc = np.abs(rgba.astype(np.int16) - o.color().astype(np.int16)).max()
+        assert dc <= 1, "%s: colour differs from the oracle by %d LSB" % (w.name, dc)
     r.close()
     single.close()
     torch.cuda.synchronize()
