@@ -20,8 +20,8 @@ constexpr uint32_t kRecordMask = (1u << kRecordBits) - 1;
 constexpr int kMaxPoly = 16;                 // clip polygon capacity (convex input needs <= 9)
 constexpr uint32_t kMaxSampleLevel = 5;      // up to 32 samples per pixel
 constexpr uint32_t kPassSamples = 4;         // samples whose keys one raster pass keeps in LDS
-constexpr uint32_t kSegLog2Min = 8;          // raster work items hold 256 .. 2048 tile references,
-constexpr uint32_t kSegLog2Max = 11;         // chosen per frame from the total (k_bin_tilescan)
+constexpr uint32_t kSegLog2Min = 8;          // raster work items hold 256 .. 4096 tile references,
+constexpr uint32_t kSegLog2Max = 12;         // chosen per frame from the total (k_bin_tilescan)
 constexpr uint32_t kSegRefs = 1u << kSegLog2Max;
 // Tiles whose references are mostly tile-level triangles (each takes a whole
 // wave in the raster) are cut into segments of 2^kSegLog2Heavy references when

@@ -1234,7 +1234,7 @@ __device__ __forceinline__ uint32_t tile_segments(uint32_t n, uint32_t segLog2) 
 // Exclusive scan of the tile totals over every job's tiles (global tiles <=
 // 2^18, one block of 1024), and the raster work list: one item per (tile,
 // segment of <= 2^segLog2 refs).  The segment size is the smallest of 256 ..
-// 2048 that gives no more items than the raster grid has workgroups (beyond
+// 4096 that gives no more items than the raster grid has workgroups (beyond
 // one per tile), so a frame with few references is still spread over the
 // whole grid instead of being bound by one workgroup walking a deep tile.
 // Items are listed longest first, so that the last ones claimed are short and
