@@ -107,6 +107,40 @@ def test_single_job_graph_replays_follow_camera_changes(built):
 
 
 @pytest.mark.gpu
+def test_single_job_graph_replays_follow_mesh_changes(built):
+    """Two device meshes of the same size (same launch layout, so the graph
+    is not captured again) alternated under graph replay: the first kernel's
+    by-value job carries the mesh pointers, and each replay equals the eager
+    frame of the mesh it was given."""
+    import torch
+    from bench import device_mesh
+    from edxraster_amd.renderer import Mesh
+    w = configs.sphere_grid_1080p(320, 240, grid=4, segments=24)
+    a = w.mesh
+    b = Mesh(a.px + 0.35, a.py - 0.2, a.pz, a.nx, a.ny, a.nz, a.cb, a.cr, a.cg, a.indices)
+    da, ka = device_mesh(a)
+    db, kb = device_mesh(b)
+    eager = Renderer(w.width, w.height)
+    graph = Renderer(w.width, w.height)
+    graph.set_graph(True)
+    for r in (eager, graph):
+        w.apply(r)
+    depths = []
+    for step, m in enumerate((da, db, da, db)):
+        eager.render_mesh(m)
+        graph.render_mesh(m)
+        assert np.array_equal(eager.read_triangle_ids(), graph.read_triangle_ids()), "step %d: IDs" % step
+        assert np.array_equal(eager.read_depth(), graph.read_depth()), "step %d: depth" % step
+        assert np.array_equal(eager.get_back_buffer(), graph.get_back_buffer()), "step %d: colour" % step
+        depths.append(eager.read_depth())
+    # the two meshes give different frames, so a stale mesh would be caught
+    assert not np.array_equal(depths[0], depths[1])
+    eager.close()
+    graph.close()
+    torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
 def test_autotune_picks_a_candidate_and_keeps_results(built):
     jobs = small_batch()
     r = Renderer(64, 64)
