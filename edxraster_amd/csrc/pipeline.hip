@@ -115,13 +115,19 @@ __device__ __forceinline__ bool setup_record(const FrameParams& P, float ax, flo
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
-// Inclusive prefix sum over the wave.
+// Inclusive prefix sum over the wave, in registers: DPP row shifts by 1, 2,
+// 4 and 8 scan each row of 16 lanes (lanes shifted in from outside the row
+// read 0), then row broadcasts of lane 15 (into rows 1 and 3) and lane 31
+// (into rows 2 and 3) carry the rows' totals; six v_add_u32_dpp instead of
+// six LDS permutes.  Every lane of the wave must be active (DPP reads the
+// neighbours' registers whatever the exec mask).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = __lane_id();
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(v, d);
-        if (lane >= (uint32_t)d) v += y;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
     return v;
 }
 
@@ -533,11 +539,7 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
                     iw = make_float4(w0.x * s + w1.x * tt, w0.y * s + w1.y * tt, w0.z * s + w1.z * tt, 0.0f);
                 }
                 // Inclusive prefix of the emitted counts over the lanes.
-                int incl = emit;
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int y = __shfl_up(incl, d);
-                    if (lane >= d) incl += y;
-                }
+                const int incl = (int)wave_incl_scan((uint32_t)emit);
                 const int total = __shfl(incl, 63);
                 if (total > kMaxPoly) {
                     ok = false;
@@ -1199,14 +1201,7 @@ __device__ __forceinline__ void scan1024_n(uint32_t (&v)[kN], uint32_t (*wsum)[1
     const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
     uint32_t x[kN];
 #pragma unroll
-    for (int i = 0; i < kN; i++) x[i] = v[i];
-    for (int d = 1; d < 64; d <<= 1) {
-#pragma unroll
-        for (int i = 0; i < kN; i++) {
-            const uint32_t y = __shfl_up(x[i], d);
-            if (lane >= (uint32_t)d) x[i] += y;
-        }
-    }
+    for (int i = 0; i < kN; i++) x[i] = wave_incl_scan(v[i]);
     if (lane == kWave - 1) {
 #pragma unroll
         for (int i = 0; i < kN; i++) wsum[i][wave] = x[i];
