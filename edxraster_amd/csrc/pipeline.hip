@@ -130,6 +130,29 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
     return v;
 }
+// op over the wave's 64 lanes, the same DPP steps with lanes that have no
+// source keeping the identity; the result (lane 63's) is returned to every
+// lane.  Every lane must be active.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t v, uint32_t identity, Op op) {
+    const int id = (int)identity;
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x111, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x112, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x114, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x118, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x142, 0xa, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    return wave_reduce(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+    return wave_reduce(v, 0xFFFFFFFFu, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    return wave_reduce(v, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+}
 
 
 __device__ __forceinline__ uint32_t wave_count(bool p) { return (uint32_t)__popcll(__ballot(p)); }
@@ -540,7 +563,7 @@ __global__ __launch_bounds__(kClipWaves * 64) void k_clip(BatchParams Q, const F
                 }
                 // Inclusive prefix of the emitted counts over the lanes.
                 const int incl = (int)wave_incl_scan((uint32_t)emit);
-                const int total = __shfl(incl, 63);
+                const int total = __builtin_amdgcn_readlane(incl, 63);
                 if (total > kMaxPoly) {
                     ok = false;
                     break;
@@ -945,12 +968,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchParams Q, const Frame
         }
     }
     // The chunk's tile box (k_bin_scatter sets up cursors only inside it).
-    for (int o = 1; o < kWave; o <<= 1) {
-        bx0 = min(bx0, (uint32_t)__shfl_xor(bx0, o));
-        by0 = min(by0, (uint32_t)__shfl_xor(by0, o));
-        bx1 = max(bx1, (uint32_t)__shfl_xor(bx1, o));
-        by1 = max(by1, (uint32_t)__shfl_xor(by1, o));
-    }
+    bx0 = wave_min(bx0);
+    by0 = wave_min(by0);
+    bx1 = wave_max(bx1);
+    by1 = wave_max(by1);
     if (__lane_id() == 0) {
         atomicMin(&sBox[0], bx0);
         atomicMin(&sBox[1], by0);
@@ -1267,11 +1288,7 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
         localMax = max(localMax, n);
     }
     // Total and deepest tile: wave reductions, then one LDS atomic per wave.
-    uint32_t waveRefs = localRefs, waveMax = localMax;
-    for (int d = 1; d < 64; d <<= 1) {
-        waveRefs += __shfl_xor(waveRefs, d);
-        waveMax = max(waveMax, __shfl_xor(waveMax, d));
-    }
+    const uint32_t waveRefs = wave_sum(localRefs), waveMax = wave_max(localMax);
     if (__lane_id() == 0) {
         atomicAdd(&sTotal, waveRefs);
         atomicMax(&sMax, waveMax);
@@ -1303,10 +1320,8 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
             extra += tile_segments(n, tile_log2(n, big, true)) - plain;
         };
         for (uint32_t t = t0; t < t1; t++) add(total_of(t), big_of(t));
-        for (int d = 1; d < 64; d <<= 1) {
-            segs += __shfl_xor(segs, d);
-            extra += __shfl_xor(extra, d);
-        }
+        segs = wave_sum(segs);
+        extra = wave_sum(extra);
         if (__lane_id() == 0) {
             atomicAdd(&sSegs, segs);
             atomicAdd(&sExtra, extra);
