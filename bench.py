@@ -127,10 +127,10 @@ def calibration_probe():
 # the table; the commands and raw KiB are in profiles/pmc_raster_shade_traffic.md).
 # None where not measured.
 RASTER_SHADE_TRAFFIC = {
-    ("sphere_grid_16x16x64_1920x1080", 1): 66610038,        # tuning 1024,4
-    ("heightfield_2236_3840x2160", 1): 730481607,           # tuning 4096,8
-    ("sphere_grid_16x16x64_1920x1080", 4): 194236089,       # tuning 1024,4
-    ("heightfield_2236_textured_3840x2160", 1): 792082715,  # tuning 4096,8
+    ("sphere_grid_16x16x64_1920x1080", 1): 57721519,        # tuning 1024,4
+    ("heightfield_2236_3840x2160", 1): 710691778,           # tuning 4096,8
+    ("sphere_grid_16x16x64_1920x1080", 4): 194235727,       # tuning 1024,4
+    ("heightfield_2236_textured_3840x2160", 1): 771988753,  # tuning 4096,8
 }
 
 
