@@ -247,9 +247,11 @@ def frame_leg(w, steps, warmup, brief=False, tuning=None, msaa=0):
     return out
 
 
-def batched_leg(steps, warmup):
+def batched_leg(steps, warmup, tuning=None, plain_renders=True):
     """256 independent jobs (19.2M triangles, 256^2-1024^2): one batched,
-    graph-captured launch sequence vs 256 plain per-job renders."""
+    graph-captured launch sequence vs 256 plain per-job renders.  A given
+    tuning replaces the autotuner; plain_renders=False times the batch only
+    (profiling runs)."""
     import torch
     from edxraster_amd import configs, Renderer
     from edxraster_amd import _lib as L
@@ -261,7 +263,10 @@ def batched_leg(steps, warmup):
         keep.append(k)
     arr = (L.Job * len(jobs))(*[w.job(mesh=m) for w, m in zip(jobs, dm)])
     r = Renderer(64, 64)
-    tuning = r.autotune(arr)
+    if tuning:
+        r.set_tuning(*tuning)
+    else:
+        tuning = r.autotune(arr)
     r.set_graph(True)
     for _ in range(warmup):
         r.render_batch(arr)
@@ -273,6 +278,10 @@ def batched_leg(steps, warmup):
     batch_s = (time.perf_counter() - t0) / steps
     st = r.stats()
     r.close()
+    T = sum(w.mesh.num_triangles for w in jobs)
+    if not plain_renders:
+        return {"metric": "batched frames/s", "jobs": len(jobs), "triangles": T,
+                "frames_per_s": len(jobs) / batch_s, "batch_ms": batch_s * 1e3, "tuning": tuning}
     p = Renderer(64, 64)
 
     def plain():
@@ -289,7 +298,6 @@ def batched_leg(steps, warmup):
     torch.cuda.synchronize()
     plain_s = (time.perf_counter() - t0) / reps
     p.close()
-    T = sum(w.mesh.num_triangles for w in jobs)
     return {"metric": "batched frames/s", "jobs": len(jobs), "triangles": T,
             "pixels": int(sum(w.width * w.height for w in jobs)),
             "frames_per_s": len(jobs) / batch_s, "batch_ms": batch_s * 1e3,
@@ -397,6 +405,9 @@ def main():
                     help="TRIS_PER_CHUNK,RASTER_BLOCKS_PER_CU instead of autotuning (e.g. for profiling runs)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch the counter-calibration kernel once (for rocprofv3 --pmc runs)")
+    ap.add_argument("--leg", choices=["batched", "differentiable"], default=None,
+                    help="run only the batched (batch timed alone) or the differentiable leg and print its JSON "
+                         "(profiling runs)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print the raster/shade traffic from two rocprofv3 --pmc output folders and exit")
     args = ap.parse_args()
@@ -411,9 +422,15 @@ def main():
     import torch
     from edxraster_amd import configs, Renderer
 
+    tuning = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None
+    if args.leg == "batched":
+        print(json.dumps(batched_leg(args.steps, args.warmup, tuning, plain_renders=False)), flush=True)
+        return
+    if args.leg == "differentiable":
+        print(json.dumps(diff_leg(args.steps, args.warmup)), flush=True)
+        return
     w = configs.WORKLOADS[args.workload]()
     T, V, P = w.mesh.num_triangles, w.mesh.num_vertices, w.width * w.height
-    tuning = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None
     fr = frame_leg(w, args.steps, args.warmup, tuning=tuning, msaa=args.msaa)
     ms, stage_avg, st = fr["ms"], fr["stage_ms"], fr["stats"]
     b_frame = 36 * V + 12 * T + 12 * P

@@ -1400,6 +1400,145 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(BatchParams Q, DeviceBuff
     }
 }
 
+// k_bin_tilescan for batches beyond kTilescanStaged tiles, with the same
+// outputs (tile starts, image bases, the same work list in the same order).
+// The three reductions (references and deepest tile; segments with and
+// without the heavy tiles' finer ones; the size classes' totals) stream the
+// counts with coalesced loads.  The placement then takes the tiles in rounds
+// of kTilescanStaged, staged in LDS: each thread takes kRoundPer contiguous
+// tiles of a round, and the six prefixes (five classes, references) carry
+// from round to round.  (Walking a thread's contiguous share of all tiles
+// instead, 113 tiles per thread for 256 jobs, took 0.7 ms: every load
+// instruction touches 64 lines.)
+__global__ __launch_bounds__(1024) void k_bin_tilescan_rounds(BatchParams Q, DeviceBuffers B) {
+    constexpr uint32_t kRound = kTilescanStaged, kRoundPer = kRound / 1024;
+    extern __shared__ uint32_t sTiles[];  // [kRound] totals, then [kRound] big counts
+    __shared__ uint32_t sScan6[6][16];
+    __shared__ uint32_t sRed[4], sCls[5];  // references, deepest, segments, extra; class totals
+    const uint32_t nt = Q.total_tiles;
+    if (threadIdx.x < 4) sRed[threadIdx.x] = 0;
+    if (threadIdx.x < 5) sCls[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t refs = 0, deepest = 0;
+    for (uint32_t t = threadIdx.x; t < nt; t += 1024) {
+        const uint32_t n = B.tile_totals[t];
+        refs += n;
+        deepest = max(deepest, n);
+    }
+    refs = wave_sum(refs);
+    deepest = wave_max(deepest);
+    if (__lane_id() == 0) {
+        atomicAdd(&sRed[0], refs);
+        atomicMax(&sRed[1], deepest);
+    }
+    __syncthreads();
+    const uint32_t total = sRed[0], maxN = sRed[1];
+    uint32_t segLog2 = kSegLog2Min;
+    while (segLog2 < kSegLog2Max && (total >> segLog2) > B.raster_blocks) segLog2++;
+    while (segLog2 < kSegLog2Max && tile_segments(maxN, segLog2) > (1u << kSegIndexBits)) segLog2++;
+    const bool segsFit = tile_segments(maxN, segLog2) <= (1u << kSegIndexBits);
+    auto tile_log2 = [&](uint32_t n, uint32_t big, bool split) -> uint32_t {
+        if (!split || big < kHeavyBigMin || 4 * big < n) return segLog2;
+        uint32_t s = kSegLog2Heavy;
+        while (s < segLog2 && tile_segments(n, s) > (1u << kSegIndexBits)) s++;
+        return s;
+    };
+    {
+        uint32_t segs = 0, extra = 0;
+        for (uint32_t t = threadIdx.x; t < nt; t += 1024) {
+            const uint32_t n = B.tile_totals[t], big = B.tile_big[t];
+            const uint32_t plain = tile_segments(n, segLog2);
+            segs += plain;
+            extra += tile_segments(n, tile_log2(n, big, true)) - plain;
+        }
+        segs = wave_sum(segs);
+        extra = wave_sum(extra);
+        if (__lane_id() == 0) {
+            atomicAdd(&sRed[2], segs);
+            atomicAdd(&sRed[3], extra);
+        }
+        __syncthreads();
+    }
+    const bool split = (uint64_t)sRed[2] + sRed[3] <= B.work_cap;
+    auto count = [&](uint32_t n, uint32_t big, uint32_t (&cnt)[5]) {
+        const uint32_t s = tile_log2(n, big, split);
+        cnt[0] += n >> s;
+        const uint32_t r = n & ((1u << s) - 1);
+        if (r) cnt[1 + (r >> (s - 2))]++;
+    };
+    {
+        uint32_t cnt[5] = {0, 0, 0, 0, 0};
+        for (uint32_t t = threadIdx.x; t < nt; t += 1024) count(B.tile_totals[t], B.tile_big[t], cnt);
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            const uint32_t v = wave_sum(cnt[c]);
+            if (__lane_id() == 0 && v) atomicAdd(&sCls[c], v);
+        }
+        __syncthreads();
+    }
+    const uint32_t totalSegs = sCls[0] + sCls[1] + sCls[2] + sCls[3] + sCls[4];
+    const bool fits = total <= B.ref_cap && totalSegs <= B.work_cap && totalSegs <= kImageMask && segsFit;
+    // list order: full segments, then classes 3, 2, 1, 0
+    uint32_t pos[5] = {0, sCls[0] + sCls[4] + sCls[3] + sCls[2], sCls[0] + sCls[4] + sCls[3], sCls[0] + sCls[4],
+                       sCls[0]};
+    uint32_t run = 0, seg = 0;  // references and segments of the earlier rounds
+    for (uint32_t r0 = 0; r0 < nt; r0 += kRound) {
+        const uint32_t rn = min(kRound, nt - r0);
+        __syncthreads();  // the previous round's LDS entries are copied out
+        for (uint32_t i = threadIdx.x; i < rn; i += 1024) {
+            sTiles[i] = B.tile_totals[r0 + i];
+            sTiles[kRound + i] = B.tile_big[r0 + i];
+        }
+        __syncthreads();
+        const uint32_t a0 = min(rn, threadIdx.x * kRoundPer), a1 = min(rn, a0 + kRoundPer);
+        uint32_t cnt[5] = {0, 0, 0, 0, 0}, mine = 0;
+        for (uint32_t i = a0; i < a1; i++) {
+            mine += sTiles[i];
+            count(sTiles[i], sTiles[kRound + i], cnt);
+        }
+        uint32_t x[6] = {cnt[0], cnt[1], cnt[2], cnt[3], cnt[4], mine}, tot[6];
+        scan1024_n<6>(x, sScan6, tot);
+        uint32_t p[5], tRun = run + x[5], tSeg = seg + x[0] + x[1] + x[2] + x[3] + x[4];
+#pragma unroll
+        for (int c = 0; c < 5; c++) p[c] = pos[c] + x[c];
+        // a tile's start and item entry over its total and big count, which
+        // only this thread reads; copied out with coalesced stores below
+        for (uint32_t i = a0; i < a1; i++) {
+            const uint32_t t = r0 + i, n = sTiles[i], big = sTiles[kRound + i];
+            sTiles[i] = tRun;
+            tRun += n;
+            if (fits) {
+                const uint32_t s = tile_log2(n, big, split);
+                const uint32_t nf = n >> s, r = n & ((1u << s) - 1);
+                sTiles[kRound + i] = tSeg | (s << kImageBits);
+                const uint32_t single = nf + (r ? 1u : 0u) == 1 ? kSingleSegment : 0u;
+                for (uint32_t k = 0; k < nf; k++) B.work[p[0]++] = t | (k << kTileBits) | single;
+                if (r) {
+                    const uint32_t c = 1 + (r >> (s - 2));
+                    B.work[p[c]++] = t | (nf << kTileBits) | single;
+                }
+                tSeg += nf + (r ? 1u : 0u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < rn; i += 1024) {
+            B.tile_start[r0 + i] = sTiles[i];
+            if (fits) B.tile_item[r0 + i] = sTiles[kRound + i];
+        }
+#pragma unroll
+        for (int c = 0; c < 5; c++) pos[c] += tot[c];
+        run += tot[5];
+        seg += tot[0] + tot[1] + tot[2] + tot[3] + tot[4];
+    }
+    if (threadIdx.x == 0) {
+        B.tile_start[nt] = total;
+        B.counters[kCntTotalRefs] = total;
+        B.counters[kCntWorkItems] = totalSegs;
+        B.counters[kCntSegLog2] = segLog2;
+        if (!fits) atomicOr(&B.counters[kCntOverflow], (uint32_t)kOvRefs);
+    }
+}
+
 // Scatter: each reference goes to its tile's cursor, which starts at the
 // tile's list offset plus the counts of the earlier chunks (k_bin_colscan), so
 // a tile's list holds the chunks in submission order.  Within a chunk the
@@ -2437,6 +2576,9 @@ hipError_t init_kernels() {
         e = hipFuncSetAttribute((const void*)k_bin_tilescan, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(uint32_t) * (2 * kTilescanStaged + kTilescanWork)));
     if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_bin_tilescan_rounds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(uint32_t) * 2 * kTilescanStaged));
+    if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&g_rasterResident[0],
                                                          reinterpret_cast<const void*>(&k_raster<false>), kBlock, 0);
     if (e == hipSuccess)
@@ -2541,9 +2683,12 @@ hipError_t launch_frame(const BatchParams& Q, FrameParams* jobs, const FramePara
         const dim3 passGrid(colGrid.x, max(1u, colGrid.y), colGrid.z);
         hipLaunchKernelGGL(k_bin_colpass, passGrid, dim3(kBlock), 0, s, Q, jobs, B);
     }
-    const size_t tsLds =
-        Q.total_tiles <= kTilescanStaged ? sizeof(uint32_t) * (2 * (size_t)Q.total_tiles + kTilescanWork) : 0;
-    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), tsLds, s, Q, B);
+    if (Q.total_tiles <= kTilescanStaged)
+        hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024),
+                           sizeof(uint32_t) * (2 * (size_t)Q.total_tiles + kTilescanWork), s, Q, B);
+    else
+        hipLaunchKernelGGL(k_bin_tilescan_rounds, dim3(1), dim3(1024), sizeof(uint32_t) * 2 * kTilescanStaged, s, Q,
+                           B);
     mark(3);
     if (Q.total_chunks) hipLaunchKernelGGL(k_bin_scatter, dim3(Q.total_chunks), dim3(kBlock), lds, s, Q, jobs, B);
     mark(4);
